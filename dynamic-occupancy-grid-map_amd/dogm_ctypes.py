@@ -1,0 +1,341 @@
+"""ctypes binding of libdogm_hip.so (the C-ABI declared in include/dogm/dogm_c.h).
+
+Mirrors the reference's dogm::DOGM surface (dogm/include/dogm/dogm.h:20-199): ``Params``,
+``update_grid`` (updateGrid), the public stage methods, and the read-backs.  There is no fallback:
+if the HIP library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, fields
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("DOGM_HIP_LIB", _PKG / "lib" / "libdogm_hip.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python dynamic-occupancy-grid-map_amd/build.py` "
+        "(the GPU path has no CPU fallback)")
+
+lib = C.CDLL(str(LIB_PATH))
+
+DOGM_OK = 0
+DOGM_DEFAULT_SEED = 123456
+
+
+class dogm_params(C.Structure):
+    _fields_ = [
+        ("size", C.c_float),
+        ("resolution", C.c_float),
+        ("particle_count", C.c_int),
+        ("new_born_particle_count", C.c_int),
+        ("persistence_prob", C.c_float),
+        ("stddev_process_noise_position", C.c_float),
+        ("stddev_process_noise_velocity", C.c_float),
+        ("birth_prob", C.c_float),
+        ("stddev_velocity", C.c_float),
+        ("init_max_velocity", C.c_float),
+        ("freespace_discount", C.c_float),
+    ]
+
+
+GRID_CELL_DTYPE = np.dtype([
+    ("start_idx", "<i4"), ("end_idx", "<i4"), ("new_born_occ_mass", "<f4"), ("pers_occ_mass", "<f4"),
+    ("free_mass", "<f4"), ("occ_mass", "<f4"), ("pred_occ_mass", "<f4"), ("mu_A", "<f4"), ("mu_UA", "<f4"),
+    ("w_A", "<f4"), ("w_UA", "<f4"), ("mean_x_vel", "<f4"), ("mean_y_vel", "<f4"), ("var_x_vel", "<f4"),
+    ("var_y_vel", "<f4"), ("covar_xy_vel", "<f4"),
+])
+MEAS_CELL_DTYPE = np.dtype([("free_mass", "<f4"), ("occ_mass", "<f4"), ("likelihood", "<f4"), ("p_A", "<f4")])
+assert GRID_CELL_DTYPE.itemsize == 64 and MEAS_CELL_DTYPE.itemsize == 16
+
+_vp = C.c_void_p
+_h = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+
+_SIGS = {
+    "dogm_create": (_i, [C.POINTER(dogm_params), C.POINTER(_h)]),
+    "dogm_create_ex": (_i, [C.POINTER(dogm_params), C.c_uint64, C.c_uint32, C.POINTER(_h)]),
+    "dogm_destroy": (None, [_h]),
+    "dogm_last_error": (C.c_char_p, [_h]),
+    "dogm_status_string": (C.c_char_p, [_i]),
+    "dogm_update_grid": (_i, [_h, _vp, _f, _f, _f, _f, _i]),
+    "dogm_update_measurement_grid": (_i, [_h, _vp, _i]),
+    "dogm_update_pose": (_i, [_h, _f, _f, _f]),
+    "dogm_initialize_particles": (_i, [_h]),
+    "dogm_particle_prediction": (_i, [_h, _f]),
+    "dogm_particle_assignment": (_i, [_h]),
+    "dogm_grid_cell_occupancy_update": (_i, [_h, _f]),
+    "dogm_update_persistent_particles": (_i, [_h]),
+    "dogm_initialize_new_particles": (_i, [_h]),
+    "dogm_statistical_moments": (_i, [_h]),
+    "dogm_resampling": (_i, [_h]),
+    "dogm_get_grid_cells": (_i, [_h, _vp]),
+    "dogm_get_measurement_cells": (_i, [_h, _vp]),
+    "dogm_get_particles": (_i, [_h, _vp, _vp, _vp, _vp]),
+    "dogm_get_grid_size": (_i, [_h]),
+    "dogm_get_particle_count": (_i, [_h]),
+    "dogm_get_new_born_particle_count": (_i, [_h]),
+    "dogm_get_resolution": (_f, [_h]),
+    "dogm_get_position_x": (_f, [_h]),
+    "dogm_get_position_y": (_f, [_h]),
+    "dogm_get_yaw": (_f, [_h]),
+    "dogm_get_birth_particles": (_i, [_h, _vp, _vp, _vp, _vp]),
+    "dogm_get_weight_array": (_i, [_h, _vp]),
+    "dogm_get_born_masses": (_i, [_h, _vp]),
+    "dogm_get_resampled_indices": (_i, [_h, _vp]),
+    "dogm_set_particles": (_i, [_h, _vp, _vp, _vp, _vp]),
+    "dogm_set_birth_particles": (_i, [_h, _vp, _vp, _vp, _vp]),
+    "dogm_set_grid_cells": (_i, [_h, _vp]),
+    "dogm_set_weight_array": (_i, [_h, _vp]),
+    "dogm_set_born_masses": (_i, [_h, _vp]),
+    "dogm_device_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
+    "dogm_device_free": (_i, [_vp]),
+    "dogm_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),
+    "dogm_device_synchronize": (_i, []),
+    "dogm_kernel_count": (_i, []),
+    "dogm_kernel_name": (C.c_char_p, [_i]),
+    "dogm_set_kernel_timing": (_i, [_h, _i, _i]),
+    "dogm_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "dogm_reset_kernel_timing": (_i, [_h]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+class DogmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{lib.dogm_status_string(status).decode()}: {msg}")
+        self.status = status
+
+
+@dataclass
+class Params:
+    """dogm::DOGM::Params (dogm/include/dogm/dogm.h:26-60), same field order."""
+    size: float = 50.0
+    resolution: float = 0.2
+    particle_count: int = 300_000
+    new_born_particle_count: int = 30_000
+    persistence_prob: float = 0.99
+    stddev_process_noise_position: float = 0.1
+    stddev_process_noise_velocity: float = 2.0
+    birth_prob: float = 0.02
+    stddev_velocity: float = 30.0
+    init_max_velocity: float = 15.0
+    freespace_discount: float = 0.01
+
+    def to_c(self) -> dogm_params:
+        return dogm_params(*[getattr(self, f.name) for f in fields(self)])
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (for device-resident measurement grids)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        st = lib.dogm_device_malloc(C.byref(p), nbytes)
+        if st != DOGM_OK:
+            raise DogmError(st, f"device_malloc({nbytes})")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, host: np.ndarray) -> None:
+        assert host.nbytes <= self.nbytes and host.flags.c_contiguous
+        st = lib.dogm_memcpy(C.c_void_p(self.ptr), _ptr(host), host.nbytes, 0)
+        if st != DOGM_OK:
+            raise DogmError(st, "memcpy host->device")
+
+    def free(self) -> None:
+        if self.ptr:
+            lib.dogm_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DOGM:
+    """Python mirror of dogm::DOGM over the C-ABI."""
+
+    def __init__(self, params: Params, seed: int = DOGM_DEFAULT_SEED, instance: int = 0):
+        self.params = params
+        h = C.c_void_p()
+        self._cp = params.to_c()
+        st = lib.dogm_create_ex(C.byref(self._cp), seed, instance, C.byref(h))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_create_ex")
+        self.h = h
+        self.grid_size = lib.dogm_get_grid_size(h)
+        self.grid_cell_count = self.grid_size * self.grid_size
+        self.particle_count = lib.dogm_get_particle_count(h)
+        self.new_born_particle_count = lib.dogm_get_new_born_particle_count(h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.dogm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != DOGM_OK:
+            raise DogmError(st, f"{what}: {lib.dogm_last_error(self.h).decode()}")
+
+    # ---- per-frame update and stages -----------------------------------------------------------
+    def update_grid(self, meas, x: float, y: float, yaw: float, dt: float) -> None:
+        """meas: host structured array (MEAS_CELL_DTYPE) or a DeviceBuffer."""
+        if isinstance(meas, DeviceBuffer):
+            self._check(lib.dogm_update_grid(self.h, C.c_void_p(meas.ptr), x, y, yaw, dt, 1), "updateGrid")
+        else:
+            m = np.ascontiguousarray(meas, dtype=MEAS_CELL_DTYPE)
+            self._check(lib.dogm_update_grid(self.h, _ptr(m), x, y, yaw, dt, 0), "updateGrid")
+
+    def update_measurement_grid(self, meas) -> None:
+        if isinstance(meas, DeviceBuffer):
+            self._check(lib.dogm_update_measurement_grid(self.h, C.c_void_p(meas.ptr), 1), "updateMeasurementGrid")
+        else:
+            m = np.ascontiguousarray(meas, dtype=MEAS_CELL_DTYPE)
+            self._check(lib.dogm_update_measurement_grid(self.h, _ptr(m), 0), "updateMeasurementGrid")
+
+    def update_pose(self, x: float, y: float, yaw: float) -> None:
+        self._check(lib.dogm_update_pose(self.h, x, y, yaw), "updatePose")
+
+    def initialize_particles(self):
+        self._check(lib.dogm_initialize_particles(self.h), "initializeParticles")
+
+    def particle_prediction(self, dt: float):
+        self._check(lib.dogm_particle_prediction(self.h, dt), "particlePrediction")
+
+    def particle_assignment(self):
+        self._check(lib.dogm_particle_assignment(self.h), "particleAssignment")
+
+    def grid_cell_occupancy_update(self, dt: float):
+        self._check(lib.dogm_grid_cell_occupancy_update(self.h, dt), "gridCellOccupancyUpdate")
+
+    def update_persistent_particles(self):
+        self._check(lib.dogm_update_persistent_particles(self.h), "updatePersistentParticles")
+
+    def initialize_new_particles(self):
+        self._check(lib.dogm_initialize_new_particles(self.h), "initializeNewParticles")
+
+    def statistical_moments(self):
+        self._check(lib.dogm_statistical_moments(self.h), "statisticalMoments")
+
+    def resampling(self):
+        self._check(lib.dogm_resampling(self.h), "resampling")
+
+    # ---- read-backs ----------------------------------------------------------------------------
+    def get_grid_cells(self) -> np.ndarray:
+        out = np.empty(self.grid_cell_count, dtype=GRID_CELL_DTYPE)
+        self._check(lib.dogm_get_grid_cells(self.h, _ptr(out)), "getGridCells")
+        return out
+
+    def get_measurement_cells(self) -> np.ndarray:
+        out = np.empty(self.grid_cell_count, dtype=MEAS_CELL_DTYPE)
+        self._check(lib.dogm_get_measurement_cells(self.h, _ptr(out)), "getMeasurementCells")
+        return out
+
+    def _get_set(self, fn, n: int, what: str) -> dict:
+        state = np.empty((n, 4), np.float32)
+        idx = np.empty(n, np.int32)
+        w = np.empty(n, np.float32)
+        a = np.empty(n, np.uint8)
+        self._check(fn(self.h, _ptr(state), _ptr(idx), _ptr(w), _ptr(a)), what)
+        return {"state": state, "grid_cell_idx": idx, "weight": w, "associated": a}
+
+    def get_particles(self) -> dict:
+        return self._get_set(lib.dogm_get_particles, self.particle_count, "getParticles")
+
+    def get_birth_particles(self) -> dict:
+        return self._get_set(lib.dogm_get_birth_particles, self.new_born_particle_count, "getBirthParticles")
+
+    def set_particles(self, state=None, grid_cell_idx=None, weight=None, associated=None) -> None:
+        n = self.particle_count
+        args = _as_particle_arrays(n, state, grid_cell_idx, weight, associated)
+        self._check(lib.dogm_set_particles(self.h, *map(_ptr, args)), "setParticles")
+
+    def set_birth_particles(self, state=None, grid_cell_idx=None, weight=None, associated=None) -> None:
+        n = self.new_born_particle_count
+        args = _as_particle_arrays(n, state, grid_cell_idx, weight, associated)
+        self._check(lib.dogm_set_birth_particles(self.h, *map(_ptr, args)), "setBirthParticles")
+
+    def set_grid_cells(self, cells: np.ndarray) -> None:
+        c = np.ascontiguousarray(cells, dtype=GRID_CELL_DTYPE)
+        assert c.shape == (self.grid_cell_count,)
+        self._check(lib.dogm_set_grid_cells(self.h, _ptr(c)), "setGridCells")
+
+    def get_weight_array(self) -> np.ndarray:
+        out = np.empty(self.particle_count, np.float32)
+        self._check(lib.dogm_get_weight_array(self.h, _ptr(out)), "getWeightArray")
+        return out
+
+    def set_weight_array(self, w) -> None:
+        a = np.ascontiguousarray(w, dtype=np.float32)
+        assert a.shape == (self.particle_count,)
+        self._check(lib.dogm_set_weight_array(self.h, _ptr(a)), "setWeightArray")
+
+    def get_born_masses(self) -> np.ndarray:
+        out = np.empty(self.grid_cell_count, np.float32)
+        self._check(lib.dogm_get_born_masses(self.h, _ptr(out)), "getBornMasses")
+        return out
+
+    def set_born_masses(self, m) -> None:
+        a = np.ascontiguousarray(m, dtype=np.float32)
+        assert a.shape == (self.grid_cell_count,)
+        self._check(lib.dogm_set_born_masses(self.h, _ptr(a)), "setBornMasses")
+
+    def get_resampled_indices(self) -> np.ndarray:
+        out = np.empty(self.particle_count, np.int32)
+        self._check(lib.dogm_get_resampled_indices(self.h, _ptr(out)), "getResampledIndices")
+        return out
+
+    def position(self) -> tuple[float, float, float]:
+        return (lib.dogm_get_position_x(self.h), lib.dogm_get_position_y(self.h), lib.dogm_get_yaw(self.h))
+
+    # ---- kernel timing -------------------------------------------------------------------------
+    @staticmethod
+    def kernel_names() -> list[str]:
+        return [lib.dogm_kernel_name(i).decode() for i in range(lib.dogm_kernel_count())]
+
+    def set_kernel_timing(self, name: str, enable: bool = True) -> None:
+        self._check(lib.dogm_set_kernel_timing(self.h, self.kernel_names().index(name), int(enable)), "timing")
+
+    def kernel_timing(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._check(lib.dogm_get_kernel_timing(self.h, self.kernel_names().index(name), C.byref(ms), C.byref(n)),
+                    "timing")
+        return ms.value, n.value
+
+    def reset_kernel_timing(self) -> None:
+        self._check(lib.dogm_reset_kernel_timing(self.h), "timing")
+
+
+def _as_particle_arrays(n, state, idx, weight, assoc):
+    s = None if state is None else np.ascontiguousarray(state, dtype=np.float32).reshape(n, 4)
+    i = None if idx is None else np.ascontiguousarray(idx, dtype=np.int32).reshape(n)
+    w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32).reshape(n)
+    a = None if assoc is None else np.ascontiguousarray(assoc, dtype=np.uint8).reshape(n)
+    return s, i, w, a

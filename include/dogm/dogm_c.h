@@ -1,0 +1,188 @@
+/*
+ * C-ABI of the MI355X (gfx950) DOGM particle-filter update, exported by libdogm_hip.so.
+ *
+ * This is the drop-in boundary for the path behind dogm::DOGM::updateGrid of the reference
+ * (dogm/include/dogm/dogm.h:20-199, dogm/src/dogm.cu:115-131).  The inline C++ class in
+ * dogm/dogm.h wraps it with the reference's surface; other hosts (ctypes, cgo, JNI) bind these
+ * functions directly.  Plain pointers and sizes only: device memory lives behind the opaque handle.
+ *
+ * Conventions
+ *  - Every function returning int returns DOGM_OK (0) or a DOGM_ERR_* code; the message of the last
+ *    failure on a handle is available from dogm_last_error().  The reference only prints CUDA errors
+ *    and continues (dogm/include/dogm/cuda_utils.h:13-24).
+ *  - All calls on a handle are blocking and use the handle's own HIP stream (the reference's
+ *    updateGrid ends in cudaDeviceSynchronize, dogm/src/dogm.cu:130).  A handle is not thread-safe;
+ *    use one handle per host thread.
+ *  - Particle state is in grid cells and cells/s, as in the reference (predict.cu:27-33).
+ */
+#ifndef DOGM_C_H
+#define DOGM_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same fields, order and types as dogm::DOGM::Params (dogm/include/dogm/dogm.h:26-60). */
+typedef struct dogm_params
+{
+    float size;                          /* grid side [m] */
+    float resolution;                    /* [m/cell]; grid_size = (int)(size / resolution) */
+    int particle_count;                  /* persistent particles N */
+    int new_born_particle_count;         /* birth particles B */
+    float persistence_prob;              /* p_S */
+    float stddev_process_noise_position; /* [cells] */
+    float stddev_process_noise_velocity; /* [cells/s] */
+    float birth_prob;                    /* p_B */
+    float stddev_velocity;               /* birth velocity sigma [cells/s] */
+    float init_max_velocity;             /* first-frame velocity U(-v, v) [cells/s] */
+    float freespace_discount;            /* alpha = freespace_discount^dt */
+} dogm_params;
+
+/* 64-byte cell, same layout as dogm::GridCell (dogm/include/dogm/dogm_types.h:13-33). */
+typedef struct dogm_grid_cell
+{
+    int start_idx;
+    int end_idx;
+    float new_born_occ_mass;
+    float pers_occ_mass;
+    float free_mass;
+    float occ_mass;
+    float pred_occ_mass;
+    float mu_A;
+    float mu_UA;
+    float w_A;
+    float w_UA;
+    float mean_x_vel;
+    float mean_y_vel;
+    float var_x_vel;
+    float var_y_vel;
+    float covar_xy_vel;
+} dogm_grid_cell;
+
+/* 16-byte cell, same layout as dogm::MeasurementCell (dogm_types.h:35-41). */
+typedef struct dogm_measurement_cell
+{
+    float free_mass;
+    float occ_mass;
+    float likelihood;
+    float p_A;
+} dogm_measurement_cell;
+
+/* Particle state (x, y, vx, vy); same 16-byte layout as the reference's glm::vec4. */
+typedef struct dogm_vec4
+{
+    float x, y, z, w;
+} dogm_vec4;
+
+typedef struct dogm_handle dogm_handle;
+
+enum
+{
+    DOGM_OK = 0,
+    DOGM_ERR_INVALID_ARGUMENT = 1,
+    DOGM_ERR_OUT_OF_MEMORY = 2,
+    DOGM_ERR_HIP = 3,
+    DOGM_ERR_DEVICE_CHECK = 4, /* a kernel-side check failed (capacity or consistency) */
+    DOGM_ERR_NO_DEVICE = 5
+};
+
+/* Default seed: the reference seeds every RNG state with 123456 (dogm/src/dogm.cu:101). */
+#define DOGM_DEFAULT_SEED 123456ull
+
+/* ----------------------------------------------------------------------------------------------
+ * Construction (replaces DOGM::DOGM / ~DOGM / initialize, dogm/src/dogm.cu:33-113)
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_create(const dogm_params* params, dogm_handle** out);
+/* Extension: explicit Philox seed and instance id (the instance id selects an independent stream,
+ * so several filters can share a seed). */
+int dogm_create_ex(const dogm_params* params, uint64_t seed, uint32_t instance, dogm_handle** out);
+void dogm_destroy(dogm_handle* h);
+const char* dogm_last_error(const dogm_handle* h);
+const char* dogm_status_string(int status);
+
+/* ----------------------------------------------------------------------------------------------
+ * Per-frame update (replaces DOGM::updateGrid, dogm/include/dogm/dogm.h:82-83, dogm/src/dogm.cu:115-131)
+ * measurement_grid: grid_size^2 cells, row-major (index = x + grid_size * y), device memory when
+ * is_device != 0, host memory otherwise.  Borrowed: copied before the call returns.
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_update_grid(dogm_handle* h, const dogm_measurement_cell* measurement_grid, float new_x, float new_y,
+                     float new_yaw, float dt, int is_device);
+
+/* Private steps of updateGrid in the reference, exported so tests can drive them. */
+int dogm_update_measurement_grid(dogm_handle* h, const dogm_measurement_cell* measurement_grid,
+                                 int is_device);                                    /* dogm.cu:205-215 */
+int dogm_update_pose(dogm_handle* h, float new_x, float new_y, float new_yaw);  /* dogm.cu:161-203 */
+
+/* Public stage methods (dogm/include/dogm/dogm.h:148-156). */
+int dogm_initialize_particles(dogm_handle* h);                /* dogm.cu:217-241 */
+int dogm_particle_prediction(dogm_handle* h, float dt);       /* dogm.cu:243-260 */
+int dogm_particle_assignment(dogm_handle* h);                 /* dogm.cu:262-281 */
+int dogm_grid_cell_occupancy_update(dogm_handle* h, float dt); /* dogm.cu:283-298 */
+int dogm_update_persistent_particles(dogm_handle* h);         /* dogm.cu:300-321 */
+int dogm_initialize_new_particles(dogm_handle* h);            /* dogm.cu:323-348 */
+int dogm_statistical_moments(dogm_handle* h);                 /* dogm.cu:350-384 */
+int dogm_resampling(dogm_handle* h);                          /* dogm.cu:386-423 */
+
+/* ----------------------------------------------------------------------------------------------
+ * Read-backs (replace getGridCells / getMeasurementCells / getParticles, dogm/src/dogm.cu:133-159,
+ * and the scalar getters, dogm/include/dogm/dogm.h:90-139).  Output buffers are host memory owned by
+ * the caller and sized for grid_size^2 cells or particle_count particles.
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_get_grid_cells(dogm_handle* h, dogm_grid_cell* out);
+int dogm_get_measurement_cells(dogm_handle* h, dogm_measurement_cell* out);
+int dogm_get_particles(dogm_handle* h, dogm_vec4* state, int* grid_cell_idx, float* weight, uint8_t* associated);
+int dogm_get_grid_size(const dogm_handle* h);
+int dogm_get_particle_count(const dogm_handle* h);
+int dogm_get_new_born_particle_count(const dogm_handle* h);
+float dogm_get_resolution(const dogm_handle* h);
+float dogm_get_position_x(const dogm_handle* h);
+float dogm_get_position_y(const dogm_handle* h);
+float dogm_get_yaw(const dogm_handle* h);
+
+/* ----------------------------------------------------------------------------------------------
+ * Inspection and state injection.  The reference exposes all of its device buffers as public data
+ * members (dogm/include/dogm/dogm.h:158-191: particle_array, birth_particle_array, grid_cell_array,
+ * weight_array, born_masses_array ...); these calls replace that direct access.
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_get_birth_particles(dogm_handle* h, dogm_vec4* state, int* grid_cell_idx, float* weight,
+                             uint8_t* associated);
+int dogm_get_weight_array(dogm_handle* h, float* out);      /* N floats */
+int dogm_get_born_masses(dogm_handle* h, float* out);       /* grid_size^2 floats */
+int dogm_get_resampled_indices(dogm_handle* h, int* out);   /* N ancestor indices of the last resampling */
+int dogm_set_particles(dogm_handle* h, const dogm_vec4* state, const int* grid_cell_idx, const float* weight,
+                       const uint8_t* associated);
+int dogm_set_birth_particles(dogm_handle* h, const dogm_vec4* state, const int* grid_cell_idx,
+                             const float* weight, const uint8_t* associated);
+int dogm_set_grid_cells(dogm_handle* h, const dogm_grid_cell* cells);
+int dogm_set_weight_array(dogm_handle* h, const float* in);
+int dogm_set_born_masses(dogm_handle* h, const float* in);
+
+/* ----------------------------------------------------------------------------------------------
+ * Device-memory helpers for callers that pass device measurement grids (the reference's callers use
+ * cudaMalloc / cudaMemcpy for this, e.g. dogm/demo/simulator/mapping/laser_to_meas_grid.cu:15).
+ * kind: 0 host->device, 1 device->host, 2 device->device.
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_device_malloc(void** ptr, size_t bytes);
+int dogm_device_free(void* ptr);
+int dogm_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int dogm_device_synchronize(void);
+
+/* ----------------------------------------------------------------------------------------------
+ * Kernel timing.  When enabled, the handle records a HIP event pair around every launch of the
+ * selected kernel (on the handle's stream) and accumulates the durations.
+ * ---------------------------------------------------------------------------------------------- */
+int dogm_kernel_count(void);
+const char* dogm_kernel_name(int kernel_id);
+int dogm_set_kernel_timing(dogm_handle* h, int kernel_id, int enable);
+/* Total milliseconds and launch count accumulated for kernel_id since the last reset. */
+int dogm_get_kernel_timing(dogm_handle* h, int kernel_id, double* total_ms, long long* launches);
+int dogm_reset_kernel_timing(dogm_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DOGM_C_H */

@@ -1,0 +1,810 @@
+// CPU restatement of the reference DOGM particle-filter update (dogm::DOGM::updateGrid).
+//
+// TEST INFRASTRUCTURE ONLY.  This is the checker for the gfx950 HIP path: it is loaded by
+// tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and never by the
+// product library.  It follows the reference's stage order (dogm/src/dogm.cu:115-131) and
+// per-element formulas, with the deterministic resolutions of the reference's races and
+// float-order dependences listed at each function.  It shares only dogm_numerics.h (Philox,
+// Box-Muller, exact fixed-point sums) with the product; those primitives are pinned by their
+// own known-answer tests.
+//
+// Two modes:
+//   T = float   bit-exact twin of the GPU path (same f32 operations, same order, no contraction);
+//   T = double  the same algorithm with every per-element formula in double, used to report the
+//               error of the f32 results against an FP64 evaluation.
+#include "../include/dogm/dogm_c.h"
+#include "../dynamic-occupancy-grid-map_amd/csrc/dogm_numerics.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using namespace dogm_num;
+
+namespace
+{
+
+template <typename T>
+struct ParticleSet
+{
+    std::vector<T> x, y, vx, vy, w;
+    std::vector<int> idx;
+    std::vector<uint8_t> assoc;
+
+    void resize(int n)
+    {
+        x.assign(n, T(0));
+        y.assign(n, T(0));
+        vx.assign(n, T(0));
+        vy.assign(n, T(0));
+        w.assign(n, T(0));
+        idx.assign(n, 0);
+        assoc.assign(n, 0);
+    }
+};
+
+enum GridField
+{
+    GF_NEW_BORN,
+    GF_PERS,
+    GF_FREE,
+    GF_OCC,
+    GF_PRED,
+    GF_MU_A,
+    GF_MU_UA,
+    GF_W_A,
+    GF_W_UA,
+    GF_MEAN_X,
+    GF_MEAN_Y,
+    GF_VAR_X,
+    GF_VAR_Y,
+    GF_COVAR,
+    GF_COUNT
+};
+
+struct OracleBase
+{
+    virtual ~OracleBase() = default;
+    virtual int update_measurement_grid(const dogm_measurement_cell* meas) = 0;
+    virtual void update_pose(float x, float y, float yaw) = 0;
+    virtual void initialize_particles() = 0;
+    virtual void particle_prediction(float dt) = 0;
+    virtual void particle_assignment() = 0;
+    virtual void grid_cell_occupancy_update(float dt) = 0;
+    virtual void update_persistent_particles() = 0;
+    virtual void initialize_new_particles() = 0;
+    virtual void statistical_moments() = 0;
+    virtual void resampling() = 0;
+    virtual int get(int field, double* out) = 0;
+    virtual int set(int field, const double* in) = 0;
+    virtual int get_i32(int field, int* out) = 0;
+    virtual int set_i32(int field, const int* in) = 0;
+    float pos_x = 0.0f, pos_y = 0.0f, yaw = 0.0f;
+    bool first_pose = false, first_meas = false;
+    uint32_t epoch = 0;
+};
+
+// Field ids for oracle_get / oracle_set (double transport) and oracle_get_i32 / oracle_set_i32.
+enum Field
+{
+    F_P_X = 0,
+    F_P_Y,
+    F_P_VX,
+    F_P_VY,
+    F_P_W,
+    F_B_X,
+    F_B_Y,
+    F_B_VX,
+    F_B_VY,
+    F_B_W,
+    F_WEIGHT_ARRAY,
+    F_BORN,
+    F_GRID0 = 16,  // F_GRID0 + GridField
+    I_P_IDX = 100,
+    I_P_ASSOC,
+    I_B_IDX,
+    I_B_ASSOC,
+    I_G_START,
+    I_G_END,
+    I_ANCESTORS,
+};
+
+template <typename T>
+T to_T_from_fx(fx_t v);
+template <>
+float to_T_from_fx<float>(fx_t v)
+{
+    return fx_to_f32(v);
+}
+template <>
+double to_T_from_fx<double>(fx_t v)
+{
+    return fx_to_f64(v);
+}
+static inline fx_t fx_of(float v)
+{
+    return fx_from_f32(v);
+}
+static inline fx_t fx_of(double v)
+{
+    return fx_from_f64(v);
+}
+
+// Clamped cell coordinate: clamp(static_cast<int>(x), 0, grid_size - 1) of predict.cu:46-47 with
+// the out-of-range and NaN conversions (undefined in C++) pinned to the clamp bounds.
+template <typename T>
+static inline int clamp_cell(T x, int gs)
+{
+    if (x >= (T)(gs - 1))
+        return gs - 1;
+    if (x > (T)0)
+        return (int)x;
+    return 0;
+}
+
+// Branch-free-equivalent of CUDA's fminf as used by predict_free_mass (mass_update.cu:18).
+template <typename T>
+static inline T tmin(T a, T b)
+{
+    return std::fmin(a, b);
+}
+
+template <typename T>
+struct Oracle final : OracleBase
+{
+    dogm_params p;
+    int gs, C, N, B;
+    uint64_t seed;
+    uint32_t instance;
+
+    ParticleSet<T> part, birth;
+    std::vector<int> start, end;
+    std::vector<T> g[GF_COUNT];
+    std::vector<dogm_measurement_cell> meas;
+    std::vector<T> weight_array, born;
+    std::vector<int> ancestors;
+
+    Oracle(const dogm_params& params, uint64_t seed_, uint32_t instance_)
+        : p(params), seed(seed_), instance(instance_)
+    {
+        gs = (int)(params.size / params.resolution);  // dogm.cu:34 (truncation)
+        C = gs * gs;
+        N = params.particle_count;
+        B = params.new_born_particle_count;
+        part.resize(N);
+        birth.resize(B);
+        // initGridCellsKernel (init.cu:69-84): start = end = -1, masses 0; measurement (0, 0, 1, 1).
+        // The remaining fields are zero-initialised here (uninitialised device memory there).
+        start.assign(C, -1);
+        end.assign(C, -1);
+        for (auto& f : g)
+            f.assign(C, T(0));
+        meas.assign(C, dogm_measurement_cell{0.0f, 0.0f, 1.0f, 1.0f});
+        weight_array.assign(N, T(0));
+        born.assign(C, T(0));
+        ancestors.assign(N, 0);
+    }
+
+    // ---- dogm.cu:205-215 -----------------------------------------------------------------------
+    int update_measurement_grid(const dogm_measurement_cell* m) override
+    {
+        std::memcpy(meas.data(), m, sizeof(dogm_measurement_cell) * (size_t)C);
+        if (!first_meas)
+        {
+            initialize_particles();
+            first_meas = true;
+        }
+        return 0;
+    }
+
+    // ---- dogm.cu:161-203, ego_motion_compensation.cu:16-43 --------------------------------------
+    // First pose is stored only.  A move of more than one resolution in x or y shifts particles by
+    // the integer cell offset (x -= x_move) and the grid by new[x, y] = old[x + x_move, y + y_move]
+    // when 0 < source < grid_size (strict lower bound, ego_motion_compensation.cu:38), else zero.
+    // Yaw is stored but never applied; the sub-cell remainder is dropped.
+    void update_pose(float new_x, float new_y, float new_yaw) override
+    {
+        if (!first_pose)
+        {
+            pos_x = new_x;
+            pos_y = new_y;
+            yaw = new_yaw;
+            first_pose = true;
+            return;
+        }
+        const float x_diff = new_x - pos_x;
+        const float y_diff = new_y - pos_y;
+        if (std::fabs(x_diff) > p.resolution || std::fabs(y_diff) > p.resolution)
+        {
+            const int x_move = -static_cast<int>(x_diff / p.resolution);
+            const int y_move = -static_cast<int>(y_diff / p.resolution);
+            for (int i = 0; i < N; ++i)
+            {
+                part.x[i] -= (T)(float)x_move;
+                part.y[i] -= (T)(float)y_move;
+            }
+            std::vector<int> os = start, oe = end;
+            std::vector<T> og[GF_COUNT];
+            for (int f = 0; f < GF_COUNT; ++f)
+                og[f] = g[f];
+            for (int y = 0; y < gs; ++y)
+                for (int x = 0; x < gs; ++x)
+                {
+                    const int i = x + gs * y;
+                    const int nx = x + x_move, ny = y + y_move;
+                    const bool in = nx > 0 && nx < gs && ny > 0 && ny < gs;
+                    const int j = nx + gs * ny;
+                    start[i] = in ? os[j] : 0;
+                    end[i] = in ? oe[j] : 0;
+                    for (int f = 0; f < GF_COUNT; ++f)
+                        g[f][i] = in ? og[f][j] : T(0);
+                }
+            pos_x = new_x;
+            pos_y = new_y;
+            yaw = new_yaw;
+        }
+    }
+
+    // Exact inclusive prefix over per-cell masses, then cell j owns slots [P_{j-1}, P_j).
+    // Restates normalize_particle_orders + calc_start_idx/calc_end_idx
+    // (init_new_particles.cu:30-43, 66-74) with an exact prefix (see dogm_numerics.h).
+    void particle_orders(const std::vector<T>& masses, int v, std::vector<int>& pend)
+    {
+        pend.resize(C);
+        fx_t s = 0;
+        std::vector<fx_t> pref(C);
+        for (int j = 0; j < C; ++j)
+        {
+            s += fx_of(masses[j]);
+            pref[j] = s;
+        }
+        if (s == 0)
+        {
+            for (int j = 0; j < C; ++j)
+                pend[j] = order_end_uniform(j, C, v);
+            return;
+        }
+        const double scale = order_scale(s, v);
+        for (int j = 0; j < C; ++j)
+            pend[j] = order_end(pref[j], scale, v);
+        pend[C - 1] = v;
+    }
+
+    // ---- dogm.cu:217-241, init_new_particles.cu:76-124 -----------------------------------------
+    void initialize_particles() override
+    {
+        std::vector<T> masses(C);
+        for (int j = 0; j < C; ++j)
+            masses[j] = (T)meas[j].occ_mass;  // copyMassesKernel
+        std::vector<int> pend;
+        particle_orders(masses, N, pend);
+        born = masses;  // born_masses_array holds the measurement occupancy after this step
+        for (int j = 0, k = 0; j < C; ++j)
+            for (; k < pend[j]; ++k)
+                part.idx[k] = j;  // initParticlesKernel1
+        const T new_weight = (T)1 / (T)N;  // 1.0f / particle_count
+        const float v = p.init_max_velocity;
+        for (int k = 0; k < N; ++k)
+        {  // initParticlesKernel2
+            const int cell = part.idx[k];
+            const u32x4 r = draw(seed, instance, TAG_INIT_PARTICLES, epoch, (uint32_t)k);
+            part.x[k] = (T)(float)(cell % gs) + (T)0.5f;
+            part.y[k] = (T)(float)(cell / gs) + (T)0.5f;
+            part.vx[k] = (T)(-v) + ((T)v - (T)(-v)) * (T)uniform01(r.x);
+            part.vy[k] = (T)(-v) + ((T)v - (T)(-v)) * (T)uniform01(r.y);
+            part.w[k] = new_weight;
+            part.assoc[k] = 0;
+        }
+        ++epoch;
+    }
+
+    // ---- dogm.cu:243-260, predict.cu:16-52 ------------------------------------------------------
+    void particle_prediction(float dt_f) override
+    {
+        const T dt = (T)dt_f;
+        const T one = 1, zero = 0;
+        const T sp = (T)p.stddev_process_noise_position, sv = (T)p.stddev_process_noise_velocity;
+        const T ps = (T)p.persistence_prob;
+        for (int i = 0; i < N; ++i)
+        {
+            const u32x4 r = draw(seed, instance, TAG_PREDICT, epoch, (uint32_t)i);
+            float n0, n1, n2, n3;
+            box_muller(r.x, r.y, &n0, &n1);
+            box_muller(r.z, r.w, &n2, &n3);
+            const T npx = (T)n0 * sp + zero, npy = (T)n1 * sp + zero;
+            const T nvx = (T)n2 * sv + zero, nvy = (T)n3 * sv + zero;
+            const T x = part.x[i], y = part.y[i], vx = part.vx[i], vy = part.vy[i];
+            // transition_matrix * state with glm's mat4 * vec4 association
+            // ((c0 x + c1 y) + (c2 vx + c3 vy)), transposed matrix of dogm.cu:246-253.
+            const T sx = (one * x + zero * y) + (dt * vx + zero * vy);
+            const T sy = (zero * x + one * y) + (zero * vx + dt * vy);
+            const T svx = (zero * x + zero * y) + (one * vx + zero * vy);
+            const T svy = (zero * x + zero * y) + (zero * vx + one * vy);
+            const T nx = sx + npx, ny = sy + npy;
+            part.x[i] = nx;
+            part.y[i] = ny;
+            part.vx[i] = svx + nvx;
+            part.vy[i] = svy + nvy;
+            T w = ps * part.w[i];
+            if ((nx > (T)(gs - 1) || nx < zero) || (ny > (T)(gs - 1) || ny < zero))
+                w = zero;
+            part.w[i] = w;
+            part.idx[i] = clamp_cell(nx, gs) + gs * clamp_cell(ny, gs);
+        }
+        ++epoch;
+    }
+
+    // ---- dogm.cu:262-281, init.cu:86-93, particle_to_grid.cu:16-44 ------------------------------
+    // Stable sort by cell index (thrust::sort_by_key stability is not guaranteed by its API; the
+    // restatement defines it as stable), first/last occurrence bounds, weight_array = weight.
+    void particle_assignment() override
+    {
+        std::fill(start.begin(), start.end(), -1);
+        std::fill(end.begin(), end.end(), -1);
+        std::vector<int> perm(N);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return part.idx[a] < part.idx[b]; });
+        ParticleSet<T> s;
+        s.resize(N);
+        for (int k = 0; k < N; ++k)
+        {
+            const int o = perm[k];
+            s.x[k] = part.x[o];
+            s.y[k] = part.y[o];
+            s.vx[k] = part.vx[o];
+            s.vy[k] = part.vy[o];
+            s.w[k] = part.w[o];
+            s.idx[k] = part.idx[o];
+            s.assoc[k] = part.assoc[o];
+        }
+        part = std::move(s);
+        for (int i = 0; i < N; ++i)
+        {
+            const int c = part.idx[i];
+            if (i == 0 || part.idx[i - 1] != c)
+                start[c] = i;
+            if (i == N - 1 || part.idx[i + 1] != c)
+                end[c] = i;
+            weight_array[i] = part.w[i];
+        }
+    }
+
+    T cell_sum(const std::vector<T>& v, int s, int e) const
+    {
+        fx_t acc = 0;
+        for (int k = s; k <= e; ++k)
+            acc += fx_of(v[k]);
+        return to_T_from_fx<T>(acc);
+    }
+
+    // ---- dogm.cu:283-298, mass_update.cu:16-93 --------------------------------------------------
+    // m_occ_pred is the exact sum of the cell's weights (the reference takes an f32 prefix
+    // difference, common.h:24-32).
+    void grid_cell_occupancy_update(float dt) override
+    {
+        const T alpha = freespace_factor(dt);
+        const T one = 1, zero = 0;
+        const T pB = (T)p.birth_prob;
+        for (int i = 0; i < C; ++i)
+        {
+            const int s = start[i], e = end[i];
+            T m_occ_pred = zero;
+            if (s != -1)
+                m_occ_pred = cell_sum(weight_array, s, e);
+            if (m_occ_pred > one)
+            {  // normalize_weights (mass_update.cu:51-59)
+                for (int k = s; k < e + 1; ++k)
+                {
+                    weight_array[k] = weight_array[k] / m_occ_pred;
+                    part.w[k] = weight_array[k];
+                }
+                m_occ_pred = one;
+            }
+            const T m_free_pred = tmin(alpha * g[GF_FREE][i], one - m_occ_pred);
+            const T mfree = (T)meas[i].free_mass, mocc = (T)meas[i].occ_mass;
+            const T unknown_pred = one - m_occ_pred - m_free_pred;
+            const T meas_unknown = one - mfree - mocc;
+            const T K = m_free_pred * mocc + m_occ_pred * mfree;
+            const T occ_up = (m_occ_pred * meas_unknown + unknown_pred * mocc + m_occ_pred * mocc) / (one - K);
+            const T free_up = (m_free_pred * meas_unknown + unknown_pred * mfree + m_free_pred * mfree) / (one - K);
+            const T rho_b = (occ_up * pB * (one - m_occ_pred)) / (m_occ_pred + pB * (one - m_occ_pred));
+            const T rho_p = occ_up - rho_b;
+            born[i] = rho_b;
+            g[GF_PERS][i] = rho_p;
+            g[GF_NEW_BORN][i] = rho_b;
+            g[GF_FREE][i] = free_up;
+            g[GF_OCC][i] = occ_up;
+            g[GF_PRED][i] = m_occ_pred;
+        }
+    }
+
+    T freespace_factor(float dt) const;
+
+    // ---- dogm.cu:300-321, update_persistent_particles.cu:16-87 ----------------------------------
+    // mu_A / mu_UA are written for every cell (0 for cells without particles; the reference leaves
+    // those cells' previous values in place).
+    void update_persistent_particles() override
+    {
+        const T zero = 0, one = 1;
+        for (int i = 0; i < N; ++i)
+            weight_array[i] = (T)meas[part.idx[i]].likelihood * part.w[i];
+        for (int c = 0; c < C; ++c)
+        {
+            T mu_A = zero, mu_UA = zero;
+            if (start[c] != -1)
+            {
+                const T acc = cell_sum(weight_array, start[c], end[c]);
+                const T rho_p = g[GF_PERS][c];
+                mu_A = acc > zero ? rho_p / acc : zero;
+                const T pred = g[GF_PRED][c];
+                mu_UA = pred > zero ? g[GF_PERS][c] / pred : zero;
+            }
+            g[GF_MU_A][c] = mu_A;
+            g[GF_MU_UA][c] = mu_UA;
+        }
+        for (int i = 0; i < N; ++i)
+        {
+            const int c = part.idx[i];
+            const T pA = (T)meas[c].p_A;
+            weight_array[i] = pA * g[GF_MU_A][c] * weight_array[i] + (one - pA) * g[GF_MU_UA][c] * part.w[i];
+        }
+    }
+
+    // ---- dogm.cu:323-348, init.cu:46-67, init_new_particles.cu:18-195 ---------------------------
+    // Birth counts from the exact prefix (Sum n_j = B exactly).  Slot association: the first
+    // min(nu_A + 1, n_j) slots of a cell are associated; the reference's loop
+    // [start, start + nu_A] (init_new_particles.cu:145-148) writes one slot more than nu_A and
+    // spills into the next cell's first slot; the owner of a slot wins here.
+    void initialize_new_particles() override
+    {
+        const T zero = 0, one = 1;
+        std::vector<int> pend;
+        particle_orders(born, B, pend);
+        for (int j = 0; j < C; ++j)
+        {
+            const int s = j == 0 ? 0 : pend[j - 1];
+            const int n = pend[j] - s;
+            const float pA_f = meas[j].p_A;
+            const T pA = (T)pA_f;
+            const int nu_A = num_assoc(n, pA_f);
+            const int nu_UA = n - nu_A;
+            g[GF_W_A][j] = nu_A > 0 ? (pA * born[j]) / (T)nu_A : zero;
+            g[GF_W_UA][j] = nu_UA > 0 ? ((one - pA) * born[j]) / (T)nu_UA : zero;
+            for (int k = s; k < pend[j]; ++k)
+            {
+                birth.idx[k] = j;
+                birth.assoc[k] = (k - s) <= nu_A ? 1 : 0;
+            }
+        }
+        const T sd = (T)p.stddev_velocity;
+        for (int k = 0; k < B; ++k)
+        {
+            const int cell = birth.idx[k];
+            const u32x4 r = draw(seed, instance, TAG_BIRTH, epoch, (uint32_t)k);
+            float n0, n1;
+            box_muller(r.x, r.y, &n0, &n1);
+            birth.x[k] = (T)(float)(cell % gs) + (T)0.5f;
+            birth.y[k] = (T)(float)cell / (T)(float)gs + (T)0.5f;  // float division (init_new_particles.cu:173)
+            birth.vx[k] = (T)n0 * sd + zero;
+            birth.vy[k] = (T)n1 * sd + zero;
+            birth.w[k] = birth.assoc[k] ? g[GF_W_A][cell] : g[GF_W_UA][cell];
+        }
+        ++epoch;
+    }
+
+    int num_assoc(int n, float pA) const;
+
+    // ---- dogm.cu:350-384, statistical_moments.cu:16-106 ----------------------------------------
+    void statistical_moments() override
+    {
+        const T zero = 0, one = 1;
+        std::vector<T> m1(N), m2(N), m3(N), m4(N), m5(N);
+        for (int i = 0; i < N; ++i)
+        {
+            const T w = weight_array[i], vx = part.vx[i], vy = part.vy[i];
+            m1[i] = w * vx;
+            m2[i] = w * vy;
+            m3[i] = w * vx * vx;
+            m4[i] = w * vy * vy;
+            m5[i] = w * vx * vy;
+        }
+        for (int c = 0; c < C; ++c)
+        {
+            T mx = zero, my = zero, vx = zero, vy = zero, cv = zero;
+            const T rho_p = g[GF_PERS][c];
+            if (start[c] != -1 && rho_p > zero)
+            {
+                const int s = start[c], e = end[c];
+                const T inv = one / rho_p;
+                mx = inv * cell_sum(m1, s, e);
+                my = inv * cell_sum(m2, s, e);
+                vx = inv * cell_sum(m3, s, e) - mx * mx;
+                vy = inv * cell_sum(m4, s, e) - my * my;
+                cv = inv * cell_sum(m5, s, e) - mx * my;
+            }
+            g[GF_MEAN_X][c] = mx;
+            g[GF_MEAN_Y][c] = my;
+            g[GF_VAR_X][c] = vx;
+            g[GF_VAR_Y][c] = vy;
+            g[GF_COVAR][c] = cv;
+        }
+    }
+
+    // ---- dogm.cu:386-423, resampling.cu:17-68 ---------------------------------------------------
+    // Multinomial resampling: exact CDF over [posterior weights | birth weights]; N 64-bit
+    // uniforms mapped onto [0, total) and sorted; ancestor = first CDF entry >= draw (lower bound,
+    // resampling.cu:44-46); the next particle copies the ancestor with weight total / N.
+    void resampling() override
+    {
+        const int J = N + B;
+        std::vector<fx_t> cdf(J);
+        fx_t s = 0;
+        for (int k = 0; k < N; ++k)
+        {
+            s += fx_of(weight_array[k]);
+            cdf[k] = s;
+        }
+        for (int k = 0; k < B; ++k)
+        {
+            s += fx_of(birth.w[k]);
+            cdf[N + k] = s;
+        }
+        const fx_t total = s;
+        std::vector<uint64_t> u(N);
+        for (int k = 0; k < N; ++k)
+        {
+            const u32x4 r = draw(seed, instance, TAG_RESAMPLE, epoch, (uint32_t)k);
+            u[k] = ((uint64_t)r.x << 32) | (uint64_t)r.y;
+        }
+        std::sort(u.begin(), u.end());
+        for (int k = 0; k < N; ++k)
+        {
+            const fx_t rv = fx_scale_u64(total, u[k]);
+            // Binary search written out so it is the same search as the device's on any input.
+            int lo = 0, count = J;
+            while (count > 0)
+            {
+                const int step = count / 2;
+                const int mid = lo + step;
+                if (cdf[mid] < rv)
+                {
+                    lo = mid + 1;
+                    count -= step + 1;
+                }
+                else
+                {
+                    count = step;
+                }
+            }
+            ancestors[k] = lo < J ? lo : J - 1;  // only reachable with a negative total
+        }
+        const T joint_max = to_T_from_fx<T>(total);
+        const T new_weight = joint_max / (T)N;
+        ParticleSet<T> next;
+        next.resize(N);
+        for (int k = 0; k < N; ++k)
+        {
+            const int a = ancestors[k];
+            const ParticleSet<T>& src = a < N ? part : birth;
+            const int o = a < N ? a : a - N;
+            next.x[k] = src.x[o];
+            next.y[k] = src.y[o];
+            next.vx[k] = src.vx[o];
+            next.vy[k] = src.vy[o];
+            next.idx[k] = src.idx[o];
+            next.assoc[k] = src.assoc[o];
+            next.w[k] = new_weight;
+        }
+        part = std::move(next);
+        ++epoch;
+    }
+
+    // ---- field transport -----------------------------------------------------------------------
+    std::vector<T>* fvec(int field)
+    {
+        switch (field)
+        {
+        case F_P_X: return &part.x;
+        case F_P_Y: return &part.y;
+        case F_P_VX: return &part.vx;
+        case F_P_VY: return &part.vy;
+        case F_P_W: return &part.w;
+        case F_B_X: return &birth.x;
+        case F_B_Y: return &birth.y;
+        case F_B_VX: return &birth.vx;
+        case F_B_VY: return &birth.vy;
+        case F_B_W: return &birth.w;
+        case F_WEIGHT_ARRAY: return &weight_array;
+        case F_BORN: return &born;
+        default:
+            if (field >= F_GRID0 && field < F_GRID0 + GF_COUNT)
+                return &g[field - F_GRID0];
+            return nullptr;
+        }
+    }
+    int get(int field, double* out) override
+    {
+        auto* v = fvec(field);
+        if (!v)
+            return -1;
+        for (size_t i = 0; i < v->size(); ++i)
+            out[i] = (double)(*v)[i];
+        return (int)v->size();
+    }
+    int set(int field, const double* in) override
+    {
+        auto* v = fvec(field);
+        if (!v)
+            return -1;
+        for (size_t i = 0; i < v->size(); ++i)
+            (*v)[i] = (T)in[i];
+        return (int)v->size();
+    }
+    int get_i32(int field, int* out) override
+    {
+        switch (field)
+        {
+        case I_P_IDX: std::copy(part.idx.begin(), part.idx.end(), out); return N;
+        case I_P_ASSOC: std::copy(part.assoc.begin(), part.assoc.end(), out); return N;
+        case I_B_IDX: std::copy(birth.idx.begin(), birth.idx.end(), out); return B;
+        case I_B_ASSOC: std::copy(birth.assoc.begin(), birth.assoc.end(), out); return B;
+        case I_G_START: std::copy(start.begin(), start.end(), out); return C;
+        case I_G_END: std::copy(end.begin(), end.end(), out); return C;
+        case I_ANCESTORS: std::copy(ancestors.begin(), ancestors.end(), out); return N;
+        default: return -1;
+        }
+    }
+    int set_i32(int field, const int* in) override
+    {
+        switch (field)
+        {
+        case I_P_IDX: std::copy(in, in + N, part.idx.begin()); return N;
+        case I_P_ASSOC:
+            for (int i = 0; i < N; ++i)
+                part.assoc[i] = in[i] ? 1 : 0;
+            return N;
+        case I_B_IDX: std::copy(in, in + B, birth.idx.begin()); return B;
+        case I_B_ASSOC:
+            for (int i = 0; i < B; ++i)
+                birth.assoc[i] = in[i] ? 1 : 0;
+            return B;
+        case I_G_START: std::copy(in, in + C, start.begin()); return C;
+        case I_G_END: std::copy(in, in + C, end.begin()); return C;
+        default: return -1;
+        }
+    }
+};
+
+// alpha = freespace_discount^dt: std::pow(float, float) -> powf on the host (dogm.cu:291).
+template <>
+float Oracle<float>::freespace_factor(float dt) const
+{
+    return std::pow(p.freespace_discount, dt);
+}
+template <>
+double Oracle<double>::freespace_factor(float dt) const
+{
+    return std::pow((double)p.freespace_discount, (double)dt);
+}
+
+// nu_A = int(roundf(n * p_A)) (init_new_particles.cu:45-48).
+template <>
+int Oracle<float>::num_assoc(int n, float pA) const
+{
+    return (int)roundf((float)n * pA);
+}
+template <>
+int Oracle<double>::num_assoc(int n, float pA) const
+{
+    return (int)std::round((double)n * (double)pA);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// C ABI used by the tests (ctypes) and the host-core timing leg of bench.py.
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+void* oracle_create(const dogm_params* params, uint64_t seed, uint32_t instance, int fp64)
+{
+    if (!params)
+        return nullptr;
+    if (fp64)
+        return static_cast<OracleBase*>(new Oracle<double>(*params, seed, instance));
+    return static_cast<OracleBase*>(new Oracle<float>(*params, seed, instance));
+}
+
+void oracle_destroy(void* h)
+{
+    delete static_cast<OracleBase*>(h);
+}
+
+#define O static_cast<OracleBase*>(h)
+int oracle_update_measurement_grid(void* h, const dogm_measurement_cell* m) { return O->update_measurement_grid(m); }
+void oracle_update_pose(void* h, float x, float y, float yaw) { O->update_pose(x, y, yaw); }
+void oracle_initialize_particles(void* h) { O->initialize_particles(); }
+void oracle_particle_prediction(void* h, float dt) { O->particle_prediction(dt); }
+void oracle_particle_assignment(void* h) { O->particle_assignment(); }
+void oracle_grid_cell_occupancy_update(void* h, float dt) { O->grid_cell_occupancy_update(dt); }
+void oracle_update_persistent_particles(void* h) { O->update_persistent_particles(); }
+void oracle_initialize_new_particles(void* h) { O->initialize_new_particles(); }
+void oracle_statistical_moments(void* h) { O->statistical_moments(); }
+void oracle_resampling(void* h) { O->resampling(); }
+
+// dogm.cu:115-131 stage order.
+int oracle_update_grid(void* h, const dogm_measurement_cell* m, float x, float y, float yaw, float dt)
+{
+    O->update_measurement_grid(m);
+    O->update_pose(x, y, yaw);
+    O->particle_prediction(dt);
+    O->particle_assignment();
+    O->grid_cell_occupancy_update(dt);
+    O->update_persistent_particles();
+    O->initialize_new_particles();
+    O->statistical_moments();
+    O->resampling();
+    return 0;
+}
+
+int oracle_get(void* h, int field, double* out) { return O->get(field, out); }
+int oracle_set(void* h, int field, const double* in) { return O->set(field, in); }
+int oracle_get_i32(void* h, int field, int* out) { return O->get_i32(field, out); }
+int oracle_set_i32(void* h, int field, const int* in) { return O->set_i32(field, in); }
+uint32_t oracle_get_epoch(void* h) { return O->epoch; }
+void oracle_set_epoch(void* h, uint32_t e) { O->epoch = e; }
+float oracle_position_x(void* h) { return O->pos_x; }
+float oracle_position_y(void* h) { return O->pos_y; }
+float oracle_yaw(void* h) { return O->yaw; }
+void oracle_set_flags(void* h, int first_pose, int first_meas)
+{
+    O->first_pose = first_pose != 0;
+    O->first_meas = first_meas != 0;
+}
+#undef O
+
+// Numerics entry points for the known-answer tests of dogm_numerics.h.
+void oracle_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t* out)
+{
+    const u32x4 r = philox4x32_10(u32x4{c0, c1, c2, c3}, k0, k1);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+float oracle_log_f32(float x) { return log_f32(x); }
+void oracle_sincos_2pi(float t, float* s, float* c) { sincos_2pi_f32(t, s, c); }
+void oracle_box_muller(uint32_t a, uint32_t b, float* z0, float* z1) { box_muller(a, b, z0, z1); }
+float oracle_uniform01(uint32_t x) { return uniform01(x); }
+float oracle_uniform01_oc(uint32_t x) { return uniform01_oc(x); }
+// Fixed point: raw values travel as (hi, lo) 64-bit halves.
+void oracle_fx_from_f32(float v, int64_t* hi, uint64_t* lo)
+{
+    const fx_t r = fx_from_f32(v);
+    *hi = (int64_t)(r >> 64);
+    *lo = (uint64_t)r;
+}
+void oracle_fx_from_f64(double v, int64_t* hi, uint64_t* lo)
+{
+    const fx_t r = fx_from_f64(v);
+    *hi = (int64_t)(r >> 64);
+    *lo = (uint64_t)r;
+}
+float oracle_fx_to_f32(int64_t hi, uint64_t lo) { return fx_to_f32((fx_t)(((ufx_t)(uint64_t)hi << 64) | lo)); }
+double oracle_fx_to_f64(int64_t hi, uint64_t lo) { return fx_to_f64((fx_t)(((ufx_t)(uint64_t)hi << 64) | lo)); }
+void oracle_fx_scale_u64(int64_t hi, uint64_t lo, uint64_t u, int64_t* ohi, uint64_t* olo)
+{
+    const fx_t r = fx_scale_u64((fx_t)(((ufx_t)(uint64_t)hi << 64) | lo), u);
+    *ohi = (int64_t)(r >> 64);
+    *olo = (uint64_t)r;
+}
+int oracle_order_end(int64_t hi, uint64_t lo, int64_t thi, uint64_t tlo, int v)
+{
+    const fx_t pref = (fx_t)(((ufx_t)(uint64_t)hi << 64) | lo);
+    const fx_t tot = (fx_t)(((ufx_t)(uint64_t)thi << 64) | tlo);
+    return order_end(pref, order_scale(tot, v), v);
+}
+
+}  // extern "C"
