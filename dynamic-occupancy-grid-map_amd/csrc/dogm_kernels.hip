@@ -564,11 +564,16 @@ __global__ __launch_bounds__(BLOCK) void k_birth_particles(DevParticles Bp, int 
 
 void run_births(dogm_handle* h)
 {
-    if (h->B <= 0)
-        return;
+    // With B = 0 the cell pass still runs (every cell gets n_j = 0 and w_A = w_UA = 0) and the
+    // stage still consumes its RNG epoch, so the stream of later stages does not depend on B.
     SrcArray src{h->born};
     SinkBirth sink{h->pend, h->scalars, h->meas, h->born, h->grid.f[GF_W_A], h->grid.f[GF_W_UA], h->C, h->B};
     run_exact_scan(h, src, sink, h->C, SCAN_ORDERS, h->B, K_BIRTH_CELLS);
+    if (h->B <= 0)
+    {
+        ++h->epoch;
+        return;
+    }
     launch_begin(h, K_BIRTH_PARTICLES);
     hipLaunchKernelGGL(k_birth_particles, dim3(div_up(h->B, BLOCK)), dim3(BLOCK), 0, h->stream, h->birth, h->B,
                        h->pend, h->C, h->gs, h->meas, h->grid.f[GF_W_A], h->grid.f[GF_W_UA],

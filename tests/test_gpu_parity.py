@@ -112,7 +112,8 @@ def test_single_cell_spanning_many_tiles():
         orc.update_grid(m, 0.0, 0.0, 0.0, 0.1)
         assert_state_equal(f"frame {f}", gpu, orc, ancestors=True)
     cells = gpu.get_grid_cells()
-    assert cells["start_idx"][8 + 16 * 8] == 0
+    c = 8 + 16 * 8
+    assert cells["end_idx"][c] - cells["start_idx"][c] + 1 > p.particle_count // 2
 
 
 def test_zero_measurement_and_no_births():
