@@ -17,8 +17,8 @@ const char* const kKernelNames[K_COUNT] = {
     "seg_occ_fix",    "mass_update",     "normalize",       "seg_pers",        "seg_pers_fix",
     "post_weights",   "empty_cells",     "scan_tiles",      "scan_top",        "birth_cells",
     "birth_particles", "init_cells",     "init_particles",  "copy_masses",     "seg_moments",
-    "seg_moments_fix", "cdf_write",      "rs_count",        "rs_offsets",      "rs_scatter",
-    "rs_bucket",      "rs_gather",       "fill",            "pack_grid",       "unpack_grid",
+    "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
+    "unpack_grid",
 };
 
 static hipEvent_t take_event(dogm_handle* h)
@@ -117,9 +117,9 @@ int finish(dogm_handle* h, const char* stage)
     {
         const int zero = 0;
         HIP_TRY(h, hipMemcpy(&h->scalars->err, &zero, sizeof(int), hipMemcpyHostToDevice));
-        if (err & DEV_ERR_BUCKET_CAPACITY)
+        if (err & DEV_ERR_SPIN_LIMIT)
             return fail(h, DOGM_ERR_DEVICE_CHECK,
-                        std::string(stage) + ": a resampling bucket exceeded its LDS capacity");
+                        std::string(stage) + ": a workgroup gave up waiting for its predecessor");
         return fail(h, DOGM_ERR_DEVICE_CHECK, std::string(stage) + ": device check failed");
     }
     return DOGM_OK;
@@ -187,8 +187,7 @@ void release(dogm_handle* h)
                     h->m_raw,          h->pend,          h->ancestors,     h->sort_keys[0], h->sort_keys[1],
                     h->sort_vals[0],   h->sort_vals[1],  h->sort_ghist,    h->sort_thist,   h->seg.flags,
                     h->seg.head_partial, h->seg.tail_partial, h->scan_tile_sum, h->scan_tile_pre, h->cdf,
-                    h->rs_cnt,         h->rs_off,        h->rs_fill,       h->rs_vals,      h->scalars,
-                    h->aos_tmp};
+                    h->scalars,        h->aos_tmp,       h->cdf_tile_incl};
     for (void* b : bufs)
         (void)hipFree(b);
     for (auto& t : h->timed_pending)
@@ -420,14 +419,8 @@ int dogm_create_ex(const dogm_params* params, uint64_t seed, uint32_t instance, 
 
     h->sort_passes = std::max(1, (bits_for(C - 1) + 7) / 8);
     h->sort_tiles = div_up(N, 256 * 8);
-    h->seg_tiles_max = div_up(N, 256 * 2);
-    h->scan_tiles_max = div_up(std::max((long long)C, (long long)N + B), 1024);
-    h->rs_bits = 0;
-    while ((N >> h->rs_bits) > 256)
-        ++h->rs_bits;
-    if (((long long)1 << h->rs_bits) * 256 < N)
-        ++h->rs_bits;
-    h->rs_buckets = 1 << h->rs_bits;
+    h->seg_tiles_max = div_up(N, 256 * 4);
+    h->scan_tiles_max = div_up(std::max((long long)C, (long long)N + B + 1), 1024);
 
     if ((st = alloc_particles(h, h->part[0], N)) || (st = alloc_particles(h, h->part[1], N)) ||
         (st = alloc_particles(h, h->birth, B)))
@@ -450,10 +443,8 @@ int dogm_create_ex(const dogm_params* params, uint64_t seed, uint32_t instance, 
         (st = dalloc(h, &h->seg.tail_partial, (size_t)h->seg_tiles_max * 5, "segments")) ||
         (st = dalloc(h, &h->scan_tile_sum, h->scan_tiles_max, "scan")) ||
         (st = dalloc(h, &h->scan_tile_pre, h->scan_tiles_max, "scan")) ||
-        (st = dalloc(h, &h->cdf, (size_t)N + B, "cdf")) || (st = dalloc(h, &h->rs_cnt, h->rs_buckets, "resampling")) ||
-        (st = dalloc(h, &h->rs_off, h->rs_buckets, "resampling")) ||
-        (st = dalloc(h, &h->rs_fill, h->rs_buckets, "resampling")) ||
-        (st = dalloc(h, &h->rs_vals, N, "resampling")) || (st = dalloc(h, &h->scalars, 1, "scalars")) ||
+        (st = dalloc(h, &h->cdf, (size_t)N + B, "cdf")) || (st = dalloc(h, &h->scalars, 1, "scalars")) ||
+        (st = dalloc(h, &h->cdf_tile_incl, h->scan_tiles_max, "cdf")) ||
         (st = dalloc(h, &h->aos_tmp, C, "grid staging")))
         return bail(st);
 

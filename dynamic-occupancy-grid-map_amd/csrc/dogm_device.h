@@ -60,19 +60,135 @@ __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Vector access to a thread's ITEMS consecutive elements (16-byte loads and stores when the run
+// is complete; ITEMS is a multiple of 4 and runs start at multiples of ITEMS).
+// ---------------------------------------------------------------------------------------------
+template <int IT>
+__device__ __forceinline__ void load_run(const float* __restrict__ a, int p0, int cnt, float (&v)[IT])
+{
+    if (cnt == IT)
+    {
+#pragma unroll
+        for (int i = 0; i < IT; i += 4)
+        {
+            const float4 q = *reinterpret_cast<const float4*>(a + p0 + i);
+            v[i] = q.x;
+            v[i + 1] = q.y;
+            v[i + 2] = q.z;
+            v[i + 3] = q.w;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            v[i] = i < cnt ? a[p0 + i] : 0.0f;
+    }
+}
+
+template <int IT>
+__device__ __forceinline__ void load_run(const int* __restrict__ a, int p0, int cnt, int (&v)[IT], int fill)
+{
+    if (cnt == IT)
+    {
+#pragma unroll
+        for (int i = 0; i < IT; i += 4)
+        {
+            const int4 q = *reinterpret_cast<const int4*>(a + p0 + i);
+            v[i] = q.x;
+            v[i + 1] = q.y;
+            v[i + 2] = q.z;
+            v[i + 3] = q.w;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            v[i] = i < cnt ? a[p0 + i] : fill;
+    }
+}
+
+template <int IT, class T>
+__device__ __forceinline__ void store_run(T* __restrict__ a, int p0, int cnt, const T (&v)[IT])
+{
+    static_assert(sizeof(T) == 4, "32-bit elements");
+    if (cnt == IT)
+    {
+#pragma unroll
+        for (int i = 0; i < IT; i += 4)
+        {
+            float4 q;
+            __builtin_memcpy(&q, &v[i], 16);
+            *reinterpret_cast<float4*>(a + p0 + i) = q;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            if (i < cnt)
+                a[p0 + i] = v[i];
+    }
+}
+
+// Exclusive block scan of NCH fx values per thread (BLOCK threads).
+template <int NCH>
+__device__ __forceinline__ void block_exclusive_scan_fx_n(const fx_t (&v)[NCH], fx_t (&ex)[NCH])
+{
+    __shared__ fx_t s_wave[BLOCK / WAVE][NCH];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    fx_t incl[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        incl[c] = v[c];
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1)
+    {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+        {
+            const fx_t u = shfl_up_fx(incl[c], d);
+            if (lane >= d)
+                incl[c] += u;
+        }
+    }
+    if (lane == WAVE - 1)
+    {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            s_wave[wid][c] = incl[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+    {
+        fx_t carry = 0;
+        for (int w = 0; w < wid; ++w)
+            carry += s_wave[w][c];
+        ex[c] = carry + incl[c] - v[c];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------
 // Exact tile-segmented reduction over a key-sorted array.
 //
-// Segments are runs of equal keys (cells).  A workgroup reduces a tile of BLOCK * ITEMS
-// consecutive elements: every segment that starts and ends inside the tile is finalised there;
-// a segment that crosses tile boundaries leaves a head partial (its piece in the tile where it
-// ends) and tail partials (its pieces in the tiles it continues through), and the fix-up kernel
-// sums those pieces from the tile where the segment starts.  All sums are 128-bit integers, so
-// the split changes nothing in the result.
+// Segments are runs of equal keys (cells).  A workgroup takes a tile of BLOCK * ITEMS consecutive
+// elements (ITEMS consecutive per thread) and computes the tile-relative inclusive prefix P of
+// every element with one plain exact block scan; the sum of a segment piece [s, e] is then
+// P[e] - P[s - 1].  Integer arithmetic makes the subtraction exact.  The start of the piece that
+// reaches a thread's first element is found with one ballot (last earlier lane of the wave with a
+// segment start) and a per-wave table for earlier waves.  Segments that start and end inside the
+// tile are finalised here; a segment that crosses tile boundaries leaves a head partial (its piece
+// in the tile where it ends) and tail partials (its pieces in the tiles it continues through),
+// which the fix-up kernel sums from the tile where the segment starts.
 //
 // Op interface:
-//   static constexpr int NCH, ITEMS;
-//   __device__ void load(int pos, int key, float (&v)[NCH]) const;   // values of element pos
-//   __device__ void finalize(int key, const fx_t (&s)[NCH]) const;   // exact sums of a segment
+//   static constexpr int NCH, ITEMS;                                    // ITEMS % 4 == 0
+//   __device__ void load(int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const;
+//   __device__ void finalize(int key, const fx_t (&s)[NCH]) const;     // exact sums of a segment
 // ---------------------------------------------------------------------------------------------
 template <class Op>
 __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restrict__ keys, int n, SegCarry carry)
@@ -80,195 +196,174 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restric
     constexpr int NCH = Op::NCH;
     constexpr int IT = Op::ITEMS;
     constexpr int TILE = BLOCK * IT;
+    static_assert(IT % 4 == 0, "runs are loaded with 16-byte accesses");
+    constexpr int QCAP = BLOCK;  // queued segment ends per tile
+    __shared__ int s_wh[BLOCK / WAVE];
+    __shared__ fx_t s_wb[BLOCK / WAVE][NCH];
+    __shared__ int s_nq;
+    __shared__ int s_qkey[QCAP];
+    __shared__ fx_t s_qsum[QCAP][NCH];
+    if (threadIdx.x == 0)
+        s_nq = 0;
+
     const int tile = blockIdx.x;
     const int tbase = tile * TILE;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const int p0 = tbase + threadIdx.x * IT;
+    const int cnt = max(0, min(IT, n - p0));
 
     int key[IT];
-    bool head[IT];
-    fx_t val[IT][NCH];
+    load_run<IT>(keys, p0, cnt, key, -1);
     const int kprev = (p0 > 0 && p0 - 1 < n) ? keys[p0 - 1] : -1;
     const int knext = (p0 + IT < n) ? keys[p0 + IT] : -1;
+
+    float vf[IT][NCH];
+    op.load(p0, cnt, key, vf);
+    // Thread-local inclusive prefix; the thread's last segment start and the prefix before it.
+    fx_t pre[IT][NCH];
+    int lh = -1;
 #pragma unroll
     for (int i = 0; i < IT; ++i)
     {
-        const int p = p0 + i;
-        key[i] = p < n ? keys[p] : -1;
-    }
 #pragma unroll
-    for (int i = 0; i < IT; ++i)
-    {
-        const int p = p0 + i;
+        for (int c = 0; c < NCH; ++c)
+        {
+            const fx_t x = i < cnt ? fx_from_f32(vf[i][c]) : (fx_t)0;
+            pre[i][c] = (i == 0 ? (fx_t)0 : pre[i - 1][c]) + x;
+        }
         const int kp = i == 0 ? kprev : key[i - 1];
-        head[i] = p < n && (p == 0 || key[i] != kp);
-        if (p < n)
-        {
-            float v[NCH];
-            op.load(p, key[i], v);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                val[i][c] = fx_from_f32(v[c]);
-        }
-        else
-        {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                val[i][c] = 0;
-        }
+        if (i < cnt && (p0 + i == 0 || key[i] != kp))
+            lh = i;
     }
-
-    // Thread aggregate (flag = a segment starts in this thread; value = sum since the last start).
-    bool f = false;
-    fx_t v[NCH];
+    fx_t tot[NCH], ex[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
-        v[c] = 0;
-#pragma unroll
-    for (int i = 0; i < IT; ++i)
-    {
-        if (head[i])
-        {
-            f = true;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                v[c] = val[i][c];
-        }
-        else
-        {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                v[c] += val[i][c];
-        }
-    }
+        tot[c] = pre[IT - 1][c];
+    block_exclusive_scan_fx_n<NCH>(tot, ex);
 
-    // Wave inclusive segmented scan of (f, v).
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1)
-    {
-        const int fu = __shfl_up((int)f, d, WAVE);
-        fx_t vu[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            vu[c] = shfl_up_fx(v[c], d);
-        if (lane >= d)
-        {
-            if (!f)
-            {
-#pragma unroll
-                for (int c = 0; c < NCH; ++c)
-                    v[c] += vu[c];
-            }
-            f = f || (fu != 0);
-        }
-    }
-    // Exclusive within the wave.
-    bool ef = __shfl_up((int)f, 1, WAVE) != 0;
-    fx_t ev[NCH];
+    // Prefix (tile-relative, exclusive) at this thread's last segment start.
+    fx_t bt[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
-        ev[c] = shfl_up_fx(v[c], 1);
+    {
+        fx_t before = 0;
+#pragma unroll
+        for (int i = 1; i < IT; ++i)
+            if (i == lh)
+                before = pre[i - 1][c];
+        bt[c] = ex[c] + before;
+    }
+    const uint64_t hm = __ballot(lh >= 0);
+    const uint64_t below = hm & (lane == 0 ? 0ull : (~0ull >> (WAVE - lane)));
+    const int src = below ? 63 - __builtin_clzll(below) : lane;
+    fx_t base[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        base[c] = shfl_fx(bt[c], src);
+    bool in_tile = below != 0;
     if (lane == 0)
+        s_wh[wid] = hm != 0;
+    if (hm != 0 && lane == 63 - __builtin_clzll(hm))
     {
-        ef = false;
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-            ev[c] = 0;
-    }
-    __shared__ fx_t s_wv[BLOCK / WAVE][NCH];
-    __shared__ int s_wf[BLOCK / WAVE];
-    if (lane == WAVE - 1)
-    {
-        s_wf[wid] = f ? 1 : 0;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            s_wv[wid][c] = v[c];
+            s_wb[wid][c] = bt[c];
     }
     __syncthreads();
-    bool cf = false;
-    fx_t cv[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c)
-        cv[c] = 0;
-    for (int w = 0; w < wid; ++w)
+    if (!in_tile)
     {
-        if (s_wf[w])
+        for (int w = wid - 1; w >= 0; --w)
         {
-            cf = true;
+            if (s_wh[w])
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; ++c)
+                    base[c] = s_wb[w][c];
+                in_tile = true;
+                break;
+            }
+        }
+        if (!in_tile)
+        {
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
-                cv[c] = s_wv[w][c];
-        }
-        else
-        {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                cv[c] += s_wv[w][c];
+                base[c] = 0;
         }
     }
-    if (!ef)
-    {
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            ev[c] += cv[c];
-    }
-    ef = ef || cf;
 
-    // Second pass: running inclusive value per element; emit at segment ends and at the tile end.
     const int tlast = min(tbase + TILE, n) - 1;
-    bool rf = ef;
-    fx_t r[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c)
-        r[c] = ev[c];
 #pragma unroll
     for (int i = 0; i < IT; ++i)
     {
+        if (i >= cnt)
+            break;
         const int p = p0 + i;
-        if (p < n)
+        const int kp = i == 0 ? kprev : key[i - 1];
+        if (p == 0 || key[i] != kp)
         {
-            if (head[i])
-            {
-                rf = true;
 #pragma unroll
-                for (int c = 0; c < NCH; ++c)
-                    r[c] = val[i][c];
-            }
-            else
-            {
+            for (int c = 0; c < NCH; ++c)
+                base[c] = ex[c] + (i == 0 ? (fx_t)0 : pre[i - 1][c]);
+            in_tile = true;
+        }
+        const int nk = i + 1 < IT ? key[i + 1] : knext;
+        const bool is_end = (p == n - 1) || nk != key[i];
+        if (is_end || p == tlast)
+        {
+            fx_t s[NCH];
 #pragma unroll
-                for (int c = 0; c < NCH; ++c)
-                    r[c] += val[i][c];
-            }
-            const int nk = i + 1 < IT ? key[i + 1] : knext;
-            const bool is_end = (p == n - 1) || nk != key[i];
+            for (int c = 0; c < NCH; ++c)
+                s[c] = ex[c] + pre[i][c] - base[c];
             if (is_end)
             {
-                if (rf)
+                if (in_tile)
                 {
-                    op.finalize(key[i], r);
+                    // Queue the segment so that finalisation runs with every lane busy; a tile with
+                    // more than QCAP segments finalises the rest in place.
+                    const int q = atomicAdd(&s_nq, 1);
+                    if (q < QCAP)
+                    {
+                        s_qkey[q] = key[i];
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c)
+                            s_qsum[q][c] = s[c];
+                    }
+                    else
+                    {
+                        op.finalize(key[i], s);
+                    }
                 }
                 else
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; ++c)
-                        carry.head_partial[(size_t)tile * NCH + c] = r[c];
+                        carry.head_partial[(size_t)tile * NCH + c] = s[c];
                 }
             }
             if (p == tlast)
             {
-                uint8_t fl = rf ? 0 : SEG_NOHEAD;
+                uint8_t fl = in_tile ? 0 : SEG_NOHEAD;
                 if (!is_end)
                 {
                     fl |= SEG_TAIL_CONT;
 #pragma unroll
                     for (int c = 0; c < NCH; ++c)
-                        carry.tail_partial[(size_t)tile * NCH + c] = r[c];
+                        carry.tail_partial[(size_t)tile * NCH + c] = s[c];
                 }
                 carry.flags[tile] = fl;
             }
         }
+    }
+    __syncthreads();
+    const int nq = min(s_nq, QCAP);
+    for (int q = threadIdx.x; q < nq; q += BLOCK)
+    {
+        fx_t s[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            s[c] = s_qsum[q][c];
+        op.finalize(s_qkey[q], s);
     }
 }
 
@@ -370,6 +465,129 @@ __device__ __forceinline__ int upper_bound_i32(const int* __restrict__ a, int n,
         }
     }
     return lo;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exact scans (reduce, scan of tile totals, rescan with carry)
+// ---------------------------------------------------------------------------------------------
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
+// Src interface: __device__ void load4(int p0, int cnt, float (&v)[4]) const;  (values p0 .. p0+cnt-1)
+template <class Src>
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, int n, fx_t* __restrict__ tile_sum)
+{
+    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const int cnt = max(0, min(SCAN_ITEMS, n - p0));
+    float v[SCAN_ITEMS];
+    src.load4(p0, cnt, v);
+    fx_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (i < cnt)
+            s += fx_from_f32(v[i]);
+    fx_t total;
+    block_exclusive_scan_fx(s, &total);
+    if (threadIdx.x == 0)
+        tile_sum[blockIdx.x] = total;
+}
+
+enum ScanMode
+{
+    SCAN_ORDERS = 0,    // particle-order normalisation: scale = v / total
+    SCAN_RESAMPLE = 1,  // resampling: joint_max and new weight
+    SCAN_EXP = 2,       // exponential spacings: total into exp_total
+};
+
+// Single workgroup: exclusive prefix of the tile totals (each thread scans a contiguous chunk),
+// optional inclusive tile ends, and the scalars derived from the grand total.
+template <int Unused = 0>
+__global__ __launch_bounds__(BLOCK) void k_scan_top(const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
+                                                    fx_t* __restrict__ tile_incl, int ntiles, DevScalars* sc,
+                                                    int mode, int v)
+{
+    const int per = (ntiles + BLOCK - 1) / BLOCK;
+    const int b0 = min(ntiles, (int)threadIdx.x * per);
+    const int b1 = min(ntiles, b0 + per);
+    fx_t s = 0;
+    for (int t = b0; t < b1; ++t)
+        s += tile_sum[t];
+    fx_t carry;
+    fx_t run = block_exclusive_scan_fx(s, &carry);
+    for (int t = b0; t < b1; ++t)
+    {
+        tile_pre[t] = run;
+        run += tile_sum[t];
+        if (tile_incl)
+            tile_incl[t] = run;
+    }
+    if (threadIdx.x == 0)
+    {
+        if (mode == SCAN_EXP)
+        {
+            sc->exp_total = carry;
+            sc->spacing = spacing_setup(carry, sc->total);
+            return;
+        }
+        sc->total = carry;
+        if (mode == SCAN_ORDERS)
+        {
+            sc->scale = carry != 0 ? order_scale(carry, v) : 0.0;
+        }
+        else
+        {
+            const float jm = fx_to_f32(carry);
+            sc->joint_max = jm;
+            sc->new_weight = jm / (float)v;
+        }
+    }
+}
+
+// Sink interface: __device__ void apply4(int p0, int cnt, const fx_t (&prefix)[4], const fx_t (&own)[4]) const;
+template <class Src, class Sink>
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, int n, const fx_t* __restrict__ tile_pre)
+{
+    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const int cnt = max(0, min(SCAN_ITEMS, n - p0));
+    float f[SCAN_ITEMS];
+    src.load4(p0, cnt, f);
+    fx_t v[SCAN_ITEMS];
+    fx_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        v[i] = i < cnt ? fx_from_f32(f[i]) : (fx_t)0;
+        s += v[i];
+    }
+    fx_t total;
+    fx_t run = block_exclusive_scan_fx(s, &total) + tile_pre[blockIdx.x];
+    fx_t pre[SCAN_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        run += v[i];
+        pre[i] = run;
+    }
+    if (cnt > 0)
+        sink.apply4(p0, cnt, pre, v);
+}
+
+template <class Src, class Sink>
+inline void run_exact_scan(dogm_handle* h, const Src& src, const Sink& sink, int n, int mode, int v, int kid_apply,
+                           fx_t* tile_incl = nullptr)
+{
+    const int tiles = div_up(n, SCAN_TILE);
+    launch_begin(h, K_SCAN_TILES);
+    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, h->stream, src, n, h->scan_tile_sum);
+    launch_end(h, K_SCAN_TILES);
+    launch_begin(h, K_SCAN_TOP);
+    hipLaunchKernelGGL(k_scan_top<0>, dim3(1), dim3(BLOCK), 0, h->stream, h->scan_tile_sum, h->scan_tile_pre,
+                       tile_incl, tiles, h->scalars, mode, v);
+    launch_end(h, K_SCAN_TOP);
+    launch_begin(h, kid_apply);
+    hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, h->stream, src, sink, n,
+                       h->scan_tile_pre);
+    launch_end(h, kid_apply);
 }
 
 }  // namespace dogm_hip

@@ -57,7 +57,9 @@ struct DevGrid
 // Scalars produced and consumed on the device within one frame (no host round trips).
 struct DevScalars
 {
-    fx_t total;        // last exact scan total
+    fx_t total;        // last exact scan total (particle orders or resampling CDF)
+    fx_t exp_total;    // resampling: sum of the N + 1 exponentials
+    dogm_num::Spacing spacing;  // resampling: draw mapping derived from exp_total and total
     double scale;      // particle-order scale v / total
     float joint_max;   // resampling: f32 of the CDF total
     float new_weight;  // resampling: joint_max / N
@@ -67,7 +69,7 @@ struct DevScalars
 
 enum DevErr
 {
-    DEV_ERR_BUCKET_CAPACITY = 1,
+    DEV_ERR_SPIN_LIMIT = 1,  // a bounded wait on another workgroup gave up
 };
 
 // Per-tile carry of the exact segmented reductions.
@@ -111,11 +113,8 @@ enum KernelId
     K_SEG_MOM,
     K_SEG_MOM_FIX,
     K_CDF_WRITE,
-    K_RS_COUNT,
-    K_RS_OFFSETS,
-    K_RS_SCATTER,
-    K_RS_BUCKET,
-    K_RS_GATHER,
+    K_EXP_TILES,
+    K_RS_MERGE,
     K_FILL,
     K_PACK_GRID,
     K_UNPACK_GRID,
@@ -174,14 +173,7 @@ struct dogm_handle
     dogm_hip::fx_t* scan_tile_sum = nullptr;
     dogm_hip::fx_t* scan_tile_pre = nullptr;
     dogm_hip::fx_t* cdf = nullptr;
-
-    // resampling buckets
-    int rs_bits = 0;
-    int rs_buckets = 0;
-    unsigned* rs_cnt = nullptr;
-    unsigned* rs_off = nullptr;
-    unsigned* rs_fill = nullptr;
-    uint64_t* rs_vals = nullptr;
+    dogm_hip::fx_t* cdf_tile_incl = nullptr;  // inclusive CDF value at the end of every scan tile
 
     dogm_hip::DevScalars* scalars = nullptr;
     dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes

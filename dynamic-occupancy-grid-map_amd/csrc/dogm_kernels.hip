@@ -204,7 +204,14 @@ struct OccOp
     static constexpr int ITEMS = 8;
     const float* weight_array;
     float* m_raw;
-    __device__ void load(int pos, int, float (&v)[1]) const { v[0] = weight_array[pos]; }
+    __device__ void load(int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
+    {
+        float w[ITEMS];
+        load_run<ITEMS>(weight_array, p0, cnt, w);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            v[i][0] = w[i];
+    }
     __device__ void finalize(int key, const fx_t (&s)[1]) const { m_raw[key] = fx_to_f32(s[0]); }
 };
 
@@ -302,11 +309,17 @@ struct PersOp
     float* weight_array;
     const dogm_measurement_cell* meas;
     DevGrid G;
-    __device__ void load(int pos, int key, float (&v)[1]) const
+    __device__ void load(int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
-        const float lw = meas[key].likelihood * weight[pos];
-        weight_array[pos] = lw;
-        v[0] = lw;
+        float w[ITEMS], lw[ITEMS];
+        load_run<ITEMS>(weight, p0, cnt, w);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+        {
+            lw[i] = i < cnt ? meas[key[i]].likelihood * w[i] : 0.0f;
+            v[i][0] = lw[i];
+        }
+        store_run<ITEMS>(weight_array, p0, cnt, lw);
     }
     __device__ void finalize(int key, const fx_t (&s)[1]) const
     {
@@ -363,106 +376,10 @@ void run_persistent(dogm_handle* h)
     launch_end(h, K_POST_WEIGHTS);
 }
 
-// ---------------------------------------------------------------------------------------------
-// Exact scans (reduce, scan of tile totals, rescan with carry)
-// ---------------------------------------------------------------------------------------------
-constexpr int SCAN_ITEMS = 4;
-constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
-
-template <class Src>
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, int n, fx_t* __restrict__ tile_sum)
-{
-    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    fx_t s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
-        if (p0 + i < n)
-            s += fx_from_f32(src(p0 + i));
-    fx_t total;
-    block_exclusive_scan_fx(s, &total);
-    if (threadIdx.x == 0)
-        tile_sum[blockIdx.x] = total;
-}
-
-enum ScanMode
-{
-    SCAN_ORDERS = 0,    // particle-order normalisation: scale = v / total
-    SCAN_RESAMPLE = 1,  // resampling: joint_max and new weight
-};
-
-__global__ __launch_bounds__(BLOCK) void k_scan_top(const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
-                                                    int ntiles, DevScalars* sc, int mode, int v)
-{
-    fx_t carry = 0;
-    for (int b = 0; b < ntiles; b += BLOCK)
-    {
-        const int t = b + threadIdx.x;
-        const fx_t x = t < ntiles ? tile_sum[t] : (fx_t)0;
-        fx_t tot;
-        const fx_t ex = block_exclusive_scan_fx(x, &tot);
-        if (t < ntiles)
-            tile_pre[t] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0)
-    {
-        sc->total = carry;
-        if (mode == SCAN_ORDERS)
-        {
-            sc->scale = carry != 0 ? order_scale(carry, v) : 0.0;
-        }
-        else
-        {
-            const float jm = fx_to_f32(carry);
-            sc->joint_max = jm;
-            sc->new_weight = jm / (float)v;
-        }
-    }
-}
-
-template <class Src, class Sink>
-__global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, int n, const fx_t* __restrict__ tile_pre)
-{
-    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    fx_t v[SCAN_ITEMS];
-    fx_t s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
-    {
-        v[i] = p0 + i < n ? fx_from_f32(src(p0 + i)) : (fx_t)0;
-        s += v[i];
-    }
-    fx_t total;
-    fx_t run = block_exclusive_scan_fx(s, &total) + tile_pre[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
-    {
-        if (p0 + i < n)
-        {
-            run += v[i];
-            sink(p0 + i, run, v[i]);
-        }
-    }
-}
-
 struct SrcArray
 {
     const float* a;
-    __device__ float operator()(int i) const { return a[i]; }
-};
-
-struct SrcMeasOcc
-{
-    const dogm_measurement_cell* m;
-    __device__ float operator()(int i) const { return m[i].occ_mass; }
-};
-
-struct SrcJoint
-{
-    const float* w;
-    const float* bw;
-    int N;
-    __device__ float operator()(int i) const { return i < N ? w[i] : bw[i - N]; }
+    __device__ void load4(int p0, int cnt, float (&v)[4]) const { load_run<4>(a, p0, cnt, v); }
 };
 
 __device__ __forceinline__ int order_end_at(int j, fx_t prefix, const DevScalars* sc, int C, int v)
@@ -480,7 +397,14 @@ struct SinkOrders
     int* pend;
     const DevScalars* sc;
     int C, v;
-    __device__ void operator()(int j, fx_t prefix, fx_t) const { pend[j] = order_end_at(j, prefix, sc, C, v); }
+    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&)[4]) const
+    {
+        int e[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            e[i] = i < cnt ? order_end_at(p0 + i, pre[i], sc, C, v) : 0;
+        store_run<4>(pend, p0, cnt, e);
+    }
 };
 
 // Birth placement and weights per cell (init_new_particles.cu:126-155).
@@ -493,43 +417,36 @@ struct SinkBirth
     float* wA;
     float* wUA;
     int C, v;
-    __device__ void operator()(int j, fx_t prefix, fx_t own) const
+    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&own)[4]) const
     {
-        const int e = order_end_at(j, prefix, sc, C, v);
-        const int s = j == 0 ? 0 : order_end_at(j - 1, prefix - own, sc, C, v);
-        const int n = e - s;
-        const float pA = meas[j].p_A;
-        const int nu_A = (int)roundf((float)n * pA);
-        const int nu_UA = n - nu_A;
-        const float bm = born[j];
-        wA[j] = nu_A > 0 ? (pA * bm) / (float)nu_A : 0.0f;
-        wUA[j] = nu_UA > 0 ? ((1.0f - pA) * bm) / (float)nu_UA : 0.0f;
-        pend[j] = e;
+        float bm[4];
+        load_run<4>(born, p0, cnt, bm);
+        int e[4];
+        float a[4], u[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            e[i] = 0;
+            a[i] = 0.0f;
+            u[i] = 0.0f;
+            if (i < cnt)
+            {
+                const int j = p0 + i;
+                e[i] = order_end_at(j, pre[i], sc, C, v);
+                const int s = j == 0 ? 0 : order_end_at(j - 1, pre[i] - own[i], sc, C, v);
+                const int n = e[i] - s;
+                const float pA = meas[j].p_A;
+                const int nu_A = (int)roundf((float)n * pA);
+                const int nu_UA = n - nu_A;
+                a[i] = nu_A > 0 ? (pA * bm[i]) / (float)nu_A : 0.0f;
+                u[i] = nu_UA > 0 ? ((1.0f - pA) * bm[i]) / (float)nu_UA : 0.0f;
+            }
+        }
+        store_run<4>(pend, p0, cnt, e);
+        store_run<4>(wA, p0, cnt, a);
+        store_run<4>(wUA, p0, cnt, u);
     }
 };
-
-struct SinkCdf
-{
-    fx_t* cdf;
-    __device__ void operator()(int k, fx_t prefix, fx_t) const { cdf[k] = prefix; }
-};
-
-template <class Src, class Sink>
-static void run_exact_scan(dogm_handle* h, const Src& src, const Sink& sink, int n, int mode, int v, int kid_apply)
-{
-    const int tiles = div_up(n, SCAN_TILE);
-    launch_begin(h, K_SCAN_TILES);
-    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, h->stream, src, n, h->scan_tile_sum);
-    launch_end(h, K_SCAN_TILES);
-    launch_begin(h, K_SCAN_TOP);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, h->stream, h->scan_tile_sum, h->scan_tile_pre, tiles,
-                       h->scalars, mode, v);
-    launch_end(h, K_SCAN_TOP);
-    launch_begin(h, kid_apply);
-    hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, h->stream, src, sink, n,
-                       h->scan_tile_pre);
-    launch_end(h, kid_apply);
-}
 
 // ---------------------------------------------------------------------------------------------
 // Birth particles (dogm/src/dogm.cu:323-348, init.cu:46-67, init_new_particles.cu:157-195)
@@ -636,19 +553,24 @@ void run_init_particles(dogm_handle* h)
 struct MomOp
 {
     static constexpr int NCH = 5;
-    static constexpr int ITEMS = 2;
+    static constexpr int ITEMS = 4;
     const float* weight_array;
     const float4* state;
     DevGrid G;
-    __device__ void load(int pos, int, float (&v)[5]) const
+    __device__ void load(int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
-        const float w = weight_array[pos];
-        const float4 s = state[pos];
-        v[0] = w * s.z;
-        v[1] = w * s.w;
-        v[2] = w * s.z * s.z;
-        v[3] = w * s.w * s.w;
-        v[4] = w * s.z * s.w;
+        float w[ITEMS];
+        load_run<ITEMS>(weight_array, p0, cnt, w);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+        {
+            const float4 s = i < cnt ? state[p0 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            v[i][0] = w[i] * s.z;
+            v[i][1] = w[i] * s.w;
+            v[i][2] = w[i] * s.z * s.z;
+            v[i][3] = w[i] * s.w * s.w;
+            v[i][4] = w[i] * s.z * s.w;
+        }
     }
     __device__ void finalize(int key, const fx_t (&s)[5]) const
     {
@@ -681,189 +603,6 @@ void run_moments(dogm_handle* h)
                        h->grid.f[GF_MEAN_X], h->grid.f[GF_MEAN_Y], h->grid.f[GF_VAR_X], h->grid.f[GF_VAR_Y],
                        h->grid.f[GF_COVAR], 5, h->C);
     launch_end(h, K_EMPTY_CELLS);
-}
-
-// ---------------------------------------------------------------------------------------------
-// Resampling (dogm/src/dogm.cu:386-423, resampling.cu:17-68)
-// Sorted uniforms without a global sort: the 64-bit uniforms are bucketed by their top bits
-// (buckets are contiguous value ranges), each bucket is sorted in LDS, and every sorted draw
-// finds its ancestor in the exact CDF.  Equal draws are interchangeable, so the result does not
-// depend on the order in which atomics fill a bucket.
-// ---------------------------------------------------------------------------------------------
-constexpr int RS_CAP = 2048;  // LDS capacity of one bucket
-
-__device__ __forceinline__ uint64_t resample_draw(uint64_t seed, uint32_t inst, uint32_t epoch, int k)
-{
-    const u32x4 r = draw(seed, inst, TAG_RESAMPLE, epoch, (uint32_t)k);
-    return ((uint64_t)r.x << 32) | (uint64_t)r.y;
-}
-
-__device__ __forceinline__ unsigned bucket_of(uint64_t u, int bits)
-{
-    return bits == 0 ? 0u : (unsigned)(u >> (64 - bits));
-}
-
-__global__ __launch_bounds__(BLOCK) void k_rs_count(unsigned* __restrict__ cnt, int n, int bits, uint64_t seed,
-                                                    uint32_t inst, uint32_t epoch)
-{
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= n)
-        return;
-    atomicAdd(&cnt[bucket_of(resample_draw(seed, inst, epoch, k), bits)], 1u);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_rs_offsets(const unsigned* __restrict__ cnt, unsigned* __restrict__ off,
-                                                      unsigned* __restrict__ fill, int nb, DevScalars* sc)
-{
-    __shared__ unsigned s_w[BLOCK / WAVE];
-    __shared__ unsigned s_max;
-    if (threadIdx.x == 0)
-        s_max = 0;
-    __syncthreads();
-    unsigned carry = 0;
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    for (int b = 0; b < nb; b += BLOCK)
-    {
-        const int t = b + threadIdx.x;
-        const unsigned x = t < nb ? cnt[t] : 0u;
-        if (t < nb)
-        {
-            atomicMax(&s_max, x);
-            fill[t] = 0u;
-        }
-        unsigned incl = x;
-        for (int d = 1; d < WAVE; d <<= 1)
-        {
-            const unsigned u = __shfl_up(incl, d, WAVE);
-            if (lane >= d)
-                incl += u;
-        }
-        if (lane == WAVE - 1)
-            s_w[wid] = incl;
-        __syncthreads();
-        unsigned wc = 0, tot = 0;
-        for (int w = 0; w < BLOCK / WAVE; ++w)
-        {
-            if (w < wid)
-                wc += s_w[w];
-            tot += s_w[w];
-        }
-        if (t < nb)
-            off[t] = carry + wc + incl - x;
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && s_max > (unsigned)RS_CAP)
-        atomicOr(&sc->err, (int)DEV_ERR_BUCKET_CAPACITY);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_rs_scatter(const unsigned* __restrict__ off, unsigned* __restrict__ fill,
-                                                      uint64_t* __restrict__ vals, int n, int bits, uint64_t seed,
-                                                      uint32_t inst, uint32_t epoch)
-{
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= n)
-        return;
-    const uint64_t u = resample_draw(seed, inst, epoch, k);
-    const unsigned b = bucket_of(u, bits);
-    const unsigned pos = off[b] + atomicAdd(&fill[b], 1u);
-    vals[pos] = u;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_rs_bucket(const uint64_t* __restrict__ vals,
-                                                     const unsigned* __restrict__ off,
-                                                     const unsigned* __restrict__ cnt, const fx_t* __restrict__ cdf,
-                                                     int J, const DevScalars* __restrict__ sc,
-                                                     int* __restrict__ ancestors)
-{
-    __shared__ uint64_t s[RS_CAP];
-    const int b = blockIdx.x;
-    const unsigned o = off[b];
-    const int c = (int)cnt[b];
-    if (c == 0 || c > RS_CAP)
-        return;  // capacity overflow is reported by k_rs_offsets
-    int np2 = 1;
-    while (np2 < c)
-        np2 <<= 1;
-    for (int i = threadIdx.x; i < np2; i += BLOCK)
-        s[i] = i < c ? vals[o + i] : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= np2; k <<= 1)
-    {
-        for (int j = k >> 1; j > 0; j >>= 1)
-        {
-            for (int i = threadIdx.x; i < np2; i += BLOCK)
-            {
-                const int ixj = i ^ j;
-                if (ixj > i)
-                {
-                    const uint64_t a = s[i], bb = s[ixj];
-                    const bool asc = (i & k) == 0;
-                    if (asc ? (a > bb) : (a < bb))
-                    {
-                        s[i] = bb;
-                        s[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    const fx_t total = sc->total;
-    // A draw beyond the last CDF entry is only possible for a negative total; it is clamped to the
-    // last entry (same rule as the checker).
-    for (int i = threadIdx.x; i < c; i += BLOCK)
-        ancestors[o + i] = min(lower_bound_fx(cdf, J, fx_scale_u64(total, s[i])), J - 1);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_rs_gather(DevParticles src, DevParticles bsrc, DevParticles dst,
-                                                     const int* __restrict__ anc, int n,
-                                                     const DevScalars* __restrict__ sc)
-{
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= n)
-        return;
-    const int a = anc[k];
-    const bool from_p = a < n;
-    const DevParticles& S = from_p ? src : bsrc;
-    const int o = from_p ? a : a - n;
-    dst.state[k] = S.state[o];
-    dst.idx[k] = S.idx[o];
-    dst.assoc[k] = S.assoc[o];
-    dst.weight[k] = sc->new_weight;
-}
-
-void run_resampling(dogm_handle* h)
-{
-    const int J = h->N + h->B;
-    SrcJoint src{h->weight_array, h->birth.weight, h->N};
-    SinkCdf sink{h->cdf};
-    run_exact_scan(h, src, sink, J, SCAN_RESAMPLE, h->N, K_CDF_WRITE);
-
-    run_fill_u32(h, h->rs_cnt, 0u, h->rs_buckets);
-    launch_begin(h, K_RS_COUNT);
-    hipLaunchKernelGGL(k_rs_count, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->rs_cnt, h->N, h->rs_bits,
-                       h->seed, h->instance, h->epoch);
-    launch_end(h, K_RS_COUNT);
-    launch_begin(h, K_RS_OFFSETS);
-    hipLaunchKernelGGL(k_rs_offsets, dim3(1), dim3(BLOCK), 0, h->stream, h->rs_cnt, h->rs_off, h->rs_fill,
-                       h->rs_buckets, h->scalars);
-    launch_end(h, K_RS_OFFSETS);
-    launch_begin(h, K_RS_SCATTER);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->rs_off, h->rs_fill,
-                       h->rs_vals, h->N, h->rs_bits, h->seed, h->instance, h->epoch);
-    launch_end(h, K_RS_SCATTER);
-    launch_begin(h, K_RS_BUCKET);
-    hipLaunchKernelGGL(k_rs_bucket, dim3(h->rs_buckets), dim3(BLOCK), 0, h->stream, h->rs_vals, h->rs_off, h->rs_cnt,
-                       h->cdf, J, h->scalars, h->ancestors);
-    launch_end(h, K_RS_BUCKET);
-    const int other = h->cur ^ 1;
-    launch_begin(h, K_RS_GATHER);
-    hipLaunchKernelGGL(k_rs_gather, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->part[h->cur], h->birth,
-                       h->part[other], h->ancestors, h->N, h->scalars);
-    launch_end(h, K_RS_GATHER);
-    h->cur = other;
-    ++h->epoch;
 }
 
 // ---------------------------------------------------------------------------------------------

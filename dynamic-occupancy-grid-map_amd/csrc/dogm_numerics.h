@@ -392,6 +392,71 @@ DOGM_HD fx_t fx_scale_u64(fx_t total, uint64_t u)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Multinomial resampling draws (dogm/src/kernel/resampling.cu:17-47 draws N uniforms and sorts
+// them).  Sorted uniforms are generated directly from exponential spacings: with E_0..E_N i.i.d.
+// Exp(1) and S_k = E_0 + ... + E_k, the vector (S_0 / S_N, ..., S_{N-1} / S_N) has the law of the
+// order statistics of N i.i.d. U(0, 1) (Devroye, Non-Uniform Random Variate Generation, 1986,
+// ch. V.3).  S is an exact fixed-point prefix; the ratio is one IEEE double division; the result is
+// a 64-bit fraction of the CDF total.
+// ---------------------------------------------------------------------------------------------
+// E_k for k = 4m .. 4m + 3 come from the four words of draw m.
+DOGM_HD void resample_exponentials4(uint64_t seed, uint32_t instance, uint32_t epoch, uint32_t m, float* e)
+{
+    const u32x4 r = draw(seed, instance, TAG_RESAMPLE, epoch, m);
+    e[0] = -log_f32(uniform01_oc(r.x));
+    e[1] = -log_f32(uniform01_oc(r.y));
+    e[2] = -log_f32(uniform01_oc(r.z));
+    e[3] = -log_f32(uniform01_oc(r.w));
+}
+
+DOGM_HD float resample_exponential(uint64_t seed, uint32_t instance, uint32_t epoch, uint32_t k)
+{
+    float e[4];
+    resample_exponentials4(seed, instance, epoch, k >> 2, e);
+    return e[k & 3u];
+}
+
+// The draw for prefix S_k on the CDF axis, fx(total * u_k) with u_k = (S_k >> sh) * inv, where
+// sh brings S_N below 2^53 (so S_k >> sh converts to double exactly) and inv = 1 / (S_N >> sh).
+// Every step is monotone in S_k, so sorted prefixes give sorted draws; u_k is capped below 1.
+struct Spacing
+{
+    int shift;
+    double inv;
+    double total;  // CDF total as a double
+};
+
+DOGM_HD Spacing spacing_setup(fx_t s_n, fx_t cdf_total)
+{
+    Spacing sp;
+    sp.shift = 0;
+    sp.inv = 0.0;
+    if (s_n > 0)
+    {
+        const int msb = 127 - clz128((ufx_t)s_n);
+        sp.shift = msb > 52 ? msb - 52 : 0;
+        sp.inv = 1.0 / (double)(uint64_t)(s_n >> sp.shift);
+    }
+    sp.total = fx_to_f64(cdf_total);
+    return sp;
+}
+
+DOGM_HD fx_t spacing_draw(fx_t s_k, const Spacing& sp)
+{
+    double u = (double)(uint64_t)(s_k >> sp.shift) * sp.inv;
+    if (!(u < 1.0))
+        u = 0x1.fffffffffffffp-1;
+    return fx_from_f64(sp.total * u);
+}
+
+// CDF contribution of a joint weight: negative values (from rounding in rho_p = occ - rho_b) and
+// NaN count as zero, so the CDF is non-decreasing and the ancestor search is well defined.
+DOGM_HD float cdf_weight(float w)
+{
+    return w > 0.0f ? w : 0.0f;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Particle-order normalisation (exact restatement of normalize_particle_orders,
 // dogm/src/kernel/init_new_particles.cu:30-43, 66-74).
 // The reference scales an f32 inclusive scan by v/max and truncates; here the prefix S_j is

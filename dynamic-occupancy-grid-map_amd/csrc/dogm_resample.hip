@@ -1,0 +1,252 @@
+// Multinomial resampling on gfx950 (dogm/src/dogm.cu:386-423, resampling.cu:17-68).
+//
+// 1. Exact CDF over [posterior weights | birth weights] (reduce, scan of tile totals, rescan); the
+//    inclusive CDF value at the end of every 1024-entry tile is kept as a coarse index.
+// 2. Sorted uniforms by exponential spacings: E_k for k = 0..N, exact prefix S_k, draw k is the
+//    fraction S_k / S_N of the CDF total (spacing_draw in dogm_numerics.h).  Only the tile totals of E are
+//    stored; the merge kernel regenerates E from the counter-based RNG (four E per Philox call).
+// 3. Merge: a workgroup owns 1024 consecutive (hence sorted) draws.  Its ancestors lie between the
+//    CDF tiles holding its first and last draw; those are found by a two-level k-ary search over
+//    the coarse index (two dependent loads instead of a 22-level binary search), the CDF entries of
+//    those tiles are staged in LDS, every draw finds its ancestor there, and the next particle is
+//    written directly (the reference's lower_bound and resamplingKernel gather in one pass).
+// No atomics on global memory and no global sort: the result is a pure function of the inputs.
+#include "dogm_device.h"
+
+namespace dogm_hip
+{
+
+constexpr int RS_ITEMS = 4;
+constexpr int RS_TILE = BLOCK * RS_ITEMS;  // draws per workgroup
+constexpr int RS_SLICE_TILES = 2;          // CDF tiles staged in LDS
+constexpr int RS_SLICE = RS_SLICE_TILES * SCAN_TILE;
+
+static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-scan tile size");
+
+struct SrcJointPos
+{
+    const float* w;
+    const float* bw;
+    int N;
+    __device__ void load4(int p0, int cnt, float (&v)[4]) const
+    {
+        if (p0 + cnt <= N)
+            load_run<4>(w, p0, cnt, v);
+        else if (p0 >= N && ((p0 - N) & 3) == 0)
+            load_run<4>(bw, p0 - N, cnt, v);
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                v[i] = i < cnt ? (p0 + i < N ? w[p0 + i] : bw[p0 + i - N]) : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v[i] = cdf_weight(v[i]);
+    }
+};
+
+struct SinkCdf
+{
+    fx_t* cdf;
+    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&)[4]) const
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i < cnt)
+                cdf[p0 + i] = pre[i];
+    }
+};
+
+// E for the four consecutive draws k = p0 .. p0 + 3 of one thread (p0 is a multiple of 4).
+__device__ __forceinline__ void exp4(uint64_t seed, uint32_t inst, uint32_t epoch, int p0, int n, fx_t* e)
+{
+    float f[4];
+    resample_exponentials4(seed, inst, epoch, (uint32_t)(p0 >> 2), f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        e[i] = p0 + i < n ? fx_from_f32(f[i]) : (fx_t)0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_exp_tiles(uint64_t seed, uint32_t inst, uint32_t epoch, int n,
+                                                     fx_t* __restrict__ tile_sum)
+{
+    const int p0 = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
+    fx_t e[4];
+    exp4(seed, inst, epoch, p0, n, e);
+    fx_t total;
+    block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &total);
+    if (threadIdx.x == 0)
+        tile_sum[blockIdx.x] = total;
+}
+
+// Index of the first CDF tile whose inclusive end is >= r (T - 1 if none): two-level k-ary search.
+__device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* s_cnt)
+{
+    const int G = (T + BLOCK - 1) / BLOCK;  // tiles per group; at most BLOCK groups
+    const int ng = (T + G - 1) / G;
+    if (threadIdx.x == 0)
+        s_cnt[0] = 0;
+    __syncthreads();
+    const int i = threadIdx.x;
+    const bool below = i < ng && incl[min(T - 1, (i + 1) * G - 1)] < r;
+    const int groups_below = __syncthreads_count(below);
+    const int g = min(groups_below, ng - 1);
+    for (int j = threadIdx.x; j < G; j += BLOCK)
+    {
+        const int t = g * G + j;
+        if (t < T && incl[t] < r)
+            atomicAdd(&s_cnt[0], 1);
+    }
+    __syncthreads();
+    const int t = min(g * G + s_cnt[0], T - 1);
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst, uint32_t epoch, int n,
+                                                    const fx_t* __restrict__ exp_tile_pre,
+                                                    const fx_t* __restrict__ cdf, const fx_t* __restrict__ cdf_incl,
+                                                    int J, const DevScalars* __restrict__ sc, DevParticles P,
+                                                    DevParticles Bp, DevParticles out, int* __restrict__ ancestors)
+{
+    __shared__ fx_t s_cdf[RS_SLICE];
+    __shared__ fx_t s_rb[2];
+    __shared__ int s_cnt[1];
+
+    const int base = blockIdx.x * RS_TILE;
+    const int p0 = base + threadIdx.x * RS_ITEMS;
+    fx_t e[4];
+    exp4(seed, inst, epoch, p0, n, e);
+    fx_t tot;
+    fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) + exp_tile_pre[blockIdx.x];
+    const Spacing sp = sc->spacing;
+    fx_t r[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+    {
+        run += e[i];
+        r[i] = spacing_draw(run, sp);
+    }
+    const int last = min(base + RS_TILE, n) - 1;
+    if (p0 == base)
+        s_rb[0] = r[0];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+        if (p0 + i == last)
+            s_rb[1] = r[i];
+    __syncthreads();
+    const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
+    const int t_lo = find_cdf_tile(cdf_incl, T, s_rb[0], s_cnt);
+    const int t_hi = find_cdf_tile(cdf_incl, T, s_rb[1], s_cnt);
+    // Walk the CDF tiles [t_lo, t_hi] in LDS-sized chunks; a draw is resolved in the first chunk
+    // whose last entry is >= its value (the CDF is non-decreasing, so its lower bound is there).
+    // A thread's four draws are consecutive, so after one binary search the next ancestors are
+    // found by stepping forward.
+    int anc[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+        anc[i] = -1;
+    for (int tc = t_lo; tc <= t_hi; tc += RS_SLICE_TILES)
+    {
+        const int c0 = tc * SCAN_TILE;
+        const int clen = min(J, min(tc + RS_SLICE_TILES, t_hi + 1) * SCAN_TILE) - c0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < clen; i += BLOCK)
+            s_cdf[i] = cdf[c0 + i];
+        __syncthreads();
+        const fx_t cmax = s_cdf[clen - 1];
+        int pos = -1;  // lower bound of the previous draw inside this chunk
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+        {
+            if (anc[i] < 0 && p0 + i < n && !(cmax < r[i]))
+            {
+                int lo;
+                if (pos < 0)
+                {
+                    lo = 0;
+                    int count = clen;
+                    while (count > 0)
+                    {
+                        const int step = count >> 1;
+                        if (s_cdf[lo + step] < r[i])
+                        {
+                            lo += step + 1;
+                            count -= step + 1;
+                        }
+                        else
+                        {
+                            count = step;
+                        }
+                    }
+                }
+                else
+                {
+                    lo = pos;
+                    while (s_cdf[lo] < r[i])
+                        ++lo;
+                }
+                pos = lo;
+                anc[i] = c0 + lo;
+            }
+        }
+    }
+    // Transpose through LDS so the gather and the stores run wave-contiguous.
+    __syncthreads();
+    int* s_anc = reinterpret_cast<int*>(s_cdf);
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+        s_anc[threadIdx.x * RS_ITEMS + i] = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);
+    __syncthreads();
+    const float nw = sc->new_weight;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+    {
+        const int j = i * BLOCK + threadIdx.x;
+        const int k = base + j;
+        if (k >= n)
+            break;
+        const int a = s_anc[j];
+        ancestors[k] = a;
+        const bool from_p = a < n;
+        const int o = from_p ? a : a - n;
+        const float4* st = from_p ? P.state : Bp.state;
+        const int* ix = from_p ? P.idx : Bp.idx;
+        const uint8_t* as = from_p ? P.assoc : Bp.assoc;
+        out.state[k] = st[o];
+        out.idx[k] = ix[o];
+        out.assoc[k] = as[o];
+        out.weight[k] = nw;
+    }
+}
+
+void run_resampling(dogm_handle* h)
+{
+    const int N = h->N;
+    const int J = N + h->B;
+    SrcJointPos src{h->weight_array, h->birth.weight, N};
+    SinkCdf sink{h->cdf};
+    run_exact_scan(h, src, sink, J, SCAN_RESAMPLE, N, K_CDF_WRITE, h->cdf_tile_incl);
+
+    const int etiles = div_up(N + 1, RS_TILE);
+    launch_begin(h, K_EXP_TILES);
+    hipLaunchKernelGGL(k_exp_tiles, dim3(etiles), dim3(BLOCK), 0, h->stream, h->seed, h->instance, h->epoch, N + 1,
+                       h->scan_tile_sum);
+    launch_end(h, K_EXP_TILES);
+    launch_begin(h, K_SCAN_TOP);
+    hipLaunchKernelGGL(k_scan_top<0>, dim3(1), dim3(BLOCK), 0, h->stream, h->scan_tile_sum, h->scan_tile_pre,
+                       (fx_t*)nullptr, etiles, h->scalars, (int)SCAN_EXP, N);
+    launch_end(h, K_SCAN_TOP);
+
+    const int other = h->cur ^ 1;
+    launch_begin(h, K_RS_MERGE);
+    hipLaunchKernelGGL(k_rs_merge, dim3(div_up(N, RS_TILE)), dim3(BLOCK), 0, h->stream, h->seed, h->instance,
+                       h->epoch, N, h->scan_tile_pre, h->cdf, h->cdf_tile_incl, J, h->scalars, h->part[h->cur],
+                       h->birth, h->part[other], h->ancestors);
+    launch_end(h, K_RS_MERGE);
+    h->cur = other;
+    ++h->epoch;
+}
+
+}  // namespace dogm_hip

@@ -19,6 +19,42 @@ constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int SORT_WAVES = BLOCK / WAVE;
 
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Lanes of the wave holding the same digit (among `active` lanes).  Nearly sorted keys usually
+// give one digit per wave: that case costs one ballot.
+__device__ __forceinline__ uint64_t match_digit(unsigned d, uint64_t active)
+{
+    if (active == 0)
+        return 0;
+    const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
+    if ((__ballot(d == first) & active) == active)
+        return active;
+    uint64_t peers = active;
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; ++b)
+    {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+// Adds one to s_h[d] for every active lane; lanes with equal digits are combined first, so
+// nearly sorted keys (the usual input: resampled particles are in cell order) do not serialise on
+// one LDS address.
+__device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid)
+{
+    const uint64_t peers = match_digit(d, __ballot(valid));
+    if (valid && (peers & lanemask_lt()) == 0)
+        atomicAdd(&s_h[d], (unsigned)__popcll(peers));
+}
+
 // Histogram of every pass's digit over all keys.
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const int* __restrict__ keys, int n, int passes,
                                                      unsigned* __restrict__ ghist)
@@ -31,12 +67,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const int* __restrict__ key
     for (int i = 0; i < SORT_ITEMS; ++i)
     {
         const int p = base + i * BLOCK + threadIdx.x;
-        if (p < n)
-        {
-            const unsigned k = (unsigned)keys[p];
-            for (int ps = 0; ps < passes; ++ps)
-                atomicAdd(&s_h[ps][(k >> (ps * RADIX_BITS)) & (RADIX - 1)], 1u);
-        }
+        const bool valid = p < n;
+        const unsigned k = valid ? (unsigned)keys[p] : 0u;
+        for (int ps = 0; ps < passes; ++ps)
+            wave_count(s_h[ps], (k >> (ps * RADIX_BITS)) & (RADIX - 1), valid);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < passes * RADIX; i += BLOCK)
@@ -59,15 +93,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort_upsweep(const int* __restrict__ 
     for (int i = 0; i < SORT_ITEMS; ++i)
     {
         const int p = base + i * BLOCK + threadIdx.x;
-        if (p < n)
-            atomicAdd(&s_h[((unsigned)keys[p] >> shift) & (RADIX - 1)], 1u);
+        const bool valid = p < n;
+        wave_count(s_h, valid ? ((unsigned)keys[p] >> shift) & (RADIX - 1) : 0u, valid);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < RADIX; d += BLOCK)
         thist[(size_t)d * tiles + blockIdx.x] = s_h[d];
 }
 
-// One workgroup per digit: exclusive scan over tiles plus the digit's global base.
+// One workgroup per digit: exclusive scan over tiles plus the digit's global base.  Each thread
+// scans a contiguous chunk of the digit's row.
 __global__ __launch_bounds__(BLOCK) void k_sort_scan(unsigned* __restrict__ thist, int tiles,
                                                      const unsigned* __restrict__ ghist_pass)
 {
@@ -75,50 +110,46 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(unsigned* __restrict__ this
     __shared__ unsigned s_base;
     const int d = blockIdx.x;
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    if (threadIdx.x == 0)
+    if (threadIdx.x < WAVE)
     {
         unsigned b = 0;
-        for (int e = 0; e < d; ++e)
+        for (int e = lane; e < d; e += WAVE)
             b += ghist_pass[e];
-        s_base = b;
+#pragma unroll
+        for (int s = WAVE / 2; s > 0; s >>= 1)
+            b += __shfl_xor(b, s, WAVE);
+        if (lane == 0)
+            s_base = b;
     }
-    __syncthreads();
-    unsigned carry = s_base;
     unsigned* row = thist + (size_t)d * tiles;
-    for (int b = 0; b < tiles; b += BLOCK)
+    const int per = (tiles + BLOCK - 1) / BLOCK;
+    const int b0 = min(tiles, (int)threadIdx.x * per);
+    const int b1 = min(tiles, b0 + per);
+    unsigned sum = 0;
+    for (int t = b0; t < b1; ++t)
+        sum += row[t];
+    unsigned incl = sum;
+#pragma unroll
+    for (int s = 1; s < WAVE; s <<= 1)
     {
-        const int t = b + threadIdx.x;
-        const unsigned x = t < tiles ? row[t] : 0u;
-        unsigned incl = x;
-#pragma unroll
-        for (int s = 1; s < WAVE; s <<= 1)
-        {
-            const unsigned u = __shfl_up(incl, s, WAVE);
-            if (lane >= s)
-                incl += u;
-        }
-        if (lane == WAVE - 1)
-            s_w[wid] = incl;
-        __syncthreads();
-        unsigned wc = 0, tot = 0;
-#pragma unroll
-        for (int w = 0; w < SORT_WAVES; ++w)
-        {
-            if (w < wid)
-                wc += s_w[w];
-            tot += s_w[w];
-        }
-        if (t < tiles)
-            row[t] = carry + wc + incl - x;
-        carry += tot;
-        __syncthreads();
+        const unsigned u = __shfl_up(incl, s, WAVE);
+        if (lane >= s)
+            incl += u;
     }
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt()
-{
-    const int lane = threadIdx.x & (WAVE - 1);
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (lane == WAVE - 1)
+        s_w[wid] = incl;
+    __syncthreads();
+    unsigned run = s_base + incl - sum;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w)
+        if (w < wid)
+            run += s_w[w];
+    for (int t = b0; t < b1; ++t)
+    {
+        const unsigned x = row[t];
+        row[t] = run;
+        run += x;
+    }
 }
 
 // Stable ranking and scatter of one tile.  vals_in == nullptr means identity values.
@@ -152,14 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const int* __restrict_
         key[j] = valid ? keys_in[p] : 0;
         val[j] = valid ? (vals_in ? vals_in[p] : p) : 0;
         const unsigned d = ((unsigned)key[j] >> shift) & (RADIX - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < RADIX_BITS; ++b)
-        {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
+        const uint64_t peers = match_digit(d, __ballot(valid));
         unsigned r = 0;
         if (valid)
         {

@@ -533,9 +533,12 @@ struct Oracle final : OracleBase
     }
 
     // ---- dogm.cu:386-423, resampling.cu:17-68 ---------------------------------------------------
-    // Multinomial resampling: exact CDF over [posterior weights | birth weights]; N 64-bit
-    // uniforms mapped onto [0, total) and sorted; ancestor = first CDF entry >= draw (lower bound,
-    // resampling.cu:44-46); the next particle copies the ancestor with weight total / N.
+    // Multinomial resampling: exact CDF over [posterior weights | birth weights] (negative weights
+    // count as 0); N sorted uniforms on [0, total) from exponential spacings (dogm_numerics.h);
+    // ancestor = first CDF entry >= draw (lower bound, resampling.cu:44-46); the next particle
+    // copies the ancestor with weight total / N.
+    static T pos_weight(T w) { return w > T(0) ? w : T(0); }
+
     void resampling() override
     {
         const int J = N + B;
@@ -543,42 +546,33 @@ struct Oracle final : OracleBase
         fx_t s = 0;
         for (int k = 0; k < N; ++k)
         {
-            s += fx_of(weight_array[k]);
+            s += fx_of(pos_weight(weight_array[k]));
             cdf[k] = s;
         }
         for (int k = 0; k < B; ++k)
         {
-            s += fx_of(birth.w[k]);
+            s += fx_of(pos_weight(birth.w[k]));
             cdf[N + k] = s;
         }
         const fx_t total = s;
-        std::vector<uint64_t> u(N);
-        for (int k = 0; k < N; ++k)
+        std::vector<fx_t> spac(N + 1);
+        fx_t e = 0;
+        float e4[4];
+        for (int k = 0; k <= N; ++k)
         {
-            const u32x4 r = draw(seed, instance, TAG_RESAMPLE, epoch, (uint32_t)k);
-            u[k] = ((uint64_t)r.x << 32) | (uint64_t)r.y;
+            if ((k & 3) == 0)
+                resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
+            e += fx_from_f32(e4[k & 3]);
+            spac[k] = e;
         }
-        std::sort(u.begin(), u.end());
+        const Spacing sp = spacing_setup(spac[N], total);
         for (int k = 0; k < N; ++k)
         {
-            const fx_t rv = fx_scale_u64(total, u[k]);
-            // Binary search written out so it is the same search as the device's on any input.
-            int lo = 0, count = J;
-            while (count > 0)
-            {
-                const int step = count / 2;
-                const int mid = lo + step;
-                if (cdf[mid] < rv)
-                {
-                    lo = mid + 1;
-                    count -= step + 1;
-                }
-                else
-                {
-                    count = step;
-                }
-            }
-            ancestors[k] = lo < J ? lo : J - 1;  // only reachable with a negative total
+            const fx_t rv = spacing_draw(spac[k], sp);
+            // The CDF is non-decreasing, so any lower bound gives the same index; draws are < total
+            // (or 0 with total 0), so the index is always < J.
+            const int lo = (int)(std::lower_bound(cdf.begin(), cdf.end(), rv) - cdf.begin());
+            ancestors[k] = lo < J ? lo : J - 1;
         }
         const T joint_max = to_T_from_fx<T>(total);
         const T new_weight = joint_max / (T)N;

@@ -24,7 +24,7 @@ def test_status_strings_and_kernel_names():
     assert dogm_ctypes.lib.dogm_status_string(99) == b"unknown status"
     names = dogm_ctypes.DOGM.kernel_names()
     assert len(names) == dogm_ctypes.lib.dogm_kernel_count()
-    assert "predict" in names and "rs_bucket" in names
+    assert "predict" in names and "rs_merge" in names
     assert dogm_ctypes.lib.dogm_kernel_name(-1) is None
 
 
