@@ -45,14 +45,24 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, uint64_t active)
     return peers;
 }
 
-// Adds one to s_h[d] for every active lane; lanes with equal digits are combined first, so
-// nearly sorted keys (the usual input: resampled particles are in cell order) do not serialise on
-// one LDS address.
+// Adds one to s_h[d] for every active lane.  A wave whose keys share the digit (the usual case
+// for the upper digits: resampled particles arrive in cell order) adds once; otherwise lanes add
+// individually.
 __device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid)
 {
-    const uint64_t peers = match_digit(d, __ballot(valid));
-    if (valid && (peers & lanemask_lt()) == 0)
-        atomicAdd(&s_h[d], (unsigned)__popcll(peers));
+    const uint64_t active = __ballot(valid);
+    if (active == 0)
+        return;
+    const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
+    if ((__ballot(d == first) & active) == active)
+    {
+        if (valid && (active & lanemask_lt()) == 0)
+            atomicAdd(&s_h[d], (unsigned)__popcll(active));
+    }
+    else if (valid)
+    {
+        atomicAdd(&s_h[d], 1u);
+    }
 }
 
 // Histogram of every pass's digit over all keys.
