@@ -173,25 +173,45 @@ __device__ __forceinline__ void block_exclusive_scan_fx_n(const fx_t (&v)[NCH], 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Work-table lookup: the vehicle and local element base of this workgroup's tile of q units.
+// ---------------------------------------------------------------------------------------------
+struct TileRef
+{
+    int v;     // vehicle index in the step
+    int base;  // first local element of the tile
+};
+
+__device__ __forceinline__ TileRef tile_ref(const uint32_t* __restrict__ tab, int q, int tile)
+{
+    const uint32_t e = tab[(size_t)tile * q];
+    return TileRef{(int)(e >> TAB_VEH_SHIFT), (int)(e & TAB_UNIT_MASK) * UNIT};
+}
+
+// ---------------------------------------------------------------------------------------------
 // Exact tile-segmented reduction over a key-sorted array.
 //
-// Segments are runs of equal keys (cells).  A workgroup takes a tile of BLOCK * ITEMS consecutive
-// elements (ITEMS consecutive per thread) and computes the tile-relative inclusive prefix P of
-// every element with one plain exact block scan; the sum of a segment piece [s, e] is then
-// P[e] - P[s - 1].  Integer arithmetic makes the subtraction exact.  The start of the piece that
-// reaches a thread's first element is found with one ballot (last earlier lane of the wave with a
-// segment start) and a per-wave table for earlier waves.  Segments that start and end inside the
-// tile are finalised here; a segment that crosses tile boundaries leaves a head partial (its piece
-// in the tile where it ends) and tail partials (its pieces in the tiles it continues through),
-// which the fix-up kernel sums from the tile where the segment starts.
+// Segments are runs of equal keys (cells) inside one vehicle's particles.  A workgroup takes a
+// tile of BLOCK * ITEMS consecutive particles of one vehicle (ITEMS consecutive per thread) and
+// computes the tile-relative inclusive prefix P of every element with one plain exact block scan;
+// the sum of a segment piece [s, e] is then P[e] - P[s - 1].  Integer arithmetic makes the
+// subtraction exact.  The start of the piece that reaches a thread's first element is found with
+// one ballot (last earlier lane of the wave with a segment start) and a per-wave table for
+// earlier waves.  Segments that start and end inside the tile are finalised here; a segment that
+// crosses tile boundaries leaves a head partial (its piece in the tile where it ends) and tail
+// partials (its pieces in the tiles it continues through), which the fix-up kernel sums from the
+// tile where the segment starts.  A vehicle's tiles are consecutive in the table and its last
+// tile ends every segment, so no carry crosses vehicles.
 //
 // Op interface:
 //   static constexpr int NCH, ITEMS;                                    // ITEMS % 4 == 0
-//   __device__ void load(int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const;
-//   __device__ void finalize(int key, const fx_t (&s)[NCH]) const;     // exact sums of a segment
+//   __device__ const int* keys(const VehDesc& d) const;                 // cell index per particle
+//   __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS],
+//                        float (&v)[ITEMS][NCH]) const;
+//   __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[NCH]) const;
 // ---------------------------------------------------------------------------------------------
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restrict__ keys, int n, SegCarry carry)
+__global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const VehDesc* __restrict__ descs,
+                                                     const uint32_t* __restrict__ tab, SegCarry carry)
 {
     constexpr int NCH = Op::NCH;
     constexpr int IT = Op::ITEMS;
@@ -203,11 +223,22 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restric
     __shared__ int s_nq;
     __shared__ int s_qkey[QCAP];
     __shared__ fx_t s_qsum[QCAP][NCH];
-    if (threadIdx.x == 0)
-        s_nq = 0;
 
     const int tile = blockIdx.x;
-    const int tbase = tile * TILE;
+    const TileRef tr = tile_ref(tab, IT * BLOCK / UNIT, tile);
+    const VehDesc& d = descs[tr.v];
+    const int n = d.N;
+    const int tbase = tr.base;
+    if (tbase >= n)
+    {
+        // Padding tile past the vehicle's particles: no segment, no carry.
+        if (threadIdx.x == 0)
+            carry.flags[tile] = 0;
+        return;
+    }
+    if (threadIdx.x == 0)
+        s_nq = 0;
+    const int* keys = op.keys(d);
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const int p0 = tbase + threadIdx.x * IT;
@@ -219,7 +250,7 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restric
     const int knext = (p0 + IT < n) ? keys[p0 + IT] : -1;
 
     float vf[IT][NCH];
-    op.load(p0, cnt, key, vf);
+    op.load(d, p0, cnt, key, vf);
     // Thread-local inclusive prefix; the thread's last segment start and the prefix before it.
     fx_t pre[IT][NCH];
     int lh = -1;
@@ -331,7 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restric
                     }
                     else
                     {
-                        op.finalize(key[i], s);
+                        op.finalize(d, key[i], s);
                     }
                 }
                 else
@@ -363,14 +394,15 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const int* __restric
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
             s[c] = s_qsum[q][c];
-        op.finalize(s_qkey[q], s);
+        op.finalize(d, s_qkey[q], s);
     }
 }
 
 // Finalises segments that cross tile boundaries; one thread per tile, the owner is the tile in
 // which the segment starts.
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_segfix(Op op, const int* __restrict__ keys, int n, int ntiles, SegCarry carry)
+__global__ __launch_bounds__(BLOCK) void k_segfix(Op op, const VehDesc* __restrict__ descs,
+                                                  const uint32_t* __restrict__ tab, int ntiles, SegCarry carry)
 {
     constexpr int NCH = Op::NCH;
     constexpr int TILE = BLOCK * Op::ITEMS;
@@ -401,49 +433,29 @@ __global__ __launch_bounds__(BLOCK) void k_segfix(Op op, const int* __restrict__
         for (int c = 0; c < NCH; ++c)
             s[c] += carry.head_partial[(size_t)u * NCH + c];
     }
-    const int last = min((t + 1) * TILE, n) - 1;
-    op.finalize(keys[last], s);
+    const TileRef tr = tile_ref(tab, TILE / UNIT, t);
+    const VehDesc& d = descs[tr.v];
+    const int last = min(tr.base + TILE, d.N) - 1;
+    op.finalize(d, op.keys(d)[last], s);
 }
 
 template <class Op>
-inline void launch_segreduce(dogm_handle* h, int kid, int kid_fix, const Op& op, const int* keys, int n)
+inline void launch_segreduce(Engine* E, int kid, int kid_fix, const Op& op)
 {
-    if (n <= 0)
-        return;
     constexpr int TILE = BLOCK * Op::ITEMS;
-    const int tiles = div_up(n, TILE);
-    launch_begin(h, kid);
-    hipLaunchKernelGGL(k_segreduce<Op>, dim3(tiles), dim3(BLOCK), 0, h->stream, op, keys, n, h->seg);
-    launch_end(h, kid);
+    const int tiles = E->tiles(SP_P, TILE / UNIT);
+    if (tiles <= 0)
+        return;
+    launch_begin(E, kid);
+    hipLaunchKernelGGL(k_segreduce<Op>, dim3(tiles), dim3(BLOCK), 0, E->stream, op, E->d_desc, E->tab(SP_P), E->seg);
+    launch_end(E, kid);
     if (tiles > 1)
     {
-        launch_begin(h, kid_fix);
-        hipLaunchKernelGGL(k_segfix<Op>, dim3(div_up(tiles, BLOCK)), dim3(BLOCK), 0, h->stream, op, keys, n, tiles,
-                           h->seg);
-        launch_end(h, kid_fix);
+        launch_begin(E, kid_fix);
+        hipLaunchKernelGGL(k_segfix<Op>, dim3(div_up(tiles, BLOCK)), dim3(BLOCK), 0, E->stream, op, E->d_desc,
+                           E->tab(SP_P), tiles, E->seg);
+        launch_end(E, kid_fix);
     }
-}
-
-// Binary search shared with the checker: first index in [0, n) with a[index] >= v
-// (n if none).  Written out so that it is the same search on any input, sorted or not.
-__device__ __forceinline__ int lower_bound_fx(const fx_t* __restrict__ a, int n, fx_t v)
-{
-    int lo = 0, count = n;
-    while (count > 0)
-    {
-        const int step = count / 2;
-        const int mid = lo + step;
-        if (a[mid] < v)
-        {
-            lo = mid + 1;
-            count -= step + 1;
-        }
-        else
-        {
-            count = step;
-        }
-    }
-    return lo;
 }
 
 // First index j in [0, n) with a[j] > v (n if none).
@@ -468,19 +480,31 @@ __device__ __forceinline__ int upper_bound_i32(const int* __restrict__ a, int n,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Exact scans (reduce, scan of tile totals, rescan with carry)
+// Exact scans over one space, segmented by vehicle (reduce per tile, per-vehicle scan of the tile
+// totals, rescan with carry).  Prefixes are vehicle-relative.
 // ---------------------------------------------------------------------------------------------
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+constexpr int SCAN_Q = SCAN_TILE / UNIT;  // units per scan tile
 
-// Src interface: __device__ void load4(int p0, int cnt, float (&v)[4]) const;  (values p0 .. p0+cnt-1)
+// Src interface:
+//   __device__ bool active(const VehDesc& d) const;                   // vehicle takes part
+//   __device__ int len(const VehDesc& d) const;                       // elements of the vehicle
+//   __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const;
 template <class Src>
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, int n, fx_t* __restrict__ tile_sum)
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tab, fx_t* __restrict__ tile_sum)
 {
-    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    if (!src.active(d))
+        return;
+    const int n = src.len(d);
+    const int p0 = tr.base + threadIdx.x * SCAN_ITEMS;
     const int cnt = max(0, min(SCAN_ITEMS, n - p0));
-    float v[SCAN_ITEMS];
-    src.load4(p0, cnt, v);
+    float v[SCAN_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (cnt > 0)
+        src.load4(d, p0, cnt, v);
     fx_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
@@ -494,21 +518,30 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, int n, fx_t* __re
 
 enum ScanMode
 {
-    SCAN_ORDERS = 0,    // particle-order normalisation: scale = v / total
-    SCAN_RESAMPLE = 1,  // resampling: joint_max and new weight
-    SCAN_EXP = 2,       // exponential spacings: total into exp_total
+    SCAN_ORDERS_N = 0,  // first-frame placement: scale = N / total
+    SCAN_ORDERS_B = 1,  // birth placement: scale = B / total
+    SCAN_RESAMPLE = 2,  // resampling: joint_max and new weight
+    SCAN_EXP = 3,       // exponential spacings: total into exp_total
 };
 
-// Single workgroup: exclusive prefix of the tile totals (each thread scans a contiguous chunk),
-// optional inclusive tile ends, and the scalars derived from the grand total.
+// One workgroup per vehicle: exclusive prefix of the vehicle's tile totals (each thread scans a
+// contiguous chunk), optional inclusive tile ends, and the scalars derived from the vehicle's
+// total.  `init_only`: vehicles without first-frame initialisation are skipped.
 template <int Unused = 0>
-__global__ __launch_bounds__(BLOCK) void k_scan_top(const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
-                                                    fx_t* __restrict__ tile_incl, int ntiles, DevScalars* sc,
-                                                    int mode, int v)
+__global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ descs, int space,
+                                                    const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
+                                                    fx_t* __restrict__ tile_incl, DevScalars* __restrict__ scs,
+                                                    int mode, int init_only)
 {
+    const VehDesc& d = descs[blockIdx.x];
+    if (init_only && !d.init)
+        return;
+    DevScalars* sc = scs + blockIdx.x;
+    const int t0 = d.u0[space] / SCAN_Q;
+    const int ntiles = d.nu[space] / SCAN_Q;
     const int per = (ntiles + BLOCK - 1) / BLOCK;
-    const int b0 = min(ntiles, (int)threadIdx.x * per);
-    const int b1 = min(ntiles, b0 + per);
+    const int b0 = t0 + min(ntiles, (int)threadIdx.x * per);
+    const int b1 = t0 + min(ntiles, (int)threadIdx.x * per + per);
     fx_t s = 0;
     for (int t = b0; t < b1; ++t)
         s += tile_sum[t];
@@ -530,27 +563,41 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const fx_t* __restrict__ til
             return;
         }
         sc->total = carry;
-        if (mode == SCAN_ORDERS)
+        if (mode == SCAN_ORDERS_N || mode == SCAN_ORDERS_B)
         {
+            const int v = mode == SCAN_ORDERS_N ? d.N : d.B;
             sc->scale = carry != 0 ? order_scale(carry, v) : 0.0;
         }
         else
         {
             const float jm = fx_to_f32(carry);
             sc->joint_max = jm;
-            sc->new_weight = jm / (float)v;
+            sc->new_weight = jm / (float)d.N;
         }
     }
 }
 
-// Sink interface: __device__ void apply4(int p0, int cnt, const fx_t (&prefix)[4], const fx_t (&own)[4]) const;
+// Sink interface:
+//   __device__ void apply4(const VehDesc& d, const DevScalars* sc, int p0, int cnt,
+//                          const fx_t (&prefix)[4], const fx_t (&own)[4]) const;
 template <class Src, class Sink>
-__global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, int n, const fx_t* __restrict__ tile_pre)
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, const VehDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tab,
+                                                      const fx_t* __restrict__ tile_pre,
+                                                      const DevScalars* __restrict__ scs)
 {
-    const int p0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    if (!src.active(d))
+        return;
+    const int n = src.len(d);
+    if (tr.base >= n)
+        return;
+    const int p0 = tr.base + threadIdx.x * SCAN_ITEMS;
     const int cnt = max(0, min(SCAN_ITEMS, n - p0));
-    float f[SCAN_ITEMS];
-    src.load4(p0, cnt, f);
+    float f[SCAN_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (cnt > 0)
+        src.load4(d, p0, cnt, f);
     fx_t v[SCAN_ITEMS];
     fx_t s = 0;
 #pragma unroll
@@ -569,25 +616,28 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, int n,
         pre[i] = run;
     }
     if (cnt > 0)
-        sink.apply4(p0, cnt, pre, v);
+        sink.apply4(d, scs + tr.v, p0, cnt, pre, v);
 }
 
 template <class Src, class Sink>
-inline void run_exact_scan(dogm_handle* h, const Src& src, const Sink& sink, int n, int mode, int v, int kid_apply,
-                           fx_t* tile_incl = nullptr)
+inline void run_exact_scan(Engine* E, int space, const Src& src, const Sink& sink, int mode, int kid_apply,
+                           bool init_only, fx_t* tile_incl = nullptr)
 {
-    const int tiles = div_up(n, SCAN_TILE);
-    launch_begin(h, K_SCAN_TILES);
-    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, h->stream, src, n, h->scan_tile_sum);
-    launch_end(h, K_SCAN_TILES);
-    launch_begin(h, K_SCAN_TOP);
-    hipLaunchKernelGGL(k_scan_top<0>, dim3(1), dim3(BLOCK), 0, h->stream, h->scan_tile_sum, h->scan_tile_pre,
-                       tile_incl, tiles, h->scalars, mode, v);
-    launch_end(h, K_SCAN_TOP);
-    launch_begin(h, kid_apply);
-    hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, h->stream, src, sink, n,
-                       h->scan_tile_pre);
-    launch_end(h, kid_apply);
+    const int tiles = E->tiles(space, SCAN_Q);
+    if (tiles <= 0)
+        return;
+    launch_begin(E, K_SCAN_TILES);
+    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, E->stream, src, E->d_desc, E->tab(space),
+                       E->scan_tile_sum);
+    launch_end(E, K_SCAN_TILES);
+    launch_begin(E, K_SCAN_TOP);
+    hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, space, E->scan_tile_sum,
+                       E->scan_tile_pre, tile_incl, E->scalars, mode, init_only ? 1 : 0);
+    launch_end(E, K_SCAN_TOP);
+    launch_begin(E, kid_apply);
+    hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, E->stream, src, sink, E->d_desc,
+                       E->tab(space), E->scan_tile_pre, E->scalars);
+    launch_end(E, kid_apply);
 }
 
 }  // namespace dogm_hip

@@ -1,0 +1,569 @@
+// Step engine: per-vehicle allocation, step descriptors and work tables, shared scratch, the stage
+// sequence of dogm::DOGM::updateGrid (dogm/src/dogm.cu:115-131) over any list of vehicles, and
+// the optional per-kernel event timing.
+#include "dogm_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dogm_hip
+{
+
+const char* const kKernelNames[K_COUNT] = {
+    "predict",        "move_particles",  "shift_field",     "sort_hist",       "sort_upsweep",
+    "sort_scan",      "sort_downsweep",  "sort_gather",     "reset_bounds",    "seg_occ",
+    "seg_occ_fix",    "mass_update",     "normalize",       "seg_pers",        "seg_pers_fix",
+    "post_weights",   "empty_cells",     "scan_tiles",      "scan_top",        "birth_cells",
+    "birth_particles", "init_cells",     "init_particles",  "copy_masses",     "seg_moments",
+    "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
+    "unpack_grid",    "build_tables",
+};
+
+// ---- timing ------------------------------------------------------------------------------------
+
+static hipEvent_t take_event(Engine* E)
+{
+    if (!E->event_pool.empty())
+    {
+        hipEvent_t e = E->event_pool.back();
+        E->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+void launch_begin(Engine* E, int kid)
+{
+    if (!E->timing_on[kid])
+        return;
+    TimedLaunch t{kid, take_event(E), take_event(E)};
+    if (t.a)
+        (void)hipEventRecord(t.a, E->stream);
+    E->timed_pending.push_back(t);
+}
+
+void launch_end(Engine* E, int kid)
+{
+    if (!E->timing_on[kid] || E->timed_pending.empty())
+        return;
+    TimedLaunch& t = E->timed_pending.back();
+    if (t.b)
+        (void)hipEventRecord(t.b, E->stream);
+}
+
+void engine_collect_timing(Engine* E)
+{
+    for (auto& t : E->timed_pending)
+    {
+        float ms = 0.0f;
+        if (t.a && t.b && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
+        {
+            E->timing_ms[t.kid] += ms;
+            E->timing_n[t.kid] += 1;
+        }
+        if (t.a)
+            E->event_pool.push_back(t.a);
+        if (t.b)
+            E->event_pool.push_back(t.b);
+    }
+    E->timed_pending.clear();
+}
+
+// ---- allocation helpers ------------------------------------------------------------------------
+
+namespace
+{
+
+int set_err(std::string* why, int code, const std::string& msg)
+{
+    if (why)
+        *why = msg;
+    return code;
+}
+
+template <typename T>
+int dalloc(T** p, size_t count, const char* what, std::string* why, size_t* bytes = nullptr)
+{
+    *p = nullptr;
+    if (count == 0)
+        return DOGM_OK;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        return set_err(why, e == hipErrorOutOfMemory ? DOGM_ERR_OUT_OF_MEMORY : DOGM_ERR_HIP,
+                       std::string("allocating ") + what + ": " + hipGetErrorString(e));
+    if (bytes)
+        *bytes += count * sizeof(T);
+    return DOGM_OK;
+}
+
+// Grows a device buffer to hold `need` elements (contents are not kept).
+template <typename T>
+int grow(T** p, long long* cap, long long need, const char* what, std::string* why)
+{
+    if (need <= *cap)
+        return DOGM_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    int st = dalloc(p, (size_t)need, what, why);
+    if (st == DOGM_OK)
+        *cap = need;
+    return st;
+}
+
+int bits_for(long long x)  // number of bits needed to represent x >= 0
+{
+    int b = 0;
+    while (b < 62 && (1ll << b) <= x)
+        ++b;
+    return b;
+}
+
+long long units_for(long long len)
+{
+    const long long u = (len + UNIT - 1) / UNIT;
+    return (u + UNIT_ALIGN - 1) / UNIT_ALIGN * UNIT_ALIGN;
+}
+
+long long space_len(const Vehicle* v, int s)
+{
+    switch (s)
+    {
+    case SP_P: return v->N;
+    case SP_B: return v->B;
+    case SP_C: return v->C;
+    case SP_J: return (long long)v->N + v->B;
+    default: return (long long)v->N + 1;
+    }
+}
+
+void free_particles(DevParticles& p)
+{
+    (void)hipFree(p.state);
+    (void)hipFree(p.idx);
+    (void)hipFree(p.weight);
+    (void)hipFree(p.assoc);
+    p = DevParticles{};
+}
+
+int alloc_particles(DevParticles& p, int n, std::string* why, size_t* bytes)
+{
+    int st;
+    if ((st = dalloc(&p.state, n, "particle state", why, bytes)) ||
+        (st = dalloc(&p.idx, n, "particle index", why, bytes)) ||
+        (st = dalloc(&p.weight, n, "particle weight", why, bytes)) ||
+        (st = dalloc(&p.assoc, n, "particle flags", why, bytes)))
+        return st;
+    if (n > 0 && (hipMemset(p.state, 0, sizeof(float4) * n) != hipSuccess ||
+                  hipMemset(p.idx, 0, sizeof(int) * n) != hipSuccess ||
+                  hipMemset(p.weight, 0, sizeof(float) * n) != hipSuccess || hipMemset(p.assoc, 0, n) != hipSuccess))
+        return set_err(why, DOGM_ERR_HIP, "clearing particle buffers");
+    return DOGM_OK;
+}
+
+uint64_t next_uid()
+{
+    static uint64_t uid = 0;
+    return ++uid;
+}
+
+}  // namespace
+
+// ---- engine ------------------------------------------------------------------------------------
+
+int engine_init(Engine* E, std::string* why)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(why, DOGM_ERR_NO_DEVICE, "no HIP device");
+    if (hipGetDevice(&E->device) != hipSuccess)
+        return set_err(why, DOGM_ERR_NO_DEVICE, "hipGetDevice failed");
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess)
+        return set_err(why, DOGM_ERR_HIP, "hipStreamCreateWithFlags failed");
+    return DOGM_OK;
+}
+
+void engine_release(Engine* E)
+{
+    if (E->stream)
+        (void)hipStreamSynchronize(E->stream);
+    if (E->h_desc)
+        (void)hipHostFree(E->h_desc);
+    if (E->h_upre)
+        (void)hipHostFree(E->h_upre);
+    void* bufs[] = {E->d_desc,          E->d_upre,           E->d_tab,           E->scan_tile_sum, E->scan_tile_pre,
+                    E->cdf_tile_incl,   E->seg.flags,        E->seg.head_partial, E->seg.tail_partial,
+                    E->sort_thist,      E->sort_ghist,       E->scalars};
+    for (void* b : bufs)
+        (void)hipFree(b);
+    for (auto& t : E->timed_pending)
+    {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : E->event_pool)
+        (void)hipEventDestroy(e);
+    if (E->stream)
+        (void)hipStreamDestroy(E->stream);
+    E->stream = nullptr;
+}
+
+static int engine_reserve(Engine* E, int m, std::string* why)
+{
+    if (m > E->m_cap)
+    {
+        const int cap = std::max(m, 2 * E->m_cap);
+        if (E->h_desc)
+            (void)hipHostFree(E->h_desc);
+        if (E->h_upre)
+            (void)hipHostFree(E->h_upre);
+        (void)hipFree(E->d_desc);
+        (void)hipFree(E->d_upre);
+        (void)hipFree(E->sort_ghist);
+        (void)hipFree(E->scalars);
+        E->h_desc = nullptr;
+        E->h_upre = nullptr;
+        E->d_desc = nullptr;
+        E->d_upre = nullptr;
+        E->sort_ghist = nullptr;
+        E->scalars = nullptr;
+        E->m_cap = 0;
+        if (hipHostMalloc(reinterpret_cast<void**>(&E->h_desc), sizeof(VehDesc) * cap) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&E->h_upre), sizeof(int) * SP_COUNT * (cap + 1)) != hipSuccess)
+            return set_err(why, DOGM_ERR_OUT_OF_MEMORY, "allocating pinned step descriptors");
+        int st;
+        if ((st = dalloc(&E->d_desc, cap, "step descriptors", why)) ||
+            (st = dalloc(&E->d_upre, (size_t)SP_COUNT * (cap + 1), "step tables", why)) ||
+            (st = dalloc(&E->sort_ghist, (size_t)cap * 4 * 256, "sort histograms", why)) ||
+            (st = dalloc(&E->scalars, cap, "step scalars", why)))
+            return st;
+        if (hipMemset(E->scalars, 0, sizeof(DevScalars) * cap) != hipSuccess)
+            return set_err(why, DOGM_ERR_HIP, "clearing step scalars");
+        E->m_cap = cap;
+    }
+    return DOGM_OK;
+}
+
+int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
+{
+    std::string why;
+    if (m < 1 || m > MAX_STEP_VEHICLES)
+        return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT,
+                       "a step takes 1 to " + std::to_string(MAX_STEP_VEHICLES) + " vehicles");
+    int st = engine_reserve(E, m, &why);
+    if (st != DOGM_OK)
+        return set_err(&E->last_error, st, why);
+
+    bool same = (int)E->plan_veh.size() == m && E->m == m;
+    for (int i = 0; same && i < m; ++i)
+        same = E->plan_veh[i] == veh[i]->uid;
+    if (!same)
+    {
+        long long tot[SP_COUNT] = {};
+        int passes = 1;
+        for (int s = 0; s < SP_COUNT; ++s)
+        {
+            int* pre = E->h_upre + s * (m + 1);
+            pre[0] = 0;
+            for (int i = 0; i < m; ++i)
+            {
+                const long long u = units_for(space_len(veh[i], s));
+                if (tot[s] + u > (long long)INT32_MAX)
+                    return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT, "step too large for the work tables");
+                tot[s] += u;
+                pre[i + 1] = (int)tot[s];
+            }
+        }
+        for (int i = 0; i < m; ++i)
+            passes = std::max(passes, std::max(1, (bits_for(veh[i]->C - 1) + 7) / 8));
+        long long total = 0;
+        for (int s = 0; s < SP_COUNT; ++s)
+        {
+            E->units[s] = tot[s];
+            E->tab_off[s] = total;
+            total += tot[s];
+        }
+        const long long scan_tiles = std::max({tot[SP_C], tot[SP_J], tot[SP_E]}) / 4;
+        const long long seg_tiles = tot[SP_P] / 4;
+        const long long sort_tiles = tot[SP_P] / 8;
+        if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
+            return set_err(&E->last_error, st, why);
+        long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = E->scan_tiles_cap;
+        if ((st = grow(&E->scan_tile_sum, &c1, scan_tiles, "scan", &why)) ||
+            (st = grow(&E->scan_tile_pre, &c2, scan_tiles, "scan", &why)) ||
+            (st = grow(&E->cdf_tile_incl, &c3, scan_tiles, "cdf index", &why)))
+            return set_err(&E->last_error, st, why);
+        E->scan_tiles_cap = std::min({c1, c2, c3});
+        if (seg_tiles > E->seg_tiles_cap)
+        {
+            long long a = 0, b = 0, c = 0;
+            (void)hipFree(E->seg.flags);
+            (void)hipFree(E->seg.head_partial);
+            (void)hipFree(E->seg.tail_partial);
+            E->seg = SegCarry{};
+            E->seg_tiles_cap = 0;
+            if ((st = grow(&E->seg.flags, &a, seg_tiles, "segments", &why)) ||
+                (st = grow(&E->seg.head_partial, &b, seg_tiles * 5, "segments", &why)) ||
+                (st = grow(&E->seg.tail_partial, &c, seg_tiles * 5, "segments", &why)))
+                return set_err(&E->last_error, st, why);
+            E->seg_tiles_cap = seg_tiles;
+        }
+        if ((st = grow(&E->sort_thist, &E->sort_tiles_cap, sort_tiles * 256, "sort", &why)))
+            return set_err(&E->last_error, st, why);
+        E->passes = passes;
+        E->m = m;
+        E->plan_veh.assign(m, 0);
+        for (int i = 0; i < m; ++i)
+            E->plan_veh[i] = veh[i]->uid;
+        if (hipMemcpyAsync(E->d_upre, E->h_upre, sizeof(int) * SP_COUNT * (m + 1), hipMemcpyHostToDevice,
+                           E->stream) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "uploading step tables");
+        run_build_tables(E);
+    }
+
+    // Per-step descriptors.
+    for (int i = 0; i < m; ++i)
+    {
+        Vehicle* v = veh[i];
+        VehDesc& d = E->h_desc[i];
+        std::memset(&d, 0, sizeof(d));
+        d.gs = v->gs;
+        d.C = v->C;
+        d.N = v->N;
+        d.B = v->B;
+        d.seed = v->seed;
+        d.inst = v->instance;
+        uint32_t e = v->epoch;
+        d.init = (stages & ST_INIT) && v->init_pending ? 1 : 0;
+        d.ep_init = e;
+        e += d.init ? 1u : 0u;
+        d.ep_pred = e;
+        e += (stages & ST_PREDICT) ? 1u : 0u;
+        d.ep_birth = e;
+        e += (stages & ST_BIRTH) ? 1u : 0u;
+        d.ep_rs = e;
+        const dogm_params& p = v->params;
+        d.ps = p.persistence_prob;
+        d.sp = p.stddev_process_noise_position;
+        d.sv = p.stddev_process_noise_velocity;
+        d.pB = p.birth_prob;
+        d.sd_birth = p.stddev_velocity;
+        d.vmax = p.init_max_velocity;
+        d.dt = v->dt;
+        // freespace_discount^dt with std::pow(float, float) as in dogm.cu:291.
+        d.alpha = std::pow(p.freespace_discount, v->dt);
+        d.move = (stages & ST_PREDICT) && v->pend_part_shift ? 1 : 0;
+        d.mx = v->pend_px;
+        d.my = v->pend_py;
+        d.shifted = (stages & ST_OCC) && v->pend_grid_shift ? 1 : 0;
+        d.gx = v->pend_gx;
+        d.gy = v->pend_gy;
+        d.part[0] = v->part[0];
+        d.part[1] = v->part[1];
+        d.birth = v->birth;
+        d.start = v->grid.start;
+        d.end = v->grid.end;
+        for (int f = 0; f < GF_COUNT; ++f)
+            d.f[f] = v->grid.f[f];
+        d.free[0] = v->grid.f[GF_FREE];
+        d.free[1] = v->free_alt;
+        d.meas = v->meas;
+        d.weight_array = v->weight_array;
+        d.born = v->born;
+        d.m_raw = v->m_raw;
+        d.pend = v->pend;
+        d.ancestors = v->ancestors;
+        d.sort_keys[0] = v->sort_keys[0];
+        d.sort_keys[1] = v->sort_keys[1];
+        d.sort_vals[0] = v->sort_vals[0];
+        d.sort_vals[1] = v->sort_vals[1];
+        d.cdf = v->cdf;
+    }
+    // Table offsets per vehicle (units); the prefixes are those the tables were built from.
+    {
+        long long off[SP_COUNT] = {};
+        for (int i = 0; i < m; ++i)
+            for (int s = 0; s < SP_COUNT; ++s)
+            {
+                const long long u = units_for(space_len(veh[i], s));
+                E->h_desc[i].u0[s] = (int)off[s];
+                E->h_desc[i].nu[s] = (int)u;
+                off[s] += u;
+            }
+    }
+    if (hipMemcpyAsync(E->d_desc, E->h_desc, sizeof(VehDesc) * m, hipMemcpyHostToDevice, E->stream) != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, "uploading step descriptors");
+    return DOGM_OK;
+}
+
+int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
+{
+    int st = engine_plan(E, veh, m, stages);
+    if (st != DOGM_OK)
+        return st;
+    bool any_init = false;
+    for (int i = 0; i < m; ++i)
+        any_init = any_init || ((stages & ST_INIT) && veh[i]->init_pending);
+    int pph = 0;
+    if (any_init)
+        run_init_particles(E);
+    if (stages & ST_PREDICT)
+        run_predict(E, pph);
+    if (stages & ST_ASSIGN)
+    {
+        run_assignment(E, pph);
+        pph ^= 1;
+    }
+    if (stages & ST_OCC)
+        run_occupancy(E, pph);
+    if (stages & ST_PERS)
+        run_persistent(E, pph);
+    if (stages & ST_BIRTH)
+        run_births(E);
+    if (stages & ST_MOMENTS)
+        run_moments(E, pph);
+    if (stages & ST_RESAMPLE)
+    {
+        run_resampling(E, pph);
+        pph ^= 1;
+    }
+    // Host state: the descriptors already hold this step's buffers, so swaps take effect for the
+    // next step.
+    for (int i = 0; i < m; ++i)
+    {
+        Vehicle* v = veh[i];
+        const VehDesc& d = E->h_desc[i];
+        uint32_t e = d.ep_rs;
+        if (stages & ST_RESAMPLE)
+            ++e;
+        v->epoch = e;
+        if (d.init)
+            v->init_pending = false;
+        if (pph)
+            std::swap(v->part[0], v->part[1]);
+        if (stages & ST_OCC)
+            std::swap(v->grid.f[GF_FREE], v->free_alt);
+        if (stages & ST_PREDICT)
+            v->pend_part_shift = false;
+        if (stages & ST_OCC)
+            v->pend_grid_shift = false;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return DOGM_OK;
+}
+
+// ---- vehicles ----------------------------------------------------------------------------------
+
+int vehicle_validate(const dogm_params* p, std::string* why)
+{
+    if (!p)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT, "params is null");
+    if (!(p->size > 0.0f) || !(p->resolution > 0.0f) || !std::isfinite(p->size) || !std::isfinite(p->resolution))
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT, "size and resolution must be positive and finite");
+    const double gsd = (double)(p->size / p->resolution);
+    if (!(gsd >= 1.0) || gsd > 16383.0)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT, "grid_size = int(size / resolution) must be in [1, 16383]");
+    if (p->particle_count < 1 || p->new_born_particle_count < 0)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT,
+                       "particle_count must be >= 1 and new_born_particle_count >= 0");
+    if ((long long)p->particle_count + p->new_born_particle_count + 1 > MAX_SPACE_LEN)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT,
+                       "particle_count + new_born_particle_count must be below 2^28");
+    return DOGM_OK;
+}
+
+size_t vehicle_footprint(const dogm_params* p)
+{
+    const long long gs = (long long)(p->size / p->resolution);
+    const long long C = gs * gs, N = p->particle_count, B = p->new_born_particle_count;
+    const long long part = 16 + 4 + 4 + 1;
+    // particles x2 + births, 16 grid fields + free_alt + field_tmp + meas(16) + born + m_raw + pend,
+    // weights + ancestors + sort (16 B), CDF (16 B per joint entry).
+    return (size_t)(2 * N * part + B * part + C * 4 * (GF_COUNT + 2 + 2) + C * 16 + C * 12 + N * 8 + N * 16 +
+                    (N + B) * 16);
+}
+
+int vehicle_alloc(Engine* E, Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance)
+{
+    std::string why;
+    int st = vehicle_validate(params, &why);
+    if (st != DOGM_OK)
+        return set_err(&E->last_error, st, why);
+    v->params = *params;
+    v->gs = static_cast<int>(params->size / params->resolution);  // dogm.cu:34
+    v->C = v->gs * v->gs;
+    v->N = params->particle_count;
+    v->B = params->new_born_particle_count;
+    v->seed = seed;
+    v->instance = instance;
+    v->uid = next_uid();
+    const int C = v->C, N = v->N, B = v->B;
+    size_t* by = &v->bytes;
+
+    if ((st = alloc_particles(v->part[0], N, &why, by)) || (st = alloc_particles(v->part[1], N, &why, by)) ||
+        (st = alloc_particles(v->birth, B, &why, by)))
+        return set_err(&E->last_error, st, why);
+    if ((st = dalloc(&v->grid.start, C, "grid", &why, by)) || (st = dalloc(&v->grid.end, C, "grid", &why, by)))
+        return set_err(&E->last_error, st, why);
+    for (int f = 0; f < GF_COUNT; ++f)
+        if ((st = dalloc(&v->grid.f[f], C, "grid", &why, by)))
+            return set_err(&E->last_error, st, why);
+    if ((st = dalloc(&v->free_alt, C, "grid", &why, by)) || (st = dalloc(&v->field_tmp, C, "grid", &why, by)) ||
+        (st = dalloc(&v->meas, C, "measurement grid", &why, by)) ||
+        (st = dalloc(&v->weight_array, N, "weights", &why, by)) || (st = dalloc(&v->born, C, "born masses", &why, by)) ||
+        (st = dalloc(&v->m_raw, C, "cell masses", &why, by)) || (st = dalloc(&v->pend, C, "particle orders", &why, by)) ||
+        (st = dalloc(&v->ancestors, N, "ancestors", &why, by)) ||
+        (st = dalloc(&v->sort_keys[0], N, "sort", &why, by)) || (st = dalloc(&v->sort_keys[1], N, "sort", &why, by)) ||
+        (st = dalloc(&v->sort_vals[0], N, "sort", &why, by)) || (st = dalloc(&v->sort_vals[1], N, "sort", &why, by)) ||
+        (st = dalloc(&v->cdf, (size_t)N + B, "cdf", &why, by)))
+        return set_err(&E->last_error, st, why);
+
+    // initGridCellsKernel (init.cu:69-84): start = end = -1, masses 0, measurement (0, 0, 1, 1);
+    // the remaining grid fields and buffers start at zero.
+    if (hipMemset(v->grid.start, 0xff, sizeof(int) * C) != hipSuccess ||
+        hipMemset(v->grid.end, 0xff, sizeof(int) * C) != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, "clearing the grid");
+    for (int f = 0; f < GF_COUNT; ++f)
+        if (hipMemset(v->grid.f[f], 0, sizeof(float) * C) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "clearing the grid");
+    float* zbufs[] = {v->free_alt, v->weight_array, v->born, v->m_raw};
+    const size_t zcnt[] = {(size_t)C, (size_t)N, (size_t)C, (size_t)C};
+    for (int i = 0; i < 4; ++i)
+        if (zcnt[i] && hipMemset(zbufs[i], 0, sizeof(float) * zcnt[i]) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "clearing buffers");
+    if (hipMemset(v->ancestors, 0, sizeof(int) * N) != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, "clearing buffers");
+    {
+        std::vector<dogm_measurement_cell> m0((size_t)C, dogm_measurement_cell{0.0f, 0.0f, 1.0f, 1.0f});
+        if (hipMemcpy(v->meas, m0.data(), sizeof(dogm_measurement_cell) * C, hipMemcpyHostToDevice) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "initialising the measurement grid");
+    }
+    if (hipDeviceSynchronize() != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, "hipDeviceSynchronize");
+    return DOGM_OK;
+}
+
+void vehicle_free(Vehicle* v)
+{
+    free_particles(v->part[0]);
+    free_particles(v->part[1]);
+    free_particles(v->birth);
+    (void)hipFree(v->grid.start);
+    (void)hipFree(v->grid.end);
+    for (int f = 0; f < GF_COUNT; ++f)
+        (void)hipFree(v->grid.f[f]);
+    void* bufs[] = {v->free_alt, v->field_tmp,    v->meas,         v->weight_array, v->born,
+                    v->m_raw,    v->pend,         v->ancestors,    v->sort_keys[0], v->sort_keys[1],
+                    v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp};
+    for (void* b : bufs)
+        (void)hipFree(b);
+    *v = Vehicle{};
+}
+
+}  // namespace dogm_hip

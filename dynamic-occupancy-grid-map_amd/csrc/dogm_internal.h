@@ -1,4 +1,14 @@
-// Internal declarations of libdogm_hip.so: device buffers, handle state, kernel launchers.
+// Internal declarations of libdogm_hip.so.
+//
+// Execution model.  Every update runs as a *step* over a list of vehicles (independent grids,
+// each with its own particles, cells, parameters and Philox instance).  The plain dogm::DOGM
+// handle is a step over one vehicle; the batched and serving entry points put many vehicles of
+// uneven size into the same kernel launches.  A step uploads one VehDesc per vehicle (sizes,
+// parameters, RNG epochs, buffer pointers) and a work table per element space: entry u of a
+// space's table names the vehicle and the local 256-element unit that workgroup-unit u covers.
+// Every vehicle's units are padded to a multiple of 8, so a workgroup tile of up to 2048
+// elements never straddles two vehicles, and all indices inside a vehicle are its own (cell
+// bounds, ancestors, particle slots) exactly as in a single-grid run.
 #pragma once
 
 #include "../../include/dogm/dogm_c.h"
@@ -54,7 +64,63 @@ struct DevGrid
     float* f[GF_COUNT];
 };
 
-// Scalars produced and consumed on the device within one frame (no host round trips).
+// ---------------------------------------------------------------------------------------------
+// Work tables
+// ---------------------------------------------------------------------------------------------
+enum Space
+{
+    SP_P,  // particles (N per vehicle); also the N resampling draws
+    SP_B,  // birth particles (B)
+    SP_C,  // grid cells (C)
+    SP_J,  // joint resampling weights [persistent | birth] (N + B)
+    SP_E,  // exponentials of the resampling spacings (N + 1)
+    SP_COUNT
+};
+
+constexpr int UNIT = 256;          // elements per table entry
+constexpr int UNIT_ALIGN = 8;      // units per vehicle are padded to a multiple of this
+constexpr int TAB_VEH_SHIFT = 20;  // entry = vehicle << 20 | local unit
+constexpr uint32_t TAB_UNIT_MASK = (1u << TAB_VEH_SHIFT) - 1u;
+constexpr int MAX_STEP_VEHICLES = 1 << (32 - TAB_VEH_SHIFT);
+constexpr long long MAX_SPACE_LEN = (long long)UNIT << TAB_VEH_SHIFT;  // elements per vehicle and space
+
+// What the kernels read about one vehicle of a step.  The current particle buffer is part[0]
+// at the start of the step and the current free-mass buffer is free[0]; stages that write out of
+// place flip to index 1 and the host swaps the vehicle's buffers after the step.
+struct VehDesc
+{
+    int gs, C, N, B;
+    uint64_t seed;
+    uint32_t inst;
+    uint32_t ep_init, ep_pred, ep_birth, ep_rs;  // Philox epochs of this step's random stages
+    float ps, sp, sv, pB, sd_birth, vmax;        // dogm.h:29-59 (cell units, as passed)
+    float alpha;                                 // freespace_discount^dt (host std::pow)
+    float dt;
+    int init;                // first measurement: particles are initialised in this step
+    int move;                // pending particle shift (fused into predict)
+    float mx, my;
+    int shifted;             // pending grid shift (fused into the occupancy update)
+    int gx, gy;
+    int u0[SP_COUNT];        // first table unit of this vehicle per space
+    int nu[SP_COUNT];        // its units per space (multiple of UNIT_ALIGN)
+    DevParticles part[2];
+    DevParticles birth;
+    int* start;
+    int* end;
+    float* f[GF_COUNT];      // f[GF_FREE] is not used by step kernels: see free[]
+    float* free[2];
+    dogm_measurement_cell* meas;
+    float* weight_array;
+    float* born;
+    float* m_raw;
+    int* pend;
+    int* ancestors;
+    int* sort_keys[2];
+    int* sort_vals[2];
+    fx_t* cdf;
+};
+
+// Scalars produced and consumed on the device within one step, one per vehicle.
 struct DevScalars
 {
     fx_t total;        // last exact scan total (particle orders or resampling CDF)
@@ -63,13 +129,7 @@ struct DevScalars
     double scale;      // particle-order scale v / total
     float joint_max;   // resampling: f32 of the CDF total
     float new_weight;  // resampling: joint_max / N
-    int err;           // non-zero: a device-side check failed (see DevErr)
-    int pad[3];
-};
-
-enum DevErr
-{
-    DEV_ERR_SPIN_LIMIT = 1,  // a bounded wait on another workgroup gave up
+    int pad[2];
 };
 
 // Per-tile carry of the exact segmented reductions.
@@ -118,6 +178,7 @@ enum KernelId
     K_FILL,
     K_PACK_GRID,
     K_UNPACK_GRID,
+    K_BUILD_TABLES,
     K_COUNT
 };
 
@@ -129,108 +190,161 @@ struct TimedLaunch
     hipEvent_t a, b;
 };
 
-}  // namespace dogm_hip
+// ---------------------------------------------------------------------------------------------
+// Host-side objects
+// ---------------------------------------------------------------------------------------------
 
-// Handle state.  All device buffers are allocated once at creation.
-struct dogm_handle
+// Device buffers and host state of one vehicle.
+struct Vehicle
 {
     dogm_params params{};
     int gs = 0, C = 0, N = 0, B = 0;
     uint64_t seed = 0;
     uint32_t instance = 0;
     uint32_t epoch = 0;  // Philox epoch: incremented by every random-consuming stage
+    uint64_t uid = 0;    // unique per allocation (work tables are reused while the list is unchanged)
+    bool init_pending = false;  // initialise particles in the next step that includes ST_INIT
+    float dt = 0.0f;            // time step of the next step
 
-    int device = 0;
-    hipStream_t stream = nullptr;
-
-    dogm_hip::DevParticles part[2]{};  // part[cur] is the current particle array
-    int cur = 0;
-    dogm_hip::DevParticles birth{};
-    dogm_hip::DevGrid grid{};
-    float* free_alt = nullptr;    // second buffer for free_mass (occupancy reads one, writes the other)
-    float* field_tmp = nullptr;   // scratch for eager grid shifts
+    DevParticles part[2]{};  // part[0] is current
+    DevParticles birth{};
+    DevGrid grid{};          // grid.f[GF_FREE] is the current free mass
+    float* free_alt = nullptr;
+    float* field_tmp = nullptr;  // scratch for eager grid shifts
     dogm_measurement_cell* meas = nullptr;
     float* weight_array = nullptr;
     float* born = nullptr;
     float* m_raw = nullptr;  // per-cell predicted occupancy before the >1 clamp
     int* pend = nullptr;     // particle-order ends P_j (birth / initial placement)
     int* ancestors = nullptr;
-
-    // radix sort scratch
-    int sort_passes = 0;
-    int sort_tiles = 0;
     int* sort_keys[2] = {nullptr, nullptr};
     int* sort_vals[2] = {nullptr, nullptr};
-    unsigned* sort_ghist = nullptr;  // [passes * 256]
-    unsigned* sort_thist = nullptr;  // [256 * tiles]
-
-    // exact segmented reductions
-    int seg_tiles_max = 0;
-    dogm_hip::SegCarry seg{};
-
-    // exact scans
-    int scan_tiles_max = 0;
-    dogm_hip::fx_t* scan_tile_sum = nullptr;
-    dogm_hip::fx_t* scan_tile_pre = nullptr;
-    dogm_hip::fx_t* cdf = nullptr;
-    dogm_hip::fx_t* cdf_tile_incl = nullptr;  // inclusive CDF value at the end of every scan tile
-
-    dogm_hip::DevScalars* scalars = nullptr;
-    dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes
+    fx_t* cdf = nullptr;
+    dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes (lazy)
+    size_t bytes = 0;                   // device bytes held
 
     // pose (dogm/src/dogm.cu:161-203)
     bool first_pose = false;
     bool first_meas = false;
     float pos_x = 0.0f, pos_y = 0.0f, yaw = 0.0f;
-    // Shift decided by the pose update inside dogm_update_grid, applied inside predict (particles)
-    // and the occupancy update (free mass) of the same frame.
+    // Shift decided by a pose update inside an update step, applied inside predict (particles) and
+    // the occupancy update (free mass) of the same step.
     bool pend_part_shift = false;
     float pend_px = 0.0f, pend_py = 0.0f;
     bool pend_grid_shift = false;
     int pend_gx = 0, pend_gy = 0;
+};
+
+// Stage selection of one step.
+enum StageBits
+{
+    ST_ALL = 0xff,
+    ST_INIT = 1 << 0,
+    ST_PREDICT = 1 << 1,
+    ST_ASSIGN = 1 << 2,
+    ST_OCC = 1 << 3,
+    ST_PERS = 1 << 4,
+    ST_BIRTH = 1 << 5,
+    ST_MOMENTS = 1 << 6,
+    ST_RESAMPLE = 1 << 7,
+};
+
+// Execution context on one device and stream: step descriptors, work tables and the shared
+// scratch of the scans, segmented reductions and the sort.
+struct Engine
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // step plan
+    int m = 0;                       // vehicles of the current plan
+    std::vector<uint64_t> plan_veh;  // uids of the vehicles the tables were built for
+    long long units[SP_COUNT] = {};
+    long long tab_off[SP_COUNT] = {};
+    int passes = 1;
+    int m_cap = 0;
+    VehDesc* h_desc = nullptr;  // pinned staging
+    VehDesc* d_desc = nullptr;
+    int* h_upre = nullptr;      // pinned: per space, m + 1 unit prefixes
+    int* d_upre = nullptr;
+    long long tab_cap = 0;
+    uint32_t* d_tab = nullptr;
+
+    // scratch sized for the largest plan so far
+    long long scan_tiles_cap = 0;
+    fx_t* scan_tile_sum = nullptr;
+    fx_t* scan_tile_pre = nullptr;
+    fx_t* cdf_tile_incl = nullptr;
+    long long seg_tiles_cap = 0;
+    SegCarry seg{};
+    long long sort_tiles_cap = 0;
+    unsigned* sort_thist = nullptr;  // [256 * sort tiles]
+    int ghist_cap = 0;
+    unsigned* sort_ghist = nullptr;  // [m * 4 * 256]
+    int sc_cap = 0;
+    DevScalars* scalars = nullptr;   // [m]
 
     // kernel timing
-    bool timing_on[dogm_hip::K_COUNT] = {};
-    double timing_ms[dogm_hip::K_COUNT] = {};
-    long long timing_n[dogm_hip::K_COUNT] = {};
-    std::vector<dogm_hip::TimedLaunch> timed_pending;
+    bool timing_on[K_COUNT] = {};
+    double timing_ms[K_COUNT] = {};
+    long long timing_n[K_COUNT] = {};
+    std::vector<TimedLaunch> timed_pending;
     std::vector<hipEvent_t> event_pool;
 
     std::string last_error;
+
+    const uint32_t* tab(int s) const { return d_tab + tab_off[s]; }
+    int tiles(int s, int q) const { return (int)(units[s] / q); }
 };
 
-namespace dogm_hip
-{
-
-// Launch helpers (dogm_capi.hip): open / close an event bracket for kernel `kid` when timed.
-void launch_begin(dogm_handle* h, int kid);
-void launch_end(dogm_handle* h, int kid);
+// ---- launch helpers (dogm_engine.hip) ----
+void launch_begin(Engine* E, int kid);
+void launch_end(Engine* E, int kid);
 
 inline int div_up(long long a, long long b)
 {
     return (int)((a + b - 1) / b);
 }
 
-// ---- kernel launchers (dogm_kernels.hip) ----
-void run_fill_u32(dogm_handle* h, unsigned* p, unsigned v, long long n);
-void run_fill_i32(dogm_handle* h, int* p, int v, long long n);
-void run_move_particles(dogm_handle* h, float mx, float my);
-void run_shift_field(dogm_handle* h, const void* src, void* dst, int ix, int iy);
-void run_predict(dogm_handle* h, float dt, bool move, float mx, float my);
-void run_reset_bounds(dogm_handle* h);
-void run_sort_gather(dogm_handle* h, const int* skeys, const int* perm);
-void run_occupancy(dogm_handle* h, float alpha, bool shifted, int ix, int iy);
-void run_persistent(dogm_handle* h);
-void run_births(dogm_handle* h);
-void run_init_particles(dogm_handle* h);
-void run_moments(dogm_handle* h);
-void run_resampling(dogm_handle* h);
-void run_pack_grid(dogm_handle* h);
-void run_unpack_grid(dogm_handle* h);
+// ---- engine (dogm_engine.hip) ----
+int engine_init(Engine* E, std::string* why);
+void engine_release(Engine* E);
+// Builds descriptors and tables for `veh` (reusing the tables while the vehicle list is
+// unchanged) and uploads the descriptors; the stage set fixes the per-step fields (epochs, shifts).
+int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages);
+// Runs the selected stages of one step on `veh` (asynchronous on E->stream) and updates the
+// vehicles' host state (epochs, buffer swaps, pending shifts).
+int engine_run(Engine* E, Vehicle* const* veh, int m, int stages);
+void engine_collect_timing(Engine* E);
+int vehicle_validate(const dogm_params* p, std::string* why);
+size_t vehicle_footprint(const dogm_params* p);  // device bytes vehicle_alloc takes
+int vehicle_alloc(Engine* E, Vehicle* v, const dogm_params* p, uint64_t seed, uint32_t instance);
+void vehicle_free(Vehicle* v);
 
-// ---- radix sort (dogm_sort.hip) ----
-// Stable ascending sort of h->part[cur].idx; returns device pointers to the sorted keys and the
-// permutation (source slot of every sorted position).
-void run_sort(dogm_handle* h, const int** skeys, const int** perm);
+// ---- stage launchers (dogm_kernels.hip, dogm_sort.hip, dogm_resample.hip) ----
+// `pph` is the index (0 / 1) of the current particle buffer of the step's descriptors.
+void run_fill_u32(Engine* E, unsigned* p, unsigned v, long long n);
+void run_build_tables(Engine* E);
+void run_init_particles(Engine* E);                // pph 0
+void run_predict(Engine* E, int pph);
+void run_assignment(Engine* E, int pph);           // reads pph, writes pph ^ 1
+void run_occupancy(Engine* E, int pph);            // reads free[0], writes free[1]
+void run_persistent(Engine* E, int pph);
+void run_births(Engine* E);
+void run_moments(Engine* E, int pph);
+void run_resampling(Engine* E, int pph);           // reads pph, writes pph ^ 1
+void run_sort(Engine* E, int pph);                 // per vehicle: sorted keys / permutation
+// Single-vehicle kernels (no tables): eager ego shift and grid packing.
+void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
+void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);
+void run_pack_grid(Engine* E, const Vehicle* v);
+void run_unpack_grid(Engine* E, const Vehicle* v);
 
 }  // namespace dogm_hip
+
+// The plain handle: one engine, one vehicle.
+struct dogm_handle
+{
+    dogm_hip::Engine eng;
+    dogm_hip::Vehicle veh;
+};

@@ -1,9 +1,10 @@
 // Stage kernels of the DOGM particle-filter update on gfx950.
 //
-// Layout in HBM: particles structure-of-arrays (float4 state, int cell index, float weight,
-// uint8 association), grid cells structure-of-arrays (one array per field), measurement cells as
-// given by the caller (16-byte AoS).  Every per-cell sum is an exact 128-bit fixed-point
-// segmented reduction over the cell-sorted particles (dogm_device.h).
+// Layout in HBM, per vehicle: particles structure-of-arrays (float4 state, int cell index, float
+// weight, uint8 association), grid cells structure-of-arrays (one array per field), measurement
+// cells as given by the caller (16-byte AoS).  Every per-cell sum is an exact 128-bit fixed-point
+// segmented reduction over the cell-sorted particles (dogm_device.h).  Step kernels find their
+// vehicle and local tile through the step's work tables (dogm_internal.h).
 #include "dogm_device.h"
 
 namespace dogm_hip
@@ -19,22 +20,75 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(unsigned* __restrict__ p, un
         p[i] = v;
 }
 
-void run_fill_u32(dogm_handle* h, unsigned* p, unsigned v, long long n)
+void run_fill_u32(Engine* E, unsigned* p, unsigned v, long long n)
 {
     if (n <= 0)
         return;
-    launch_begin(h, K_FILL);
-    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, h->stream, p, v, n);
-    launch_end(h, K_FILL);
+    launch_begin(E, K_FILL);
+    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, E->stream, p, v, n);
+    launch_end(E, K_FILL);
 }
 
-void run_fill_i32(dogm_handle* h, int* p, int v, long long n)
+struct TabArgs
 {
-    run_fill_u32(h, reinterpret_cast<unsigned*>(p), (unsigned)v, n);
+    long long units[SP_COUNT];
+};
+
+// Expands the per-space unit prefixes of the step's vehicles (upre[s * (m + 1) + v]) into the
+// work tables: entry i of space s = vehicle << 20 | local unit.
+__global__ __launch_bounds__(BLOCK) void k_build_tables(const int* __restrict__ upre, int m, TabArgs a,
+                                                        uint32_t* __restrict__ tab, long long total)
+{
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= total)
+        return;
+    int s = 0;
+    long long u = i;
+    while (s < SP_COUNT - 1 && u >= a.units[s])
+    {
+        u -= a.units[s];
+        ++s;
+    }
+    const int* pre = upre + s * (m + 1);
+    // last v in [0, m) with pre[v] <= u
+    int lo = 0, count = m;
+    while (count > 0)
+    {
+        const int step = count / 2;
+        if (pre[lo + step + 1] <= u)
+        {
+            lo += step + 1;
+            count -= step + 1;
+        }
+        else
+        {
+            count = step;
+        }
+    }
+    tab[i] = ((uint32_t)lo << TAB_VEH_SHIFT) | (uint32_t)(u - pre[lo]);
+}
+
+void run_build_tables(Engine* E)
+{
+    TabArgs a{};
+    long long total = 0;
+    for (int s = 0; s < SP_COUNT; ++s)
+    {
+        a.units[s] = E->units[s];
+        total += E->units[s];
+    }
+    if (total <= 0)
+        return;
+    launch_begin(E, K_BUILD_TABLES);
+    hipLaunchKernelGGL(k_build_tables, dim3(div_up(total, BLOCK)), dim3(BLOCK), 0, E->stream, E->d_upre, E->m, a,
+                       E->d_tab, total);
+    launch_end(E, K_BUILD_TABLES);
 }
 
 // ---------------------------------------------------------------------------------------------
-// Ego-motion compensation (dogm/src/kernel/ego_motion_compensation.cu:16-43)
+// Ego-motion compensation (dogm/src/kernel/ego_motion_compensation.cu:16-43), eager form for the
+// single-vehicle pose update.  Inside an update step the shift is fused into predict and the
+// occupancy update.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_move_particles(DevParticles P, int n, float mx, float my)
 {
@@ -47,12 +101,12 @@ __global__ __launch_bounds__(BLOCK) void k_move_particles(DevParticles P, int n,
     P.state[i] = s;
 }
 
-void run_move_particles(dogm_handle* h, float mx, float my)
+void run_move_particles(Engine* E, const Vehicle* v, float mx, float my)
 {
-    launch_begin(h, K_MOVE_PARTICLES);
-    hipLaunchKernelGGL(k_move_particles, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->part[h->cur],
-                       h->N, mx, my);
-    launch_end(h, K_MOVE_PARTICLES);
+    launch_begin(E, K_MOVE_PARTICLES);
+    hipLaunchKernelGGL(k_move_particles, dim3(div_up(v->N, BLOCK)), dim3(BLOCK), 0, E->stream, v->part[0], v->N, mx,
+                       my);
+    launch_end(E, K_MOVE_PARTICLES);
 }
 
 // new[x, y] = old[x + ix, y + iy] when 0 < x + ix < gs and 0 < y + iy < gs (strict lower bound,
@@ -69,16 +123,16 @@ __global__ __launch_bounds__(BLOCK) void k_shift_field(const uint32_t* __restric
     dst[c] = in ? src[sx + gs * sy] : 0u;
 }
 
-void run_shift_field(dogm_handle* h, const void* src, void* dst, int ix, int iy)
+void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy)
 {
-    launch_begin(h, K_SHIFT_FIELD);
-    hipLaunchKernelGGL(k_shift_field, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream,
-                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), h->gs, ix, iy);
-    launch_end(h, K_SHIFT_FIELD);
+    launch_begin(E, K_SHIFT_FIELD);
+    hipLaunchKernelGGL(k_shift_field, dim3(div_up(v->C, BLOCK)), dim3(BLOCK), 0, E->stream,
+                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), v->gs, ix, iy);
+    launch_end(E, K_SHIFT_FIELD);
 }
 
 // ---------------------------------------------------------------------------------------------
-// Prediction (dogm/src/kernel/predict.cu:16-52), with the pending ego shift of the frame fused in
+// Prediction (dogm/src/kernel/predict.cu:16-52), with the pending ego shift of the step fused in
 // (moveParticlesKernel runs right before it in the reference).
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ int clamp_cell(float x, int gs)
@@ -90,27 +144,31 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
     return 0;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_predict(DevParticles P, int n, float dt, float sp, float sv, float ps,
-                                                   int gs, uint64_t seed, uint32_t inst, uint32_t epoch, int move,
-                                                   float mx, float my)
+__global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
+                                                   int pph)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int i = tr.base + threadIdx.x;
+    if (i >= d.N)
         return;
+    const DevParticles P = d.part[pph];
+    const int gs = d.gs;
+    const float dt = d.dt;
     float4 s = P.state[i];
     const float w_in = P.weight[i];
-    if (move)
+    if (d.move)
     {
-        s.x = s.x - mx;
-        s.y = s.y - my;
+        s.x = s.x - d.mx;
+        s.y = s.y - d.my;
     }
-    const u32x4 r = draw(seed, inst, TAG_PREDICT, epoch, (uint32_t)i);
+    const u32x4 r = draw(d.seed, d.inst, TAG_PREDICT, d.ep_pred, (uint32_t)i);
     float n0, n1, n2, n3;
     box_muller(r.x, r.y, &n0, &n1);
     box_muller(r.z, r.w, &n2, &n3);
     const float zero = 0.0f, one = 1.0f;
-    const float npx = normal_scale(n0, zero, sp), npy = normal_scale(n1, zero, sp);
-    const float nvx = normal_scale(n2, zero, sv), nvy = normal_scale(n3, zero, sv);
+    const float npx = normal_scale(n0, zero, d.sp), npy = normal_scale(n1, zero, d.sp);
+    const float nvx = normal_scale(n2, zero, d.sv), nvy = normal_scale(n3, zero, d.sv);
     // glm mat4 * vec4 association ((c0 x + c1 y) + (c2 vx + c3 vy)) of the transposed constant
     // velocity matrix (dogm.cu:246-253).
     const float sx = (one * s.x + zero * s.y) + (dt * s.z + zero * s.w);
@@ -122,7 +180,7 @@ __global__ __launch_bounds__(BLOCK) void k_predict(DevParticles P, int n, float 
     o.y = sy + npy;
     o.z = svx + nvx;
     o.w = svy + nvy;
-    float w = ps * w_in;
+    float w = d.ps * w_in;
     if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
         w = zero;
     P.state[i] = o;
@@ -130,69 +188,69 @@ __global__ __launch_bounds__(BLOCK) void k_predict(DevParticles P, int n, float 
     P.idx[i] = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
 }
 
-void run_predict(dogm_handle* h, float dt, bool move, float mx, float my)
+void run_predict(Engine* E, int pph)
 {
-    const dogm_params& p = h->params;
-    launch_begin(h, K_PREDICT);
-    hipLaunchKernelGGL(k_predict, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->part[h->cur], h->N, dt,
-                       p.stddev_process_noise_position, p.stddev_process_noise_velocity, p.persistence_prob, h->gs,
-                       h->seed, h->instance, h->epoch, move ? 1 : 0, mx, my);
-    launch_end(h, K_PREDICT);
-    ++h->epoch;
+    const int blocks = E->tiles(SP_P, 1);
+    if (blocks <= 0)
+        return;
+    launch_begin(E, K_PREDICT);
+    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph);
+    launch_end(E, K_PREDICT);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Assignment (dogm/src/dogm.cu:262-281, init.cu:86-93, particle_to_grid.cu:16-44)
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_reset_bounds(int* __restrict__ start, int* __restrict__ end, int C)
+__global__ __launch_bounds__(BLOCK) void k_reset_bounds(const VehDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ tab)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c < C)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + threadIdx.x;
+    if (c < d.C)
     {
-        start[c] = -1;
-        end[c] = -1;
+        d.start[c] = -1;
+        d.end[c] = -1;
     }
 }
 
-void run_reset_bounds(dogm_handle* h)
-{
-    launch_begin(h, K_RESET_BOUNDS);
-    hipLaunchKernelGGL(k_reset_bounds, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid.start,
-                       h->grid.end, h->C);
-    launch_end(h, K_RESET_BOUNDS);
-}
-
 // Gathers the payload into sorted order and records first/last occurrence of every cell.
-__global__ __launch_bounds__(BLOCK) void k_sort_gather(DevParticles src, DevParticles dst,
-                                                       const int* __restrict__ skeys, const int* __restrict__ perm,
-                                                       float* __restrict__ weight_array, int* __restrict__ start,
-                                                       int* __restrict__ end, int n)
+__global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab, int pph, int fin)
 {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int k = tr.base + threadIdx.x;
+    const int n = d.N;
     if (k >= n)
         return;
-    const int o = perm[k];
+    const DevParticles src = d.part[pph];
+    const DevParticles dst = d.part[pph ^ 1];
+    const int* skeys = d.sort_keys[fin];
+    const int o = d.sort_vals[fin][k];
     const int key = skeys[k];
     const float w = src.weight[o];
     dst.state[k] = src.state[o];
     dst.weight[k] = w;
     dst.assoc[k] = src.assoc[o];
     dst.idx[k] = key;
-    weight_array[k] = w;
+    d.weight_array[k] = w;
     if (k == 0 || skeys[k - 1] != key)
-        start[key] = k;
+        d.start[key] = k;
     if (k == n - 1 || skeys[k + 1] != key)
-        end[key] = k;
+        d.end[key] = k;
 }
 
-void run_sort_gather(dogm_handle* h, const int* skeys, const int* perm)
+void run_assignment(Engine* E, int pph)
 {
-    const int other = h->cur ^ 1;
-    launch_begin(h, K_SORT_GATHER);
-    hipLaunchKernelGGL(k_sort_gather, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->part[h->cur],
-                       h->part[other], skeys, perm, h->weight_array, h->grid.start, h->grid.end, h->N);
-    launch_end(h, K_SORT_GATHER);
-    h->cur = other;
+    launch_begin(E, K_RESET_BOUNDS);
+    hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
+    launch_end(E, K_RESET_BOUNDS);
+    run_sort(E, pph);
+    launch_begin(E, K_SORT_GATHER);
+    hipLaunchKernelGGL(k_sort_gather, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
+                       pph, (E->passes - 1) & 1);
+    launch_end(E, K_SORT_GATHER);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -202,48 +260,49 @@ struct OccOp
 {
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
-    const float* weight_array;
-    float* m_raw;
-    __device__ void load(int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
+    int pph;
+    __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];
-        load_run<ITEMS>(weight_array, p0, cnt, w);
+        load_run<ITEMS>(d.weight_array, p0, cnt, w);
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
             v[i][0] = w[i];
     }
-    __device__ void finalize(int key, const fx_t (&s)[1]) const { m_raw[key] = fx_to_f32(s[0]); }
+    __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[1]) const { d.m_raw[key] = fx_to_f32(s[0]); }
 };
 
-__global__ __launch_bounds__(BLOCK) void k_mass_update(DevGrid G, const float* __restrict__ free_src,
-                                                       float* __restrict__ free_dst,
-                                                       const dogm_measurement_cell* __restrict__ meas,
-                                                       const float* __restrict__ m_raw, float* __restrict__ born,
-                                                       int C, int gs, float alpha, float pB, int shifted, int ix,
-                                                       int iy)
+__global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= C)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + threadIdx.x;
+    if (c >= d.C)
         return;
+    const int gs = d.gs;
+    const float* free_src = d.free[0];
     const float one = 1.0f;
     float m_occ_pred = 0.0f;
-    if (G.start[c] != -1)
-        m_occ_pred = m_raw[c];
+    if (d.start[c] != -1)
+        m_occ_pred = d.m_raw[c];
     if (m_occ_pred > one)
         m_occ_pred = one;
     float fprev;
-    if (shifted)
+    if (d.shifted)
     {
         const int x = c % gs, y = c / gs;
-        const int sx = x + ix, sy = y + iy;
+        const int sx = x + d.gx, sy = y + d.gy;
         fprev = (sx > 0 && sx < gs && sy > 0 && sy < gs) ? free_src[sx + gs * sy] : 0.0f;
     }
     else
     {
         fprev = free_src[c];
     }
-    const dogm_measurement_cell mc = meas[c];
-    const float m_free_pred = fminf(alpha * fprev, one - m_occ_pred);
+    const dogm_measurement_cell mc = d.meas[c];
+    const float pB = d.pB;
+    const float m_free_pred = fminf(d.alpha * fprev, one - m_occ_pred);
     const float unknown_pred = one - m_occ_pred - m_free_pred;
     const float meas_unknown = one - mc.free_mass - mc.occ_mass;
     const float K = m_free_pred * mc.occ_mass + m_occ_pred * mc.free_mass;
@@ -253,48 +312,43 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(DevGrid G, const float* _
         (m_free_pred * meas_unknown + unknown_pred * mc.free_mass + m_free_pred * mc.free_mass) / (one - K);
     const float rho_b = (occ_up * pB * (one - m_occ_pred)) / (m_occ_pred + pB * (one - m_occ_pred));
     const float rho_p = occ_up - rho_b;
-    born[c] = rho_b;
-    G.f[GF_PERS][c] = rho_p;
-    G.f[GF_NEW_BORN][c] = rho_b;
-    free_dst[c] = free_up;
-    G.f[GF_OCC][c] = occ_up;
-    G.f[GF_PRED][c] = m_occ_pred;
+    d.born[c] = rho_b;
+    d.f[GF_PERS][c] = rho_p;
+    d.f[GF_NEW_BORN][c] = rho_b;
+    d.free[1][c] = free_up;
+    d.f[GF_OCC][c] = occ_up;
+    d.f[GF_PRED][c] = m_occ_pred;
 }
 
 // normalize_weights (mass_update.cu:51-59) for cells whose predicted occupancy exceeds 1.
-__global__ __launch_bounds__(BLOCK) void k_normalize(DevParticles P, float* __restrict__ weight_array,
-                                                     const float* __restrict__ m_raw, int n)
+__global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__ descs,
+                                                     const uint32_t* __restrict__ tab, int pph)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int i = tr.base + threadIdx.x;
+    if (i >= d.N)
         return;
-    const float m = m_raw[P.idx[i]];
+    const DevParticles P = d.part[pph];
+    const float m = d.m_raw[P.idx[i]];
     if (m > 1.0f)
     {
-        const float w = weight_array[i] / m;
-        weight_array[i] = w;
+        const float w = d.weight_array[i] / m;
+        d.weight_array[i] = w;
         P.weight[i] = w;
     }
 }
 
-void run_occupancy(dogm_handle* h, float alpha, bool shifted, int ix, int iy)
+void run_occupancy(Engine* E, int pph)
 {
-    const DevParticles P = h->part[h->cur];
-    OccOp op{h->weight_array, h->m_raw};
-    launch_segreduce(h, K_SEG_OCC, K_SEG_OCC_FIX, op, P.idx, h->N);
-    float* free_src = h->grid.f[GF_FREE];
-    float* free_dst = h->free_alt;
-    launch_begin(h, K_MASS_UPDATE);
-    hipLaunchKernelGGL(k_mass_update, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid, free_src,
-                       free_dst, h->meas, h->m_raw, h->born, h->C, h->gs, alpha, h->params.birth_prob,
-                       shifted ? 1 : 0, ix, iy);
-    launch_end(h, K_MASS_UPDATE);
-    h->grid.f[GF_FREE] = free_dst;
-    h->free_alt = free_src;
-    launch_begin(h, K_NORMALIZE);
-    hipLaunchKernelGGL(k_normalize, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, P, h->weight_array,
-                       h->m_raw, h->N);
-    launch_end(h, K_NORMALIZE);
+    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph});
+    launch_begin(E, K_MASS_UPDATE);
+    hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
+    launch_end(E, K_MASS_UPDATE);
+    launch_begin(E, K_NORMALIZE);
+    hipLaunchKernelGGL(k_normalize, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
+                       pph);
+    launch_end(E, K_NORMALIZE);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -305,81 +359,77 @@ struct PersOp
 {
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
-    const float* weight;
-    float* weight_array;
-    const dogm_measurement_cell* meas;
-    DevGrid G;
-    __device__ void load(int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
+    int pph;
+    __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS], lw[ITEMS];
-        load_run<ITEMS>(weight, p0, cnt, w);
+        load_run<ITEMS>(d.part[pph].weight, p0, cnt, w);
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
         {
-            lw[i] = i < cnt ? meas[key[i]].likelihood * w[i] : 0.0f;
+            lw[i] = i < cnt ? d.meas[key[i]].likelihood * w[i] : 0.0f;
             v[i][0] = lw[i];
         }
-        store_run<ITEMS>(weight_array, p0, cnt, lw);
+        store_run<ITEMS>(d.weight_array, p0, cnt, lw);
     }
-    __device__ void finalize(int key, const fx_t (&s)[1]) const
+    __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[1]) const
     {
         const float acc = fx_to_f32(s[0]);
-        const float rho_p = G.f[GF_PERS][key];
-        G.f[GF_MU_A][key] = acc > 0.0f ? rho_p / acc : 0.0f;
-        const float pred = G.f[GF_PRED][key];
-        G.f[GF_MU_UA][key] = pred > 0.0f ? rho_p / pred : 0.0f;
+        const float rho_p = d.f[GF_PERS][key];
+        d.f[GF_MU_A][key] = acc > 0.0f ? rho_p / acc : 0.0f;
+        const float pred = d.f[GF_PRED][key];
+        d.f[GF_MU_UA][key] = pred > 0.0f ? rho_p / pred : 0.0f;
     }
 };
 
-// Writes 0 into up to five fields of every cell without particles.
-__global__ __launch_bounds__(BLOCK) void k_empty_cells(const int* __restrict__ start, float* f0, float* f1,
-                                                       float* f2, float* f3, float* f4, int nf, int C)
+// Writes 0 into fields [f_lo, f_hi] of every cell without particles.
+__global__ __launch_bounds__(BLOCK) void k_empty_cells(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab, int f_lo, int f_hi)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= C || start[c] != -1)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + threadIdx.x;
+    if (c >= d.C || d.start[c] != -1)
         return;
-    f0[c] = 0.0f;
-    if (nf > 1)
-        f1[c] = 0.0f;
-    if (nf > 2)
-        f2[c] = 0.0f;
-    if (nf > 3)
-        f3[c] = 0.0f;
-    if (nf > 4)
-        f4[c] = 0.0f;
+    for (int f = f_lo; f <= f_hi; ++f)
+        d.f[f][c] = 0.0f;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_post_weights(DevParticles P, float* __restrict__ weight_array,
-                                                        const dogm_measurement_cell* __restrict__ meas, DevGrid G,
-                                                        int n)
+__global__ __launch_bounds__(BLOCK) void k_post_weights(const VehDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ tab, int pph)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int i = tr.base + threadIdx.x;
+    if (i >= d.N)
         return;
+    const DevParticles P = d.part[pph];
     const int c = P.idx[i];
-    const float pA = meas[c].p_A;
-    weight_array[i] = pA * G.f[GF_MU_A][c] * weight_array[i] + (1.0f - pA) * G.f[GF_MU_UA][c] * P.weight[i];
+    const float pA = d.meas[c].p_A;
+    d.weight_array[i] = pA * d.f[GF_MU_A][c] * d.weight_array[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * P.weight[i];
 }
 
-void run_persistent(dogm_handle* h)
+void run_persistent(Engine* E, int pph)
 {
-    const DevParticles P = h->part[h->cur];
-    PersOp op{P.weight, h->weight_array, h->meas, h->grid};
-    launch_segreduce(h, K_SEG_PERS, K_SEG_PERS_FIX, op, P.idx, h->N);
-    launch_begin(h, K_EMPTY_CELLS);
-    hipLaunchKernelGGL(k_empty_cells, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid.start,
-                       h->grid.f[GF_MU_A], h->grid.f[GF_MU_UA], nullptr, nullptr, nullptr, 2, h->C);
-    launch_end(h, K_EMPTY_CELLS);
-    launch_begin(h, K_POST_WEIGHTS);
-    hipLaunchKernelGGL(k_post_weights, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, P, h->weight_array,
-                       h->meas, h->grid, h->N);
-    launch_end(h, K_POST_WEIGHTS);
+    launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp{pph});
+    launch_begin(E, K_EMPTY_CELLS);
+    hipLaunchKernelGGL(k_empty_cells, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
+                       (int)GF_MU_A, (int)GF_MU_UA);
+    launch_end(E, K_EMPTY_CELLS);
+    launch_begin(E, K_POST_WEIGHTS);
+    hipLaunchKernelGGL(k_post_weights, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
+                       pph);
+    launch_end(E, K_POST_WEIGHTS);
 }
 
-struct SrcArray
+// Born masses per cell (first-frame placement when init_only, births otherwise).
+struct SrcBorn
 {
-    const float* a;
-    __device__ void load4(int p0, int cnt, float (&v)[4]) const { load_run<4>(a, p0, cnt, v); }
+    int init_only;
+    __device__ bool active(const VehDesc& d) const { return !init_only || d.init; }
+    __device__ int len(const VehDesc& d) const { return d.C; }
+    __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const { load_run<4>(d.born, p0, cnt, v); }
 };
 
 __device__ __forceinline__ int order_end_at(int j, fx_t prefix, const DevScalars* sc, int C, int v)
@@ -394,33 +444,26 @@ __device__ __forceinline__ int order_end_at(int j, fx_t prefix, const DevScalars
 // First-frame placement: P_j only.
 struct SinkOrders
 {
-    int* pend;
-    const DevScalars* sc;
-    int C, v;
-    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&)[4]) const
+    __device__ void apply4(const VehDesc& d, const DevScalars* sc, int p0, int cnt, const fx_t (&pre)[4],
+                           const fx_t (&)[4]) const
     {
         int e[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            e[i] = i < cnt ? order_end_at(p0 + i, pre[i], sc, C, v) : 0;
-        store_run<4>(pend, p0, cnt, e);
+            e[i] = i < cnt ? order_end_at(p0 + i, pre[i], sc, d.C, d.N) : 0;
+        store_run<4>(d.pend, p0, cnt, e);
     }
 };
 
 // Birth placement and weights per cell (init_new_particles.cu:126-155).
 struct SinkBirth
 {
-    int* pend;
-    const DevScalars* sc;
-    const dogm_measurement_cell* meas;
-    const float* born;
-    float* wA;
-    float* wUA;
-    int C, v;
-    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&own)[4]) const
+    __device__ void apply4(const VehDesc& d, const DevScalars* sc, int p0, int cnt, const fx_t (&pre)[4],
+                           const fx_t (&own)[4]) const
     {
         float bm[4];
-        load_run<4>(born, p0, cnt, bm);
+        load_run<4>(d.born, p0, cnt, bm);
+        const int C = d.C, v = d.B;
         int e[4];
         float a[4], u[4];
 #pragma unroll
@@ -435,116 +478,112 @@ struct SinkBirth
                 e[i] = order_end_at(j, pre[i], sc, C, v);
                 const int s = j == 0 ? 0 : order_end_at(j - 1, pre[i] - own[i], sc, C, v);
                 const int n = e[i] - s;
-                const float pA = meas[j].p_A;
+                const float pA = d.meas[j].p_A;
                 const int nu_A = (int)roundf((float)n * pA);
                 const int nu_UA = n - nu_A;
                 a[i] = nu_A > 0 ? (pA * bm[i]) / (float)nu_A : 0.0f;
                 u[i] = nu_UA > 0 ? ((1.0f - pA) * bm[i]) / (float)nu_UA : 0.0f;
             }
         }
-        store_run<4>(pend, p0, cnt, e);
-        store_run<4>(wA, p0, cnt, a);
-        store_run<4>(wUA, p0, cnt, u);
+        store_run<4>(d.pend, p0, cnt, e);
+        store_run<4>(d.f[GF_W_A], p0, cnt, a);
+        store_run<4>(d.f[GF_W_UA], p0, cnt, u);
     }
 };
 
 // ---------------------------------------------------------------------------------------------
 // Birth particles (dogm/src/dogm.cu:323-348, init.cu:46-67, init_new_particles.cu:157-195)
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_birth_particles(DevParticles Bp, int nb, const int* __restrict__ pend,
-                                                           int C, int gs, const dogm_measurement_cell* __restrict__ meas,
-                                                           const float* __restrict__ wA,
-                                                           const float* __restrict__ wUA, float sd, uint64_t seed,
-                                                           uint32_t inst, uint32_t epoch)
+__global__ __launch_bounds__(BLOCK) void k_birth_particles(const VehDesc* __restrict__ descs,
+                                                           const uint32_t* __restrict__ tab)
 {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= nb)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int k = tr.base + threadIdx.x;
+    if (k >= d.B)
         return;
-    const int j = upper_bound_i32(pend, C, k);
+    const int* pend = d.pend;
+    const int j = upper_bound_i32(pend, d.C, k);
     const int s = j > 0 ? pend[j - 1] : 0;
     const int n = pend[j] - s;
-    const int nu_A = (int)roundf((float)n * meas[j].p_A);
+    const int nu_A = (int)roundf((float)n * d.meas[j].p_A);
     const bool assoc = (k - s) <= nu_A;
-    const u32x4 r = draw(seed, inst, TAG_BIRTH, epoch, (uint32_t)k);
+    const u32x4 r = draw(d.seed, d.inst, TAG_BIRTH, d.ep_birth, (uint32_t)k);
     float n0, n1;
     box_muller(r.x, r.y, &n0, &n1);
+    const int gs = d.gs;
     float4 st;
     st.x = (float)(j % gs) + 0.5f;
     st.y = (float)j / (float)gs + 0.5f;
-    st.z = normal_scale(n0, 0.0f, sd);
-    st.w = normal_scale(n1, 0.0f, sd);
-    Bp.state[k] = st;
-    Bp.idx[k] = j;
-    Bp.weight[k] = assoc ? wA[j] : wUA[j];
-    Bp.assoc[k] = assoc ? 1 : 0;
+    st.z = normal_scale(n0, 0.0f, d.sd_birth);
+    st.w = normal_scale(n1, 0.0f, d.sd_birth);
+    d.birth.state[k] = st;
+    d.birth.idx[k] = j;
+    d.birth.weight[k] = assoc ? d.f[GF_W_A][j] : d.f[GF_W_UA][j];
+    d.birth.assoc[k] = assoc ? 1 : 0;
 }
 
-void run_births(dogm_handle* h)
+void run_births(Engine* E)
 {
     // With B = 0 the cell pass still runs (every cell gets n_j = 0 and w_A = w_UA = 0) and the
     // stage still consumes its RNG epoch, so the stream of later stages does not depend on B.
-    SrcArray src{h->born};
-    SinkBirth sink{h->pend, h->scalars, h->meas, h->born, h->grid.f[GF_W_A], h->grid.f[GF_W_UA], h->C, h->B};
-    run_exact_scan(h, src, sink, h->C, SCAN_ORDERS, h->B, K_BIRTH_CELLS);
-    if (h->B <= 0)
-    {
-        ++h->epoch;
+    run_exact_scan(E, SP_C, SrcBorn{0}, SinkBirth{}, SCAN_ORDERS_B, K_BIRTH_CELLS, false);
+    const int blocks = E->tiles(SP_B, 1);
+    if (blocks <= 0)
         return;
-    }
-    launch_begin(h, K_BIRTH_PARTICLES);
-    hipLaunchKernelGGL(k_birth_particles, dim3(div_up(h->B, BLOCK)), dim3(BLOCK), 0, h->stream, h->birth, h->B,
-                       h->pend, h->C, h->gs, h->meas, h->grid.f[GF_W_A], h->grid.f[GF_W_UA],
-                       h->params.stddev_velocity, h->seed, h->instance, h->epoch);
-    launch_end(h, K_BIRTH_PARTICLES);
-    ++h->epoch;
+    launch_begin(E, K_BIRTH_PARTICLES);
+    hipLaunchKernelGGL(k_birth_particles, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_B));
+    launch_end(E, K_BIRTH_PARTICLES);
 }
 
 // ---------------------------------------------------------------------------------------------
-// First-measurement initialisation (dogm/src/dogm.cu:217-241, init_new_particles.cu:76-124)
+// First-measurement initialisation (dogm/src/dogm.cu:217-241, init_new_particles.cu:76-124),
+// for the vehicles of the step whose descriptor has `init` set.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_copy_masses(const dogm_measurement_cell* __restrict__ meas,
-                                                       float* __restrict__ born, int C)
+__global__ __launch_bounds__(BLOCK) void k_copy_masses(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c < C)
-        born[c] = meas[c].occ_mass;
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + threadIdx.x;
+    if (d.init && c < d.C)
+        d.born[c] = d.meas[c].occ_mass;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_init_particles(DevParticles P, int n, const int* __restrict__ pend, int C,
-                                                          int gs, float v, float new_weight, uint64_t seed,
-                                                          uint32_t inst, uint32_t epoch)
+__global__ __launch_bounds__(BLOCK) void k_init_particles(const VehDesc* __restrict__ descs,
+                                                          const uint32_t* __restrict__ tab)
 {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= n)
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int k = tr.base + threadIdx.x;
+    if (!d.init || k >= d.N)
         return;
-    const int j = upper_bound_i32(pend, C, k);
-    const u32x4 r = draw(seed, inst, TAG_INIT_PARTICLES, epoch, (uint32_t)k);
+    const int gs = d.gs;
+    const int j = upper_bound_i32(d.pend, d.C, k);
+    const u32x4 r = draw(d.seed, d.inst, TAG_INIT_PARTICLES, d.ep_init, (uint32_t)k);
+    const float v = d.vmax;
     float4 st;
     st.x = (float)(j % gs) + 0.5f;
     st.y = (float)(j / gs) + 0.5f;
     st.z = -v + (v - -v) * uniform01(r.x);
     st.w = -v + (v - -v) * uniform01(r.y);
+    const DevParticles P = d.part[0];
     P.state[k] = st;
     P.idx[k] = j;
-    P.weight[k] = new_weight;
+    P.weight[k] = 1.0f / (float)d.N;  // dogm.cu:228
     P.assoc[k] = 0;
 }
 
-void run_init_particles(dogm_handle* h)
+void run_init_particles(Engine* E)
 {
-    launch_begin(h, K_COPY_MASSES);
-    hipLaunchKernelGGL(k_copy_masses, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->meas, h->born, h->C);
-    launch_end(h, K_COPY_MASSES);
-    SrcArray src{h->born};
-    SinkOrders sink{h->pend, h->scalars, h->C, h->N};
-    run_exact_scan(h, src, sink, h->C, SCAN_ORDERS, h->N, K_INIT_CELLS);
-    const float new_weight = 1.0f / (float)h->N;
-    launch_begin(h, K_INIT_PARTICLES);
-    hipLaunchKernelGGL(k_init_particles, dim3(div_up(h->N, BLOCK)), dim3(BLOCK), 0, h->stream, h->part[h->cur],
-                       h->N, h->pend, h->C, h->gs, h->params.init_max_velocity, new_weight, h->seed, h->instance,
-                       h->epoch);
-    launch_end(h, K_INIT_PARTICLES);
-    ++h->epoch;
+    launch_begin(E, K_COPY_MASSES);
+    hipLaunchKernelGGL(k_copy_masses, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
+    launch_end(E, K_COPY_MASSES);
+    run_exact_scan(E, SP_C, SrcBorn{1}, SinkOrders{}, SCAN_ORDERS_N, K_INIT_CELLS, true);
+    launch_begin(E, K_INIT_PARTICLES);
+    hipLaunchKernelGGL(k_init_particles, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                       E->tab(SP_P));
+    launch_end(E, K_INIT_PARTICLES);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -554,13 +593,13 @@ struct MomOp
 {
     static constexpr int NCH = 5;
     static constexpr int ITEMS = 4;
-    const float* weight_array;
-    const float4* state;
-    DevGrid G;
-    __device__ void load(int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
+    int pph;
+    __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];
-        load_run<ITEMS>(weight_array, p0, cnt, w);
+        load_run<ITEMS>(d.weight_array, p0, cnt, w);
+        const float4* state = d.part[pph].state;
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
         {
@@ -572,9 +611,9 @@ struct MomOp
             v[i][4] = w[i] * s.z * s.w;
         }
     }
-    __device__ void finalize(int key, const fx_t (&s)[5]) const
+    __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[5]) const
     {
-        const float rho_p = G.f[GF_PERS][key];
+        const float rho_p = d.f[GF_PERS][key];
         float mx = 0.0f, my = 0.0f, vx = 0.0f, vy = 0.0f, cv = 0.0f;
         if (rho_p > 0.0f)
         {
@@ -585,24 +624,21 @@ struct MomOp
             vy = inv * fx_to_f32(s[3]) - my * my;
             cv = inv * fx_to_f32(s[4]) - mx * my;
         }
-        G.f[GF_MEAN_X][key] = mx;
-        G.f[GF_MEAN_Y][key] = my;
-        G.f[GF_VAR_X][key] = vx;
-        G.f[GF_VAR_Y][key] = vy;
-        G.f[GF_COVAR][key] = cv;
+        d.f[GF_MEAN_X][key] = mx;
+        d.f[GF_MEAN_Y][key] = my;
+        d.f[GF_VAR_X][key] = vx;
+        d.f[GF_VAR_Y][key] = vy;
+        d.f[GF_COVAR][key] = cv;
     }
 };
 
-void run_moments(dogm_handle* h)
+void run_moments(Engine* E, int pph)
 {
-    const DevParticles P = h->part[h->cur];
-    MomOp op{h->weight_array, P.state, h->grid};
-    launch_segreduce(h, K_SEG_MOM, K_SEG_MOM_FIX, op, P.idx, h->N);
-    launch_begin(h, K_EMPTY_CELLS);
-    hipLaunchKernelGGL(k_empty_cells, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid.start,
-                       h->grid.f[GF_MEAN_X], h->grid.f[GF_MEAN_Y], h->grid.f[GF_VAR_X], h->grid.f[GF_VAR_Y],
-                       h->grid.f[GF_COVAR], 5, h->C);
-    launch_end(h, K_EMPTY_CELLS);
+    launch_segreduce(E, K_SEG_MOM, K_SEG_MOM_FIX, MomOp{pph});
+    launch_begin(E, K_EMPTY_CELLS);
+    hipLaunchKernelGGL(k_empty_cells, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
+                       (int)GF_MEAN_X, (int)GF_COVAR);
+    launch_end(E, K_EMPTY_CELLS);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -657,19 +693,19 @@ __global__ __launch_bounds__(BLOCK) void k_unpack_grid(DevGrid G, const dogm_gri
     G.f[GF_COVAR][c] = g.covar_xy_vel;
 }
 
-void run_pack_grid(dogm_handle* h)
+void run_pack_grid(Engine* E, const Vehicle* v)
 {
-    launch_begin(h, K_PACK_GRID);
-    hipLaunchKernelGGL(k_pack_grid, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid, h->aos_tmp, h->C);
-    launch_end(h, K_PACK_GRID);
+    launch_begin(E, K_PACK_GRID);
+    hipLaunchKernelGGL(k_pack_grid, dim3(div_up(v->C, BLOCK)), dim3(BLOCK), 0, E->stream, v->grid, v->aos_tmp, v->C);
+    launch_end(E, K_PACK_GRID);
 }
 
-void run_unpack_grid(dogm_handle* h)
+void run_unpack_grid(Engine* E, const Vehicle* v)
 {
-    launch_begin(h, K_UNPACK_GRID);
-    hipLaunchKernelGGL(k_unpack_grid, dim3(div_up(h->C, BLOCK)), dim3(BLOCK), 0, h->stream, h->grid, h->aos_tmp,
-                       h->C);
-    launch_end(h, K_UNPACK_GRID);
+    launch_begin(E, K_UNPACK_GRID);
+    hipLaunchKernelGGL(k_unpack_grid, dim3(div_up(v->C, BLOCK)), dim3(BLOCK), 0, E->stream, v->grid, v->aos_tmp,
+                       v->C);
+    launch_end(E, K_UNPACK_GRID);
 }
 
 }  // namespace dogm_hip

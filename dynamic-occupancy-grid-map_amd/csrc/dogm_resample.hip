@@ -11,6 +11,7 @@
 //    those tiles are staged in LDS, every draw finds its ancestor there, and the next particle is
 //    written directly (the reference's lower_bound and resamplingKernel gather in one pass).
 // No atomics on global memory and no global sort: the result is a pure function of the inputs.
+// Every vehicle of a step resamples from its own CDF with its own draws.
 #include "dogm_device.h"
 
 namespace dogm_hip
@@ -25,20 +26,20 @@ static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-sc
 
 struct SrcJointPos
 {
-    const float* w;
-    const float* bw;
-    int N;
-    __device__ void load4(int p0, int cnt, float (&v)[4]) const
+    __device__ bool active(const VehDesc&) const { return true; }
+    __device__ int len(const VehDesc& d) const { return d.N + d.B; }
+    __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const
     {
+        const int N = d.N;
         if (p0 + cnt <= N)
-            load_run<4>(w, p0, cnt, v);
+            load_run<4>(d.weight_array, p0, cnt, v);
         else if (p0 >= N && ((p0 - N) & 3) == 0)
-            load_run<4>(bw, p0 - N, cnt, v);
+            load_run<4>(d.birth.weight, p0 - N, cnt, v);
         else
         {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                v[i] = i < cnt ? (p0 + i < N ? w[p0 + i] : bw[p0 + i - N]) : 0.0f;
+                v[i] = i < cnt ? (p0 + i < N ? d.weight_array[p0 + i] : d.birth.weight[p0 + i - N]) : 0.0f;
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -48,37 +49,36 @@ struct SrcJointPos
 
 struct SinkCdf
 {
-    fx_t* cdf;
-    __device__ void apply4(int p0, int cnt, const fx_t (&pre)[4], const fx_t (&)[4]) const
+    __device__ void apply4(const VehDesc& d, const DevScalars*, int p0, int cnt, const fx_t (&pre)[4],
+                           const fx_t (&)[4]) const
     {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (i < cnt)
-                cdf[p0 + i] = pre[i];
+                d.cdf[p0 + i] = pre[i];
     }
 };
 
 // E for the four consecutive draws k = p0 .. p0 + 3 of one thread (p0 is a multiple of 4).
-__device__ __forceinline__ void exp4(uint64_t seed, uint32_t inst, uint32_t epoch, int p0, int n, fx_t* e)
+__device__ __forceinline__ void exp4(const VehDesc& d, int p0, int n, fx_t* e)
 {
     float f[4];
-    resample_exponentials4(seed, inst, epoch, (uint32_t)(p0 >> 2), f);
+    resample_exponentials4(d.seed, d.inst, d.ep_rs, (uint32_t)(p0 >> 2), f);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
         e[i] = p0 + i < n ? fx_from_f32(f[i]) : (fx_t)0;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_exp_tiles(uint64_t seed, uint32_t inst, uint32_t epoch, int n,
-                                                     fx_t* __restrict__ tile_sum)
+// The N + 1 exponentials as a scan source (tile totals only).
+struct SrcExp
 {
-    const int p0 = blockIdx.x * RS_TILE + threadIdx.x * RS_ITEMS;
-    fx_t e[4];
-    exp4(seed, inst, epoch, p0, n, e);
-    fx_t total;
-    block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &total);
-    if (threadIdx.x == 0)
-        tile_sum[blockIdx.x] = total;
-}
+    __device__ bool active(const VehDesc&) const { return true; }
+    __device__ int len(const VehDesc& d) const { return d.N + 1; }
+    __device__ void load4(const VehDesc& d, int p0, int, float (&v)[4]) const
+    {
+        resample_exponentials4(d.seed, d.inst, d.ep_rs, (uint32_t)(p0 >> 2), v);
+    }
+};
 
 // Index of the first CDF tile whose inclusive end is >= r (T - 1 if none): two-level k-ary search.
 __device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* s_cnt)
@@ -104,22 +104,31 @@ __device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* 
     return t;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst, uint32_t epoch, int n,
+__global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ descs,
+                                                    const uint32_t* __restrict__ tab, int pph,
                                                     const fx_t* __restrict__ exp_tile_pre,
-                                                    const fx_t* __restrict__ cdf, const fx_t* __restrict__ cdf_incl,
-                                                    int J, const DevScalars* __restrict__ sc, DevParticles P,
-                                                    DevParticles Bp, DevParticles out, int* __restrict__ ancestors)
+                                                    const fx_t* __restrict__ cdf_incl,
+                                                    const DevScalars* __restrict__ scs)
 {
     __shared__ fx_t s_cdf[RS_SLICE];
     __shared__ fx_t s_rb[2];
     __shared__ int s_cnt[1];
 
-    const int base = blockIdx.x * RS_TILE;
+    const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int n = d.N;
+    const int base = tr.base;
+    if (base >= n)
+        return;
+    const DevScalars* sc = scs + tr.v;
+    const int J = n + d.B;
+    const fx_t* cdf = d.cdf;
     const int p0 = base + threadIdx.x * RS_ITEMS;
     fx_t e[4];
-    exp4(seed, inst, epoch, p0, n, e);
+    exp4(d, p0, n, e);
     fx_t tot;
-    fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) + exp_tile_pre[blockIdx.x];
+    fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) +
+               exp_tile_pre[d.u0[SP_E] / SCAN_Q + base / RS_TILE];
     const Spacing sp = sc->spacing;
     fx_t r[RS_ITEMS];
 #pragma unroll
@@ -137,12 +146,14 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst
             s_rb[1] = r[i];
     __syncthreads();
     const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
-    const int t_lo = find_cdf_tile(cdf_incl, T, s_rb[0], s_cnt);
-    const int t_hi = find_cdf_tile(cdf_incl, T, s_rb[1], s_cnt);
+    const fx_t* incl = cdf_incl + d.u0[SP_J] / SCAN_Q;
+    const int t_lo = find_cdf_tile(incl, T, s_rb[0], s_cnt);
+    const int t_hi = find_cdf_tile(incl, T, s_rb[1], s_cnt);
     // Walk the CDF tiles [t_lo, t_hi] in LDS-sized chunks; a draw is resolved in the first chunk
     // whose last entry is >= its value (the CDF is non-decreasing, so its lower bound is there).
-    // A thread's four draws are consecutive, so after one binary search the next ancestors are
-    // found by stepping forward.
+    // A thread's four draws are consecutive, so after one binary search the next lower bounds are
+    // found by galloping forward from the previous one (bounded by 2 log2(RS_SLICE) probes even
+    // where consecutive draws are thousands of entries apart).
     int anc[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
@@ -162,30 +173,32 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst
         {
             if (anc[i] < 0 && p0 + i < n && !(cmax < r[i]))
             {
-                int lo;
-                if (pos < 0)
-                {
-                    lo = 0;
-                    int count = clen;
-                    while (count > 0)
-                    {
-                        const int step = count >> 1;
-                        if (s_cdf[lo + step] < r[i])
-                        {
-                            lo += step + 1;
-                            count -= step + 1;
-                        }
-                        else
-                        {
-                            count = step;
-                        }
-                    }
-                }
-                else
+                // Lower bound of r[i] in [lo, hi]; s_cdf[hi] >= r[i] holds throughout (cmax >= r[i]).
+                int lo = 0;
+                int hi = clen - 1;
+                if (pos >= 0)
                 {
                     lo = pos;
-                    while (s_cdf[lo] < r[i])
-                        ++lo;
+                    hi = pos;
+                    for (int step = 1; hi < clen - 1 && s_cdf[hi] < r[i]; step <<= 1)
+                    {
+                        lo = hi + 1;
+                        hi = min(clen - 1, hi + step);
+                    }
+                }
+                int count = hi - lo;
+                while (count > 0)
+                {
+                    const int step = count >> 1;
+                    if (s_cdf[lo + step] < r[i])
+                    {
+                        lo += step + 1;
+                        count -= step + 1;
+                    }
+                    else
+                    {
+                        count = step;
+                    }
                 }
                 pos = lo;
                 anc[i] = c0 + lo;
@@ -200,6 +213,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst
         s_anc[threadIdx.x * RS_ITEMS + i] = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);
     __syncthreads();
     const float nw = sc->new_weight;
+    const DevParticles P = d.part[pph];
+    const DevParticles Bp = d.birth;
+    const DevParticles out = d.part[pph ^ 1];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
     {
@@ -208,7 +224,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst
         if (k >= n)
             break;
         const int a = s_anc[j];
-        ancestors[k] = a;
+        d.ancestors[k] = a;
         const bool from_p = a < n;
         const int o = from_p ? a : a - n;
         const float4* st = from_p ? P.state : Bp.state;
@@ -221,32 +237,24 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(uint64_t seed, uint32_t inst
     }
 }
 
-void run_resampling(dogm_handle* h)
+void run_resampling(Engine* E, int pph)
 {
-    const int N = h->N;
-    const int J = N + h->B;
-    SrcJointPos src{h->weight_array, h->birth.weight, N};
-    SinkCdf sink{h->cdf};
-    run_exact_scan(h, src, sink, J, SCAN_RESAMPLE, N, K_CDF_WRITE, h->cdf_tile_incl);
+    run_exact_scan(E, SP_J, SrcJointPos{}, SinkCdf{}, SCAN_RESAMPLE, K_CDF_WRITE, false, E->cdf_tile_incl);
 
-    const int etiles = div_up(N + 1, RS_TILE);
-    launch_begin(h, K_EXP_TILES);
-    hipLaunchKernelGGL(k_exp_tiles, dim3(etiles), dim3(BLOCK), 0, h->stream, h->seed, h->instance, h->epoch, N + 1,
-                       h->scan_tile_sum);
-    launch_end(h, K_EXP_TILES);
-    launch_begin(h, K_SCAN_TOP);
-    hipLaunchKernelGGL(k_scan_top<0>, dim3(1), dim3(BLOCK), 0, h->stream, h->scan_tile_sum, h->scan_tile_pre,
-                       (fx_t*)nullptr, etiles, h->scalars, (int)SCAN_EXP, N);
-    launch_end(h, K_SCAN_TOP);
+    const int etiles = E->tiles(SP_E, SCAN_Q);
+    launch_begin(E, K_EXP_TILES);
+    hipLaunchKernelGGL(k_scan_tiles<SrcExp>, dim3(etiles), dim3(BLOCK), 0, E->stream, SrcExp{}, E->d_desc,
+                       E->tab(SP_E), E->scan_tile_sum);
+    launch_end(E, K_EXP_TILES);
+    launch_begin(E, K_SCAN_TOP);
+    hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_E, E->scan_tile_sum,
+                       E->scan_tile_pre, (fx_t*)nullptr, E->scalars, (int)SCAN_EXP, 0);
+    launch_end(E, K_SCAN_TOP);
 
-    const int other = h->cur ^ 1;
-    launch_begin(h, K_RS_MERGE);
-    hipLaunchKernelGGL(k_rs_merge, dim3(div_up(N, RS_TILE)), dim3(BLOCK), 0, h->stream, h->seed, h->instance,
-                       h->epoch, N, h->scan_tile_pre, h->cdf, h->cdf_tile_incl, J, h->scalars, h->part[h->cur],
-                       h->birth, h->part[other], h->ancestors);
-    launch_end(h, K_RS_MERGE);
-    h->cur = other;
-    ++h->epoch;
+    launch_begin(E, K_RS_MERGE);
+    hipLaunchKernelGGL(k_rs_merge, dim3(E->tiles(SP_P, RS_TILE / UNIT)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                       E->tab(SP_P), pph, E->scan_tile_pre, E->cdf_tile_incl, E->scalars);
+    launch_end(E, K_RS_MERGE);
 }
 
 }  // namespace dogm_hip

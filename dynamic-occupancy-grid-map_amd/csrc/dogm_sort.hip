@@ -2,8 +2,9 @@
 // dogm/src/dogm.cu:270-276).
 //
 // Keys are cell indices < grid_size^2, so only ceil(log2(C)) bits are sorted, 8 bits per pass.
-// Per pass: tile digit counts -> per-digit exclusive scan over tiles (plus the digit's global base
-// from a one-time histogram of all passes) -> ranking and scatter.  Ranking is stable: inside a
+// Per pass: tile digit counts -> per-vehicle, per-digit exclusive scan over the vehicle's tiles (plus
+// the digit's base from a one-time histogram of all passes) -> ranking and scatter.  Every vehicle
+// of a step is sorted independently into its own buffers; all use the step's pass count.  Ranking is stable: inside a
 // tile, keys are laid out wave-striped (wave w, item j, lane l = position (w*ITEMS + j)*64 + l),
 // each wave ranks its 64 keys of one item with ballot-matched peer masks against per-wave digit
 // counters in LDS, and waves are combined in order.  The scatter is staged through LDS in
@@ -18,6 +19,7 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int SORT_WAVES = BLOCK / WAVE;
+constexpr int SORT_Q = SORT_TILE / UNIT;  // units per sort tile
 
 __device__ __forceinline__ uint64_t lanemask_lt()
 {
@@ -65,44 +67,60 @@ __device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid
     }
 }
 
-// Histogram of every pass's digit over all keys.
-__global__ __launch_bounds__(BLOCK) void k_sort_hist(const int* __restrict__ keys, int n, int passes,
-                                                     unsigned* __restrict__ ghist)
+// Histogram of every pass's digit over all keys, per vehicle: ghist[(v * 4 + pass) * 256 + digit].
+__global__ __launch_bounds__(BLOCK) void k_sort_hist(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
+                                                     int pph, int passes, unsigned* __restrict__ ghist)
 {
     __shared__ unsigned s_h[4][RADIX];
+    const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
+    const VehDesc& dv = descs[tr.v];
+    const int n = dv.N;
+    if (tr.base >= n)
+        return;
+    const int* keys = dv.part[pph].idx;
     for (int i = threadIdx.x; i < 4 * RADIX; i += BLOCK)
         (&s_h[0][0])[i] = 0u;
     __syncthreads();
-    const int base = blockIdx.x * SORT_TILE;
     for (int i = 0; i < SORT_ITEMS; ++i)
     {
-        const int p = base + i * BLOCK + threadIdx.x;
+        const int p = tr.base + i * BLOCK + threadIdx.x;
         const bool valid = p < n;
         const unsigned k = valid ? (unsigned)keys[p] : 0u;
         for (int ps = 0; ps < passes; ++ps)
             wave_count(s_h[ps], (k >> (ps * RADIX_BITS)) & (RADIX - 1), valid);
     }
     __syncthreads();
+    unsigned* gh = ghist + (size_t)tr.v * 4 * RADIX;
     for (int i = threadIdx.x; i < passes * RADIX; i += BLOCK)
     {
         const unsigned v = (&s_h[0][0])[i];
         if (v)
-            atomicAdd(&ghist[i], v);
+            atomicAdd(&gh[i], v);
     }
 }
 
+__device__ __forceinline__ const int* sort_keys_in(const VehDesc& d, int pph, int ps)
+{
+    return ps == 0 ? d.part[pph].idx : d.sort_keys[(ps - 1) & 1];
+}
+
 // Digit counts of one tile for one pass: thist[d * tiles + t].
-__global__ __launch_bounds__(BLOCK) void k_sort_upsweep(const int* __restrict__ keys, int n, int shift, int tiles,
+__global__ __launch_bounds__(BLOCK) void k_sort_upsweep(const VehDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
                                                         unsigned* __restrict__ thist)
 {
     __shared__ unsigned s_h[RADIX];
+    const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
+    const VehDesc& dv = descs[tr.v];
+    const int n = dv.N;
+    const int shift = ps * RADIX_BITS;
+    const int* keys = sort_keys_in(dv, pph, ps);
     for (int i = threadIdx.x; i < RADIX; i += BLOCK)
         s_h[i] = 0u;
     __syncthreads();
-    const int base = blockIdx.x * SORT_TILE;
     for (int i = 0; i < SORT_ITEMS; ++i)
     {
-        const int p = base + i * BLOCK + threadIdx.x;
+        const int p = tr.base + i * BLOCK + threadIdx.x;
         const bool valid = p < n;
         wave_count(s_h, valid ? ((unsigned)keys[p] >> shift) & (RADIX - 1) : 0u, valid);
     }
@@ -111,14 +129,17 @@ __global__ __launch_bounds__(BLOCK) void k_sort_upsweep(const int* __restrict__ 
         thist[(size_t)d * tiles + blockIdx.x] = s_h[d];
 }
 
-// One workgroup per digit: exclusive scan over tiles plus the digit's global base.  Each thread
-// scans a contiguous chunk of the digit's row.
-__global__ __launch_bounds__(BLOCK) void k_sort_scan(unsigned* __restrict__ thist, int tiles,
-                                                     const unsigned* __restrict__ ghist_pass)
+// One workgroup per (digit, vehicle): exclusive scan over the vehicle's tiles plus the digit's
+// base inside the vehicle.  Each thread scans a contiguous chunk of the row.
+__global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__ descs, unsigned* __restrict__ thist,
+                                                     int tiles, const unsigned* __restrict__ ghist, int ps)
 {
     __shared__ unsigned s_w[SORT_WAVES];
     __shared__ unsigned s_base;
     const int d = blockIdx.x;
+    const int v = blockIdx.y;
+    const VehDesc& dv = descs[v];
+    const unsigned* ghist_pass = ghist + ((size_t)v * 4 + ps) * RADIX;
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     if (threadIdx.x < WAVE)
     {
@@ -131,10 +152,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(unsigned* __restrict__ this
         if (lane == 0)
             s_base = b;
     }
-    unsigned* row = thist + (size_t)d * tiles;
-    const int per = (tiles + BLOCK - 1) / BLOCK;
-    const int b0 = min(tiles, (int)threadIdx.x * per);
-    const int b1 = min(tiles, b0 + per);
+    const int t0 = dv.u0[SP_P] / SORT_Q;
+    const int nt = dv.nu[SP_P] / SORT_Q;
+    unsigned* row = thist + (size_t)d * tiles + t0;
+    const int per = (nt + BLOCK - 1) / BLOCK;
+    const int b0 = min(nt, (int)threadIdx.x * per);
+    const int b1 = min(nt, b0 + per);
     unsigned sum = 0;
     for (int t = b0; t < b1; ++t)
         sum += row[t];
@@ -162,10 +185,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(unsigned* __restrict__ this
     }
 }
 
-// Stable ranking and scatter of one tile.  vals_in == nullptr means identity values.
-__global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const int* __restrict__ keys_in,
-                                                          const int* __restrict__ vals_in, int* __restrict__ keys_out,
-                                                          int* __restrict__ vals_out, int n, int shift, int tiles,
+// Stable ranking and scatter of one tile.  Pass 0 ranks the particles' cell indices with identity
+// values; later passes read the previous pass's output.
+__global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restrict__ descs,
+                                                          const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
                                                           const unsigned* __restrict__ goff)
 {
     __shared__ unsigned s_wcnt[SORT_WAVES][RADIX];
@@ -174,10 +197,21 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const int* __restrict_
     __shared__ int s_vals[SORT_TILE];
     __shared__ unsigned s_scan[SORT_WAVES];
 
+    const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
+    const VehDesc& dv = descs[tr.v];
+    const int n = dv.N;
+    const int tbase = tr.base;
+    if (tbase >= n)
+        return;
+    const int shift = ps * RADIX_BITS;
+    const int* keys_in = sort_keys_in(dv, pph, ps);
+    const int* vals_in = ps == 0 ? nullptr : dv.sort_vals[(ps - 1) & 1];
+    int* keys_out = dv.sort_keys[ps & 1];
+    int* vals_out = dv.sort_vals[ps & 1];
+
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const int tile = blockIdx.x;
-    const int tbase = tile * SORT_TILE;
     for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += BLOCK)
         (&s_wcnt[0][0])[i] = 0u;
     __syncthreads();
@@ -264,41 +298,33 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const int* __restrict_
 
 static_assert(BLOCK == RADIX, "k_sort_downsweep assigns one digit per thread");
 
-void run_sort(dogm_handle* h, const int** skeys, const int** perm)
+void run_sort(Engine* E, int pph)
 {
-    const int n = h->N;
-    const int tiles = div_up(n, SORT_TILE);
-    const int passes = h->sort_passes;
-    const int* kin = h->part[h->cur].idx;
-    const int* vin = nullptr;
-
-    run_fill_u32(h, h->sort_ghist, 0u, (long long)passes * RADIX);
-    launch_begin(h, K_SORT_HIST);
-    hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, h->stream, kin, n, passes, h->sort_ghist);
-    launch_end(h, K_SORT_HIST);
+    const int tiles = E->tiles(SP_P, SORT_Q);
+    const int passes = E->passes;
+    if (tiles <= 0)
+        return;
+    run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
+    launch_begin(E, K_SORT_HIST);
+    hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, passes,
+                       E->sort_ghist);
+    launch_end(E, K_SORT_HIST);
 
     for (int ps = 0; ps < passes; ++ps)
     {
-        const int shift = ps * RADIX_BITS;
-        int* kout = h->sort_keys[ps & 1];
-        int* vout = h->sort_vals[ps & 1];
-        launch_begin(h, K_SORT_UPSWEEP);
-        hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(BLOCK), 0, h->stream, kin, n, shift, tiles,
-                           h->sort_thist);
-        launch_end(h, K_SORT_UPSWEEP);
-        launch_begin(h, K_SORT_SCAN);
-        hipLaunchKernelGGL(k_sort_scan, dim3(RADIX), dim3(BLOCK), 0, h->stream, h->sort_thist, tiles,
-                           h->sort_ghist + ps * RADIX);
-        launch_end(h, K_SORT_SCAN);
-        launch_begin(h, K_SORT_DOWNSWEEP);
-        hipLaunchKernelGGL(k_sort_downsweep, dim3(tiles), dim3(BLOCK), 0, h->stream, kin, vin, kout, vout, n, shift,
-                           tiles, h->sort_thist);
-        launch_end(h, K_SORT_DOWNSWEEP);
-        kin = kout;
-        vin = vout;
+        launch_begin(E, K_SORT_UPSWEEP);
+        hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, ps,
+                           tiles, E->sort_thist);
+        launch_end(E, K_SORT_UPSWEEP);
+        launch_begin(E, K_SORT_SCAN);
+        hipLaunchKernelGGL(k_sort_scan, dim3(RADIX, E->m), dim3(BLOCK), 0, E->stream, E->d_desc, E->sort_thist, tiles,
+                           E->sort_ghist, ps);
+        launch_end(E, K_SORT_SCAN);
+        launch_begin(E, K_SORT_DOWNSWEEP);
+        hipLaunchKernelGGL(k_sort_downsweep, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, ps,
+                           tiles, E->sort_thist);
+        launch_end(E, K_SORT_DOWNSWEEP);
     }
-    *skeys = kin;
-    *perm = vin;
 }
 
 }  // namespace dogm_hip

@@ -102,6 +102,22 @@ _SIGS = {
     "dogm_set_kernel_timing": (_i, [_h, _i, _i]),
     "dogm_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "dogm_reset_kernel_timing": (_i, [_h]),
+    "dogm_batch_create": (_i, [C.POINTER(_h)]),
+    "dogm_batch_destroy": (None, [_h]),
+    "dogm_batch_last_error": (C.c_char_p, [_h]),
+    "dogm_vehicle_footprint": (_i, [C.POINTER(dogm_params), C.POINTER(C.c_uint64)]),
+    "dogm_batch_add_vehicle": (_i, [_h, C.POINTER(dogm_params), C.c_uint64, C.c_uint32, C.POINTER(_i)]),
+    "dogm_batch_remove_vehicle": (_i, [_h, _i]),
+    "dogm_batch_vehicle_count": (_i, [_h]),
+    "dogm_batch_device_bytes": (C.c_uint64, [_h]),
+    "dogm_batch_measurement_buffer": (_i, [_h, _i, C.POINTER(_vp)]),
+    "dogm_batch_update": (_i, [_h, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "dogm_batch_get_grid_cells": (_i, [_h, _i, _vp]),
+    "dogm_batch_get_particles": (_i, [_h, _i, _vp, _vp, _vp, _vp]),
+    "dogm_batch_get_resampled_indices": (_i, [_h, _i, _vp]),
+    "dogm_batch_set_kernel_timing": (_i, [_h, _i, _i]),
+    "dogm_batch_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "dogm_batch_reset_kernel_timing": (_i, [_h]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -331,6 +347,130 @@ class DOGM:
 
     def reset_kernel_timing(self) -> None:
         self._check(lib.dogm_reset_kernel_timing(self.h), "timing")
+
+
+def vehicle_footprint(params: Params) -> int:
+    """Device bytes one vehicle with these params holds (dogm_vehicle_footprint)."""
+    cp = params.to_c()
+    b = C.c_uint64()
+    st = lib.dogm_vehicle_footprint(C.byref(cp), C.byref(b))
+    if st != DOGM_OK:
+        raise DogmError(st, "vehicle_footprint")
+    return int(b.value)
+
+
+class Batch:
+    """Many independent grids ("vehicles") updated in the same kernel launches (dogm_batch_*)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        st = lib.dogm_batch_create(C.byref(h))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_batch_create")
+        self.h = h
+        self.vehicles: dict[int, Params] = {}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.dogm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != DOGM_OK:
+            raise DogmError(st, f"{what}: {lib.dogm_batch_last_error(self.h).decode()}")
+
+    def add_vehicle(self, params: Params, seed: int = DOGM_DEFAULT_SEED, instance: int = 0) -> int:
+        cp = params.to_c()
+        vid = C.c_int()
+        self._check(lib.dogm_batch_add_vehicle(self.h, C.byref(cp), seed, instance, C.byref(vid)), "addVehicle")
+        self.vehicles[vid.value] = params
+        return vid.value
+
+    def remove_vehicle(self, vid: int) -> None:
+        self._check(lib.dogm_batch_remove_vehicle(self.h, vid), "removeVehicle")
+        self.vehicles.pop(vid, None)
+
+    def device_bytes(self) -> int:
+        return int(lib.dogm_batch_device_bytes(self.h))
+
+    def measurement_buffer(self, vid: int) -> int:
+        p = C.c_void_p()
+        self._check(lib.dogm_batch_measurement_buffer(self.h, vid, C.byref(p)), "measurementBuffer")
+        return p.value
+
+    def grid_cell_count(self, vid: int) -> int:
+        p = self.vehicles[vid]
+        gs = int(np.float32(p.size) / np.float32(p.resolution))
+        return gs * gs
+
+    def update(self, vids, meas, poses, dts) -> None:
+        """One frame for the vehicles `vids`.  meas[i]: host structured array, DeviceBuffer, or None
+        (the grid already sits in the vehicle's measurement buffer); poses[i] = (x, y, yaw)."""
+        n = len(vids)
+        ids = np.ascontiguousarray(vids, dtype=np.int32)
+        pose = np.asarray(poses, dtype=np.float32).reshape(n, 3)
+        xs, ys, yaws = (np.ascontiguousarray(pose[:, k]) for k in range(3))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float32), (n,)))
+        host = [m for m in meas if m is not None and not isinstance(m, DeviceBuffer)]
+        dev = [m for m in meas if isinstance(m, DeviceBuffer)]
+        if host and dev:
+            raise ValueError("one step takes host or device measurement grids, not both")
+        keep = []
+        ptrs = (C.c_void_p * n)()
+        for i, m in enumerate(meas):
+            if m is None:
+                ptrs[i] = None
+            elif isinstance(m, DeviceBuffer):
+                ptrs[i] = m.ptr
+            else:
+                a = np.ascontiguousarray(m, dtype=MEAS_CELL_DTYPE)
+                keep.append(a)
+                ptrs[i] = a.ctypes.data
+        self._check(lib.dogm_batch_update(self.h, n, _ptr(ids), C.cast(ptrs, C.c_void_p), _ptr(xs), _ptr(ys),
+                                          _ptr(yaws), _ptr(dt), 1 if dev else 0), "batchUpdate")
+
+    def get_grid_cells(self, vid: int) -> np.ndarray:
+        out = np.empty(self.grid_cell_count(vid), dtype=GRID_CELL_DTYPE)
+        self._check(lib.dogm_batch_get_grid_cells(self.h, vid, _ptr(out)), "getGridCells")
+        return out
+
+    def get_particles(self, vid: int) -> dict:
+        n = self.vehicles[vid].particle_count
+        state = np.empty((n, 4), np.float32)
+        idx = np.empty(n, np.int32)
+        w = np.empty(n, np.float32)
+        a = np.empty(n, np.uint8)
+        self._check(lib.dogm_batch_get_particles(self.h, vid, _ptr(state), _ptr(idx), _ptr(w), _ptr(a)),
+                    "getParticles")
+        return {"state": state, "grid_cell_idx": idx, "weight": w, "associated": a}
+
+    def get_resampled_indices(self, vid: int) -> np.ndarray:
+        out = np.empty(self.vehicles[vid].particle_count, np.int32)
+        self._check(lib.dogm_batch_get_resampled_indices(self.h, vid, _ptr(out)), "getResampledIndices")
+        return out
+
+    def set_kernel_timing(self, name: str, enable: bool = True) -> None:
+        self._check(lib.dogm_batch_set_kernel_timing(self.h, DOGM.kernel_names().index(name), int(enable)),
+                    "timing")
+
+    def kernel_timing(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._check(lib.dogm_batch_get_kernel_timing(self.h, DOGM.kernel_names().index(name), C.byref(ms),
+                                                     C.byref(n)), "timing")
+        return ms.value, n.value
+
+    def reset_kernel_timing(self) -> None:
+        self._check(lib.dogm_batch_reset_kernel_timing(self.h), "timing")
 
 
 def _as_particle_arrays(n, state, idx, weight, assoc):

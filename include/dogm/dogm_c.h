@@ -181,6 +181,46 @@ int dogm_set_kernel_timing(dogm_handle* h, int kernel_id, int enable);
 int dogm_get_kernel_timing(dogm_handle* h, int kernel_id, double* total_ms, long long* launches);
 int dogm_reset_kernel_timing(dogm_handle* h);
 
+/* ----------------------------------------------------------------------------------------------
+ * Batched update of many independent grids ("vehicles").  No counterpart in the reference, which
+ * runs one dogm::DOGM per grid, one frame per updateGrid call (dogm/src/dogm.cu:115-131).
+ *
+ * A batch owns vehicles of any size (each with its own dogm_params, Philox seed and instance) and
+ * one HIP stream.  dogm_batch_update runs one full updateGrid frame for a list of its vehicles in
+ * the same kernel launches: every stage launches once for the whole list, with a work table that
+ * maps each workgroup to a vehicle and a tile of that vehicle.  The result for a vehicle is
+ * bit-identical to the plain handle created with the same params, seed and instance and fed the
+ * same frames (dogm_create_ex + dogm_update_grid), whatever other vehicles share its steps.
+ * Vehicles keep all their state in device memory between steps.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct dogm_batch dogm_batch;
+
+int dogm_batch_create(dogm_batch** out);
+void dogm_batch_destroy(dogm_batch* b);
+const char* dogm_batch_last_error(const dogm_batch* b);
+/* Device bytes one vehicle with these params holds (for admission against a memory budget). */
+int dogm_vehicle_footprint(const dogm_params* params, uint64_t* bytes);
+int dogm_batch_add_vehicle(dogm_batch* b, const dogm_params* params, uint64_t seed, uint32_t instance,
+                           int* vehicle_id);
+int dogm_batch_remove_vehicle(dogm_batch* b, int vehicle_id);
+int dogm_batch_vehicle_count(const dogm_batch* b);
+uint64_t dogm_batch_device_bytes(const dogm_batch* b);
+/* Device address of a vehicle's measurement grid (grid_size^2 cells): a producer on the same
+ * device may write the next frame there and pass a NULL measurement pointer to dogm_batch_update. */
+int dogm_batch_measurement_buffer(dogm_batch* b, int vehicle_id, dogm_measurement_cell** device_ptr);
+/* One frame for `count` distinct vehicles.  Per vehicle i: measurement grid meas[i] (device memory
+ * when is_device != 0; NULL, or meas itself NULL, means the grid already sits in the vehicle's
+ * measurement buffer), ego pose (x[i], y[i], yaw[i]) and time step dt[i].  Blocking. */
+int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const dogm_measurement_cell* const* meas,
+                      const float* x, const float* y, const float* yaw, const float* dt, int is_device);
+int dogm_batch_get_grid_cells(dogm_batch* b, int vehicle_id, dogm_grid_cell* out);
+int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, int* grid_cell_idx, float* weight,
+                             uint8_t* associated);
+int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
+int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable);
+int dogm_batch_get_kernel_timing(dogm_batch* b, int kernel_id, double* total_ms, long long* launches);
+int dogm_batch_reset_kernel_timing(dogm_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
