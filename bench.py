@@ -1,14 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 DOGM update on one MI355X.
 
-One step = one dogm_update_grid call (the full per-frame filter: measurement copy, pose, predict,
-assignment, occupancy, persistent update, births, moments, resampling) on the full-scale workload:
+Measures two forms of the full per-frame filter (measurement copy, pose, predict, assignment,
+occupancy, persistent update, births, moments, resampling) on the full-scale workload:
 1200 x 1200 cells at 0.1 m, 2e6 persistent + 2e5 birth particles, 40 moving boxes seen by a
 1080-beam 60 m lidar, ego at 10 m/s with 0.1 rad/s yaw rate (a grid shift every frame), dt 0.1 s.
 The measurement grids of all frames are generated before timing and are resident in HBM; each step
-consumes the next one.  Prints one JSON line (stdout); a per-kernel breakdown goes to stderr.
+consumes the next one.
+ - plain: one dogm_update_grid call per frame (the reference's updateGrid, blocking);
+ - batched (the headline `value`): K vehicles of the same workload per dogm_batch_update step, each
+   with its own Philox instance, all fed the same frame sequence; frames/s counts K frames per step.
+Prints one JSON line (stdout); per-kernel breakdowns go to stderr.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--cpu-frames 8]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 8] [--cpu-frames 8]
 """
 from __future__ import annotations
 
@@ -97,12 +101,102 @@ def cpu_baseline(workload: str, frames: int, meas_list, poses, dt):
     }
 
 
+def _dominant(per_frame: dict) -> str:
+    return max(per_frame, key=lambda k: per_frame[k][0])
+
+
+def run_plain(args, p, meas_bufs, poses, dt):
+    """One dogm_update_grid per frame; returns timing, the warm-up breakdown and the dominant kernel."""
+    from dogm_ctypes import DOGM, lib
+    g = DOGM(p, seed=123456)
+    names = DOGM.kernel_names()
+    total = args.warmup + args.steps
+    for f in range(args.warmup):
+        if f == 1:
+            for k in names:
+                g.set_kernel_timing(k, True)
+        x, y, yaw = poses[f]
+        g.update_grid(meas_bufs[f], x, y, yaw, dt)
+    breakdown = {k: g.kernel_timing(k) for k in names}
+    g.reset_kernel_timing()
+    for k in names:
+        g.set_kernel_timing(k, False)
+    nwf = args.warmup - 1
+    per_frame = {k: (ms / nwf, n) for k, (ms, n) in breakdown.items() if n}
+    dominant = _dominant(per_frame)
+    g.set_kernel_timing(dominant, True)
+    lib.dogm_device_synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for f in range(args.warmup, total):
+        x, y, yaw = poses[f]
+        t1 = time.perf_counter()
+        g.update_grid(meas_bufs[f], x, y, yaw, dt)
+        lat.append(time.perf_counter() - t1)
+    lib.dogm_device_synchronize()
+    elapsed = time.perf_counter() - t_start
+    dom_ms, dom_n = g.kernel_timing(dominant)
+    g.close()
+    return {"elapsed": elapsed, "frames": args.steps, "lat": lat, "per_frame": per_frame, "dominant": dominant,
+            "dom_avg_s": dom_ms / max(1, dom_n) / 1e3}
+
+
+def run_batched(args, k, p, meas_bufs, poses, dt):
+    """k vehicles of the same workload (Philox instances 0..k-1, the same frame sequence) per
+    dogm_batch_update step."""
+    from dogm_ctypes import Batch, DOGM, lib
+    b = Batch()
+    ids = [b.add_vehicle(p, 123456, i) for i in range(k)]
+    names = DOGM.kernel_names()
+    total = args.warmup + args.steps
+
+    def step(f):
+        x, y, yaw = poses[f]
+        b.update(ids, [meas_bufs[f]] * k, [(x, y, yaw)] * k, dt)
+
+    for f in range(args.warmup):
+        if f == 1:
+            for n in names:
+                b.set_kernel_timing(n, True)
+        step(f)
+    breakdown = {n: b.kernel_timing(n) for n in names}
+    b.reset_kernel_timing()
+    for n in names:
+        b.set_kernel_timing(n, False)
+    nwf = args.warmup - 1
+    per_step = {n: (ms / nwf, c) for n, (ms, c) in breakdown.items() if c}
+    dominant = _dominant(per_step)
+    b.set_kernel_timing(dominant, True)
+    lib.dogm_device_synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for f in range(args.warmup, total):
+        t1 = time.perf_counter()
+        step(f)
+        lat.append(time.perf_counter() - t1)
+    lib.dogm_device_synchronize()
+    elapsed = time.perf_counter() - t_start
+    dom_ms, dom_n = b.kernel_timing(dominant)
+    b.close()
+    return {"elapsed": elapsed, "frames": args.steps * k, "lat": lat, "per_frame": per_step, "dominant": dominant,
+            "dom_avg_s": dom_ms / max(1, dom_n) / 1e3, "vehicles": k}
+
+
+def _summary(r):
+    fps = r["frames"] / r["elapsed"]
+    return {"frames_per_s": round(fps, 3), "ms_per_step": round(1e3 * r["elapsed"] * len(r["lat"]) ** -1, 4),
+            "p50_ms": round(1e3 * float(np.percentile(r["lat"], 50)), 4),
+            "p99_ms": round(1e3 * float(np.percentile(r["lat"], 99)), 4)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="plain_full", choices=sorted(WORKLOAD_TEXT))
+    ap.add_argument("--batch", type=int, default=8,
+                    help="vehicles per batched step (0: plain form only); the batched form is the headline")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -113,6 +207,7 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): every box in this setup has a single "
               "MI355X; see DESIGN.md for how the update shards by vehicle across GPUs.", file=sys.stderr)
         return 2
+    assert args.warmup >= 2, "--warmup must be at least 2"
 
     import scene
     from dogm_ctypes import DOGM, DeviceBuffer, Params, lib
@@ -127,9 +222,6 @@ def main() -> int:
     poses = [sc.ego_pose(f) for f in range(total)]
     print(f"[bench] generated {total} measurement grids in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
-
-    g = DOGM(p, seed=123456)
-    N, B, C = g.particle_count, g.new_born_particle_count, g.grid_cell_count
     bufs = []
     for m in meas:
         d = DeviceBuffer(m.nbytes)
@@ -144,47 +236,33 @@ def main() -> int:
         src.free()
         dst.free()
 
-    names = DOGM.kernel_names()
-    # Warm-up frames; after the first (which also initialises the particles) every kernel is
-    # timed to find the dominant one.
-    assert args.warmup >= 2, "--warmup must be at least 2"
-    for f in range(args.warmup):
-        if f == 1:
-            for k in names:
-                g.set_kernel_timing(k, True)
-        x, y, yaw = poses[f]
-        g.update_grid(bufs[f], x, y, yaw, dt)
-    breakdown = {k: g.kernel_timing(k) for k in names}
-    g.reset_kernel_timing()
-    for k in names:
-        g.set_kernel_timing(k, False)
-    nwf = args.warmup - 1
-    per_frame = {k: (ms / nwf, n) for k, (ms, n) in breakdown.items() if n}
-    dominant = max(per_frame, key=lambda k: per_frame[k][0])
-    # Only the dominant kernel carries events inside the timed region.
-    g.set_kernel_timing(dominant, True)
+    gs = int(np.float32(p.size) / np.float32(p.resolution))
+    N, B, C = p.particle_count, p.new_born_particle_count, gs * gs
 
-    lib.dogm_device_synchronize()
-    lat = []
-    t_start = time.perf_counter()
-    for f in range(args.warmup, total):
-        x, y, yaw = poses[f]
-        t1 = time.perf_counter()
-        g.update_grid(bufs[f], x, y, yaw, dt)
-        lat.append(time.perf_counter() - t1)
-    lib.dogm_device_synchronize()
-    elapsed = time.perf_counter() - t_start
+    plain = run_plain(args, p, bufs, poses, dt)
+    print(f"[bench] plain: {_summary(plain)}", file=sys.stderr, flush=True)
+    head = plain
+    batched = None
+    if args.batch > 0:
+        batched = run_batched(args, args.batch, p, bufs, poses, dt)
+        print(f"[bench] batched x{args.batch}: {_summary(batched)}", file=sys.stderr, flush=True)
+        head = batched
 
-    fps = args.steps / elapsed
-    dom_ms, dom_n = g.kernel_timing(dominant)
-    dom_avg_s = dom_ms / max(1, dom_n) / 1e3
+    for tag, r in (("plain", plain), ("batched", batched)):
+        if r is None:
+            continue
+        print(f"[bench] {tag}: per-kernel ms per warm-up step (launches):", file=sys.stderr)
+        for k, (ms, n) in sorted(r["per_frame"].items(), key=lambda kv: -kv[1][0]):
+            print(f"  {k:18s} {ms:8.3f} ms  ({n} launches)", file=sys.stderr)
+
+    kveh = head.get("vehicles", 1)
+    fps = head["frames"] / head["elapsed"]
+    dominant = head["dominant"]
+    dom_avg_s = head["dom_avg_s"]
     passes = max(1, (int(np.ceil(np.log2(max(2, C)))) + 7) // 8)
-    dbytes = kernel_bytes(dominant, N, B, C, passes)
+    kb = kernel_bytes(dominant, N, B, C, passes)
+    dbytes = kb * kveh if kb else None
     achieved = (dbytes / dom_avg_s / 1e9) if (dbytes and dom_avg_s > 0) else None
-
-    print("[bench] per-kernel ms per warm-up frame (launches):", file=sys.stderr)
-    for k, (ms, n) in sorted(per_frame.items(), key=lambda kv: -kv[1][0]):
-        print(f"  {k:18s} {ms:8.3f} ms  ({n} launches)", file=sys.stderr)
 
     shifted = args.workload == "plain_full"
     fb = frame_bytes(N, B, C, shifted)
@@ -192,6 +270,7 @@ def main() -> int:
     if args.cpu_frames > 0:
         cpu = cpu_baseline(args.workload, min(args.cpu_frames, total), meas, poses, dt)
 
+    hs = _summary(head)
     out = {
         "metric": METRIC,
         "value": round(fps, 3),
@@ -199,7 +278,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": hs["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(fps / PUBLISHED_FPS, 3) if args.workload == "plain_full" else None,
@@ -207,13 +286,17 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": WORKLOAD_TEXT[args.workload],
-            "form": "plain (one updateGrid call per frame, blocking)",
-            "particles": N, "birth_particles": B, "cells": C,
+            "form": (f"batched: {kveh} vehicles of this workload per dogm_batch_update step (blocking), each with "
+                     "its own Philox instance" if batched else "plain (one updateGrid call per frame, blocking)"),
+            "particles_per_vehicle": N, "birth_particles_per_vehicle": B, "cells_per_vehicle": C,
             "measurement_in_hbm": True,
             "baseline_source": "reference README: 50 ms/frame at 2e6/2e5/1.44e6 (GTX 980, Nuss et al.)",
         },
-        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 4),
-        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 4),
+        "p50_ms": hs["p50_ms"],
+        "p99_ms": hs["p99_ms"],
+        "plain": _summary(plain),
+        "batched": _summary(batched) if batched else None,
+        "batched_vs_plain": round(fps / (plain["frames"] / plain["elapsed"]), 3) if batched else None,
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),
@@ -233,7 +316,6 @@ def main() -> int:
     print(json.dumps(out))
     for d in bufs:
         d.free()
-    g.close()
     return 0
 
 

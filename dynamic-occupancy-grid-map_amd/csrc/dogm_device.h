@@ -9,15 +9,6 @@ namespace dogm_hip
 
 using namespace dogm_num;
 
-__device__ __forceinline__ fx_t shfl_up_fx(fx_t x, int d)
-{
-    long long lo = (long long)(uint64_t)x;
-    long long hi = (long long)(x >> 64);
-    lo = __shfl_up(lo, d, WAVE);
-    hi = __shfl_up(hi, d, WAVE);
-    return (fx_t)(((ufx_t)(uint64_t)hi << 64) | (ufx_t)(uint64_t)lo);
-}
-
 __device__ __forceinline__ fx_t shfl_fx(fx_t x, int src)
 {
     long long lo = (long long)(uint64_t)x;
@@ -27,21 +18,43 @@ __device__ __forceinline__ fx_t shfl_fx(fx_t x, int src)
     return (fx_t)(((ufx_t)(uint64_t)hi << 64) | (ufx_t)(uint64_t)lo);
 }
 
+// DPP move of the four dwords of an fx value (lanes whose source is outside the pattern, or whose
+// row is masked off, read 0).  CTRL / ROW_MASK are DPP encodings: row_shr:n = 0x110 + n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ fx_t dpp_fx(fx_t x)
+{
+    const ufx_t u = (ufx_t)x;
+    const int w0 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    const int w1 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    const int w2 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 64), CTRL, ROW_MASK, 0xf, false);
+    const int w3 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 96), CTRL, ROW_MASK, 0xf, false);
+    return (fx_t)(((ufx_t)(uint32_t)w3 << 96) | ((ufx_t)(uint32_t)w2 << 64) | ((ufx_t)(uint32_t)w1 << 32) |
+                  (ufx_t)(uint32_t)w0);
+}
+
+// Inclusive scan over the 64 lanes of a wave: Hillis-Steele inside rows of 16 lanes (row_shr 1, 2,
+// 4, 8), then the row totals through row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2 and 3).
+// Integer adds: the result does not depend on the association.
+__device__ __forceinline__ fx_t wave_inclusive_scan_fx(fx_t x)
+{
+    x += dpp_fx<0x111, 0xf>(x);
+    x += dpp_fx<0x112, 0xf>(x);
+    x += dpp_fx<0x114, 0xf>(x);
+    x += dpp_fx<0x118, 0xf>(x);
+    x += dpp_fx<0x142, 0xa>(x);
+    x += dpp_fx<0x143, 0xc>(x);
+    return x;
+}
+
 // Exclusive block scan of one fx value per thread (BLOCK threads); returns the exclusive prefix
-// and writes the block total.  Integer adds: the result does not depend on the association.
+// and writes the block total.
 __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
 {
     __shared__ fx_t s_wave[BLOCK / WAVE];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
-    fx_t incl = v;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1)
-    {
-        const fx_t u = shfl_up_fx(incl, d);
-        if (lane >= d)
-            incl += u;
-    }
+    const fx_t incl = wave_inclusive_scan_fx(v);
     if (lane == WAVE - 1)
         s_wave[wid] = incl;
     __syncthreads();
@@ -142,18 +155,7 @@ __device__ __forceinline__ void block_exclusive_scan_fx_n(const fx_t (&v)[NCH], 
     fx_t incl[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
-        incl[c] = v[c];
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1)
-    {
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-        {
-            const fx_t u = shfl_up_fx(incl[c], d);
-            if (lane >= d)
-                incl[c] += u;
-        }
-    }
+        incl[c] = wave_inclusive_scan_fx(v[c]);
     if (lane == WAVE - 1)
     {
 #pragma unroll

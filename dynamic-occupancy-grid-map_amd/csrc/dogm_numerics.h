@@ -266,31 +266,32 @@ DOGM_HD int clz128(ufx_t v)
 }
 
 // Exact (truncating toward zero below 2^-88).  |v| >= 2^39, inf and NaN saturate to the
-// largest representable magnitude with the sign of v (NaN counts as +).
+// largest representable magnitude with the sign of v (NaN counts as +).  Written on 64-bit halves:
+// the 24-bit significand shifted by sh = E - 62 lands in at most two adjacent words.
 DOGM_HD fx_t fx_from_f32(float v)
 {
     const uint32_t b = f32_bits(v);
     const uint32_t E = (b >> 23) & 0xffu;
-    if (E == 0)
-        return 0;  // zero and subnormals are < 2^-126 < 2^-88
-    const bool neg = (b >> 31) != 0;
-    ufx_t mag;
-    const int sh = (int)E - 150 + FX_FRAC;  // value = m * 2^(E - 150)
-    if (E == 0xffu || sh > 103)
-    {
-        mag = (((ufx_t)1) << 126);
-    }
-    else
-    {
-        const uint32_t m = (b & 0x007fffffu) | 0x00800000u;
-        if (sh >= 0)
-            mag = ((ufx_t)m) << sh;
-        else if (sh > -32)
-            mag = (ufx_t)(m >> (-sh));
-        else
-            mag = 0;
-    }
-    return neg ? -(fx_t)mag : (fx_t)mag;
+    const int sh = (int)E - 150 + FX_FRAC;  // value = m * 2^(E - 150), fixed point = m << sh
+    const uint64_t m = (uint64_t)((b & 0x007fffffu) | 0x00800000u);
+    // Candidates for the three shift ranges (every shift count kept in [0, 63]); E >= 1 gives
+    // sh >= -61, and E = 0 is cleared below.
+    const uint64_t hi_a = m << ((sh - 64) & 63);   // 64 <= sh <= 103
+    const uint64_t lo_b = m << (sh & 63);          // 0 <= sh < 64
+    const uint64_t hi_b = m >> ((64 - sh) & 63);   // 41 <= sh < 64
+    const uint64_t lo_c = m >> ((-sh) & 63);       // sh < 0
+    uint64_t lo = sh >= 64 ? 0 : (sh >= 0 ? lo_b : lo_c);
+    uint64_t hi = sh >= 64 ? hi_a : (sh > 40 ? hi_b : 0);
+    const bool sat = E == 0xffu || sh > 103;
+    lo = sat ? 0 : lo;
+    hi = sat ? (1ull << 62) : hi;  // 2^126
+    lo = E == 0 ? 0 : lo;          // zero and subnormals are < 2^-126 < 2^-88
+    hi = E == 0 ? 0 : hi;
+    // Two's-complement negation for negative v: (x ^ s) - s with s = all ones.
+    const uint64_t sgn = 0ull - (uint64_t)(b >> 31);
+    const uint64_t nlo = (lo ^ sgn) - sgn;
+    const uint64_t nhi = (hi ^ sgn) - sgn - ((lo ^ sgn) < sgn ? 1u : 0u);
+    return (fx_t)(((ufx_t)nhi << 64) | (ufx_t)nlo);
 }
 
 DOGM_HD fx_t fx_from_f64(double v)
@@ -352,18 +353,50 @@ DOGM_HD void fx_round(ufx_t mag, int mbits, uint64_t* mant_out, int* exp_out)
     *exp_out = e;
 }
 
+// Round-to-nearest-even of raw * 2^-88 to f32: the magnitude is normalised into one 64-bit word
+// (leading bit at 63) plus a sticky bit for what falls below it; the significand is its top 24 bits.
 DOGM_HD float fx_to_f32(fx_t v)
 {
-    if (v == 0)
+    const ufx_t u = (ufx_t)v;
+    uint64_t hi = (uint64_t)(u >> 64), lo = (uint64_t)u;
+    const uint64_t sgn = 0ull - (hi >> 63);
+    const uint64_t borrow = (lo ^ sgn) < sgn ? 1u : 0u;
+    lo = (lo ^ sgn) - sgn;
+    hi = (hi ^ sgn) - sgn - borrow;
+    if ((hi | lo) == 0)
         return 0.0f;
-    const bool neg = v < 0;
-    const ufx_t mag = neg ? (ufx_t)0 - (ufx_t)v : (ufx_t)v;
-    uint64_t mant;
-    int e;
-    fx_round(mag, 24, &mant, &e);
-    // e in [-88, 39]: always a normal f32.
-    const uint32_t b = ((neg ? 1u : 0u) << 31) | ((uint32_t)(e + 127) << 23) | ((uint32_t)mant & 0x007fffffu);
-    return f32_from_bits(b);
+    uint64_t t;
+    bool sticky;
+    int msb;
+    if (hi != 0)
+    {
+        const int z = __builtin_clzll(hi);
+        t = z ? (hi << z) | (lo >> (64 - z)) : hi;
+        sticky = (z ? (lo << z) : lo) != 0;
+        msb = 127 - z;
+    }
+    else
+    {
+        const int z = __builtin_clzll(lo);
+        t = lo << z;
+        sticky = false;
+        msb = 63 - z;
+    }
+    int e = msb - FX_FRAC;  // in [-88, 39]: always a normal f32
+    uint32_t mant = (uint32_t)(t >> 40);
+    const uint64_t rem = t & ((1ull << 40) - 1);
+    const uint64_t half = 1ull << 39;
+    if (rem > half || (rem == half && (sticky || (mant & 1u))))
+    {
+        mant += 1;
+        if (mant == (1u << 24))
+        {
+            mant >>= 1;
+            e += 1;
+        }
+    }
+    const uint32_t bits = ((uint32_t)(sgn & 1u) << 31) | ((uint32_t)(e + 127) << 23) | (mant & 0x007fffffu);
+    return f32_from_bits(bits);
 }
 
 DOGM_HD double fx_to_f64(fx_t v)
