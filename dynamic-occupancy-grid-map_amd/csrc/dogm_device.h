@@ -175,6 +175,38 @@ __device__ __forceinline__ void block_exclusive_scan_fx_n(const fx_t (&v)[NCH], 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Radix digits (8 bits per pass) and wave-level digit counting
+// ---------------------------------------------------------------------------------------------
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Adds one to s_h[d] for every active lane.  A wave whose keys share the digit (the usual case
+// for the upper digits: resampled particles arrive in cell order) adds once; otherwise lanes add
+// individually.
+__device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid)
+{
+    const uint64_t active = __ballot(valid);
+    if (active == 0)
+        return;
+    const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
+    if ((__ballot(d == first) & active) == active)
+    {
+        if (valid && (active & lanemask_lt()) == 0)
+            atomicAdd(&s_h[d], (unsigned)__popcll(active));
+    }
+    else if (valid)
+    {
+        atomicAdd(&s_h[d], 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Work-table lookup: the vehicle and local element base of this workgroup's tile of q units.
 // ---------------------------------------------------------------------------------------------
 struct TileRef

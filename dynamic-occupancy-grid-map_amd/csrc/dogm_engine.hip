@@ -409,23 +409,27 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     for (int i = 0; i < m; ++i)
         any_init = any_init || ((stages & ST_INIT) && veh[i]->init_pending);
     int pph = 0;
+    const bool hist = (stages & ST_PREDICT) && (stages & ST_ASSIGN);
+    const bool occ_pers = (stages & ST_OCC) && (stages & ST_PERS);
+    const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
+    const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
     if (any_init)
         run_init_particles(E);
     if (stages & ST_PREDICT)
-        run_predict(E, pph);
+        run_predict(E, pph, hist);
     if (stages & ST_ASSIGN)
     {
-        run_assignment(E, pph);
+        run_assignment(E, pph, hist);
         pph ^= 1;
     }
     if (stages & ST_OCC)
-        run_occupancy(E, pph);
+        run_occupancy(E, pph, zero_empty, occ_pers);
     if (stages & ST_PERS)
-        run_persistent(E, pph);
+        run_persistent(E, pph, occ_pers, (zero_empty & 1) != 0, pers_mom);
     if (stages & ST_BIRTH)
         run_births(E);
     if (stages & ST_MOMENTS)
-        run_moments(E, pph);
+        run_moments(E, pph, pers_mom, (zero_empty & 2) != 0);
     if (stages & ST_RESAMPLE)
     {
         run_resampling(E, pph);

@@ -325,15 +325,19 @@ void vehicle_free(Vehicle* v);
 // `pph` is the index (0 / 1) of the current particle buffer of the step's descriptors.
 void run_fill_u32(Engine* E, unsigned* p, unsigned v, long long n);
 void run_build_tables(Engine* E);
+// Stages that run in the same step share work: predict counts the sort's digit histogram (hist),
+// the occupancy update zeroes the persistent / moment fields of empty cells (zero_empty bits 0 / 1),
+// the weight normalisation moves into the persistent update's load and the posterior weights into
+// the moments' load.  Each stage run alone keeps the reference's separate passes.
 void run_init_particles(Engine* E);                // pph 0
-void run_predict(Engine* E, int pph);
-void run_assignment(Engine* E, int pph);           // reads pph, writes pph ^ 1
-void run_occupancy(Engine* E, int pph);            // reads free[0], writes free[1]
-void run_persistent(Engine* E, int pph);
+void run_predict(Engine* E, int pph, bool hist);
+void run_assignment(Engine* E, int pph, bool hist_done);  // reads pph, writes pph ^ 1
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize);  // free[0] -> free[1]
+void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
 void run_births(Engine* E);
-void run_moments(Engine* E, int pph);
+void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
 void run_resampling(Engine* E, int pph);           // reads pph, writes pph ^ 1
-void run_sort(Engine* E, int pph);                 // per vehicle: sorted keys / permutation
+void run_sort(Engine* E, int pph, bool hist_done); // per vehicle: sorted keys / permutation
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);

@@ -144,57 +144,94 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
     return 0;
 }
 
+constexpr int PRED_ITEMS = 8;  // particles per thread (strided by BLOCK)
+
+// With passes > 0 the workgroup also counts the radix digits of the new cell indices for every
+// sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram).
 __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
-                                                   int pph)
+                                                   int pph, int passes, unsigned* __restrict__ ghist)
 {
-    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    __shared__ unsigned s_h[4][RADIX];
+    const TileRef tr = tile_ref(tab, PRED_ITEMS, blockIdx.x);
     const VehDesc& d = descs[tr.v];
-    const int i = tr.base + threadIdx.x;
-    if (i >= d.N)
+    const int n = d.N;
+    if (tr.base >= n)
         return;
+    if (passes > 0)
+    {
+        for (int i = threadIdx.x; i < 4 * RADIX; i += BLOCK)
+            (&s_h[0][0])[i] = 0u;
+        __syncthreads();
+    }
     const DevParticles P = d.part[pph];
     const int gs = d.gs;
     const float dt = d.dt;
-    float4 s = P.state[i];
-    const float w_in = P.weight[i];
-    if (d.move)
-    {
-        s.x = s.x - d.mx;
-        s.y = s.y - d.my;
-    }
-    const u32x4 r = draw(d.seed, d.inst, TAG_PREDICT, d.ep_pred, (uint32_t)i);
-    float n0, n1, n2, n3;
-    box_muller(r.x, r.y, &n0, &n1);
-    box_muller(r.z, r.w, &n2, &n3);
     const float zero = 0.0f, one = 1.0f;
-    const float npx = normal_scale(n0, zero, d.sp), npy = normal_scale(n1, zero, d.sp);
-    const float nvx = normal_scale(n2, zero, d.sv), nvy = normal_scale(n3, zero, d.sv);
-    // glm mat4 * vec4 association ((c0 x + c1 y) + (c2 vx + c3 vy)) of the transposed constant
-    // velocity matrix (dogm.cu:246-253).
-    const float sx = (one * s.x + zero * s.y) + (dt * s.z + zero * s.w);
-    const float sy = (zero * s.x + one * s.y) + (zero * s.z + dt * s.w);
-    const float svx = (zero * s.x + zero * s.y) + (one * s.z + zero * s.w);
-    const float svy = (zero * s.x + zero * s.y) + (zero * s.z + one * s.w);
-    float4 o;
-    o.x = sx + npx;
-    o.y = sy + npy;
-    o.z = svx + nvx;
-    o.w = svy + nvy;
-    float w = d.ps * w_in;
-    if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
-        w = zero;
-    P.state[i] = o;
-    P.weight[i] = w;
-    P.idx[i] = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
+    for (int it = 0; it < PRED_ITEMS; ++it)
+    {
+        const int i = tr.base + it * BLOCK + threadIdx.x;
+        const bool valid = i < n;
+        int cell = 0;
+        if (valid)
+        {
+            float4 s = P.state[i];
+            const float w_in = P.weight[i];
+            if (d.move)
+            {
+                s.x = s.x - d.mx;
+                s.y = s.y - d.my;
+            }
+            const u32x4 r = draw(d.seed, d.inst, TAG_PREDICT, d.ep_pred, (uint32_t)i);
+            float n0, n1, n2, n3;
+            box_muller(r.x, r.y, &n0, &n1);
+            box_muller(r.z, r.w, &n2, &n3);
+            const float npx = normal_scale(n0, zero, d.sp), npy = normal_scale(n1, zero, d.sp);
+            const float nvx = normal_scale(n2, zero, d.sv), nvy = normal_scale(n3, zero, d.sv);
+            // glm mat4 * vec4 association ((c0 x + c1 y) + (c2 vx + c3 vy)) of the transposed
+            // constant velocity matrix (dogm.cu:246-253).
+            const float sx = (one * s.x + zero * s.y) + (dt * s.z + zero * s.w);
+            const float sy = (zero * s.x + one * s.y) + (zero * s.z + dt * s.w);
+            const float svx = (zero * s.x + zero * s.y) + (one * s.z + zero * s.w);
+            const float svy = (zero * s.x + zero * s.y) + (zero * s.z + one * s.w);
+            float4 o;
+            o.x = sx + npx;
+            o.y = sy + npy;
+            o.z = svx + nvx;
+            o.w = svy + nvy;
+            float w = d.ps * w_in;
+            if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
+                w = zero;
+            cell = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
+            P.state[i] = o;
+            P.weight[i] = w;
+            P.idx[i] = cell;
+        }
+        for (int ps = 0; ps < passes; ++ps)
+            wave_count(s_h[ps], ((unsigned)cell >> (ps * RADIX_BITS)) & (RADIX - 1), valid);
+    }
+    if (passes > 0)
+    {
+        __syncthreads();
+        unsigned* gh = ghist + (size_t)tr.v * 4 * RADIX;
+        for (int i = threadIdx.x; i < passes * RADIX; i += BLOCK)
+        {
+            const unsigned v = (&s_h[0][0])[i];
+            if (v)
+                atomicAdd(&gh[i], v);
+        }
+    }
 }
 
-void run_predict(Engine* E, int pph)
+void run_predict(Engine* E, int pph, bool hist)
 {
-    const int blocks = E->tiles(SP_P, 1);
+    const int blocks = E->tiles(SP_P, PRED_ITEMS);
     if (blocks <= 0)
         return;
+    if (hist)
+        run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
     launch_begin(E, K_PREDICT);
-    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph);
+    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                       hist ? E->passes : 0, E->sort_ghist);
     launch_end(E, K_PREDICT);
 }
 
@@ -241,12 +278,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
         d.end[key] = k;
 }
 
-void run_assignment(Engine* E, int pph)
+void run_assignment(Engine* E, int pph, bool hist_done)
 {
     launch_begin(E, K_RESET_BOUNDS);
     hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
     launch_end(E, K_RESET_BOUNDS);
-    run_sort(E, pph);
+    run_sort(E, pph, hist_done);
     launch_begin(E, K_SORT_GATHER);
     hipLaunchKernelGGL(k_sort_gather, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
                        pph, (E->passes - 1) & 1);
@@ -273,8 +310,11 @@ struct OccOp
     __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[1]) const { d.m_raw[key] = fx_to_f32(s[0]); }
 };
 
+// zero_empty: bit 0 writes mu_A = mu_UA = 0, bit 1 the five velocity moments = 0 for cells without
+// particles (what the persistent-update and moments stages write there), when those stages run
+// in the same step.
 __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
-                                                       const uint32_t* __restrict__ tab)
+                                                       const uint32_t* __restrict__ tab, int zero_empty)
 {
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -285,8 +325,20 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
     const float* free_src = d.free[0];
     const float one = 1.0f;
     float m_occ_pred = 0.0f;
-    if (d.start[c] != -1)
+    const bool occupied = d.start[c] != -1;
+    if (occupied)
         m_occ_pred = d.m_raw[c];
+    else if (zero_empty)
+    {
+        if (zero_empty & 1)
+        {
+            d.f[GF_MU_A][c] = 0.0f;
+            d.f[GF_MU_UA][c] = 0.0f;
+        }
+        if (zero_empty & 2)
+            for (int f = GF_MEAN_X; f <= GF_COVAR; ++f)
+                d.f[f][c] = 0.0f;
+    }
     if (m_occ_pred > one)
         m_occ_pred = one;
     float fprev;
@@ -339,12 +391,15 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
     }
 }
 
-void run_occupancy(Engine* E, int pph)
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize)
 {
     launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph});
     launch_begin(E, K_MASS_UPDATE);
-    hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
+    hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
+                       zero_empty);
     launch_end(E, K_MASS_UPDATE);
+    if (defer_normalize)
+        return;
     launch_begin(E, K_NORMALIZE);
     hipLaunchKernelGGL(k_normalize, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
                        pph);
@@ -355,16 +410,36 @@ void run_occupancy(Engine* E, int pph)
 // Persistent particle update (dogm/src/dogm.cu:300-321, update_persistent_particles.cu:16-87)
 // ---------------------------------------------------------------------------------------------
 // Kernel1 (lw = likelihood * w, written to weight_array) is fused into the reduction's load.
+// norm: normalize_weights (mass_update.cu:51-59) is applied here, when the occupancy update ran in the
+// same step: w <- w / m_occ_pred for cells whose predicted occupancy exceeds 1 (the particle weight is
+// rewritten; weight_array is overwritten by lw below either way).
 struct PersOp
 {
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
     int pph;
+    int norm;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS], lw[ITEMS];
         load_run<ITEMS>(d.part[pph].weight, p0, cnt, w);
+        if (norm)
+        {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+            {
+                if (i < cnt)
+                {
+                    const float m = d.m_raw[key[i]];
+                    if (m > 1.0f)
+                    {
+                        w[i] = w[i] / m;
+                        d.part[pph].weight[p0 + i] = w[i];
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
         {
@@ -410,13 +485,18 @@ __global__ __launch_bounds__(BLOCK) void k_post_weights(const VehDesc* __restric
     d.weight_array[i] = pA * d.f[GF_MU_A][c] * d.weight_array[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * P.weight[i];
 }
 
-void run_persistent(Engine* E, int pph)
+void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post)
 {
-    launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp{pph});
-    launch_begin(E, K_EMPTY_CELLS);
-    hipLaunchKernelGGL(k_empty_cells, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       (int)GF_MU_A, (int)GF_MU_UA);
-    launch_end(E, K_EMPTY_CELLS);
+    launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp{pph, fused_normalize ? 1 : 0});
+    if (!empty_done)
+    {
+        launch_begin(E, K_EMPTY_CELLS);
+        hipLaunchKernelGGL(k_empty_cells, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                           E->tab(SP_C), (int)GF_MU_A, (int)GF_MU_UA);
+        launch_end(E, K_EMPTY_CELLS);
+    }
+    if (defer_post)
+        return;
     launch_begin(E, K_POST_WEIGHTS);
     hipLaunchKernelGGL(k_post_weights, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
                        pph);
@@ -589,16 +669,36 @@ void run_init_particles(Engine* E)
 // ---------------------------------------------------------------------------------------------
 // Statistical moments (dogm/src/dogm.cu:350-384, statistical_moments.cu:16-106)
 // ---------------------------------------------------------------------------------------------
+// post: the posterior weights (update_persistent_particles.cu:76-87) are formed here, when the
+// persistent update ran in the same step: w = p_A mu_A lw + (1 - p_A) mu_UA w_prior, written to
+// weight_array and used for the moments.
 struct MomOp
 {
     static constexpr int NCH = 5;
     static constexpr int ITEMS = 4;
     int pph;
+    int post;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
-    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
+    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];
         load_run<ITEMS>(d.weight_array, p0, cnt, w);
+        if (post)
+        {
+            float wp[ITEMS];
+            load_run<ITEMS>(d.part[pph].weight, p0, cnt, wp);
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+            {
+                if (i < cnt)
+                {
+                    const int c = key[i];
+                    const float pA = d.meas[c].p_A;
+                    w[i] = pA * d.f[GF_MU_A][c] * w[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * wp[i];
+                }
+            }
+            store_run<ITEMS>(d.weight_array, p0, cnt, w);
+        }
         const float4* state = d.part[pph].state;
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
@@ -632,9 +732,11 @@ struct MomOp
     }
 };
 
-void run_moments(Engine* E, int pph)
+void run_moments(Engine* E, int pph, bool fused_post, bool empty_done)
 {
-    launch_segreduce(E, K_SEG_MOM, K_SEG_MOM_FIX, MomOp{pph});
+    launch_segreduce(E, K_SEG_MOM, K_SEG_MOM_FIX, MomOp{pph, fused_post ? 1 : 0});
+    if (empty_done)
+        return;
     launch_begin(E, K_EMPTY_CELLS);
     hipLaunchKernelGGL(k_empty_cells, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
                        (int)GF_MEAN_X, (int)GF_COVAR);

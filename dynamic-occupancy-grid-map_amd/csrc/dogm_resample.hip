@@ -113,6 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
     __shared__ fx_t s_cdf[RS_SLICE];
     __shared__ fx_t s_rb[2];
     __shared__ int s_cnt[1];
+    __shared__ int s_kmin;
 
     const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -158,10 +159,11 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
         anc[i] = -1;
-    for (int tc = t_lo; tc <= t_hi; tc += RS_SLICE_TILES)
+    for (int tc = t_lo; tc <= t_hi;)
     {
+        const int tc_end = min(tc + RS_SLICE_TILES, t_hi + 1) - 1;
         const int c0 = tc * SCAN_TILE;
-        const int clen = min(J, min(tc + RS_SLICE_TILES, t_hi + 1) * SCAN_TILE) - c0;
+        const int clen = min(J, (tc_end + 1) * SCAN_TILE) - c0;
         __syncthreads();
         for (int i = threadIdx.x; i < clen; i += BLOCK)
             s_cdf[i] = cdf[c0 + i];
@@ -204,6 +206,35 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
                 anc[i] = c0 + lo;
             }
         }
+        // Next chunk: the one holding the smallest unresolved draw.  Tiles no draw lands in (long
+        // runs of zero weights) are skipped with one coarse-index search instead of being staged.
+        if (tc_end >= t_hi)
+            break;
+        if (threadIdx.x == 0)
+            s_kmin = n;
+        __syncthreads();
+        int mine = n;
+#pragma unroll
+        for (int i = RS_ITEMS - 1; i >= 0; --i)
+            if (anc[i] < 0 && p0 + i < n)
+                mine = p0 + i;
+        if (mine < n)
+            atomicMin(&s_kmin, mine);
+        __syncthreads();
+        const int kmin = s_kmin;
+        if (kmin >= n)
+            break;
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+            if (p0 + i == kmin)
+                s_rb[0] = r[i];
+        __syncthreads();
+        const fx_t rmin = s_rb[0];
+        const int next_end = min(tc_end + RS_SLICE_TILES, t_hi);
+        if (!(incl[next_end] < rmin))
+            tc = tc_end + 1;
+        else
+            tc = max(tc_end + 1, find_cdf_tile(incl, T, rmin, s_cnt));
     }
     // Transpose through LDS so the gather and the stores run wave-contiguous.
     __syncthreads();

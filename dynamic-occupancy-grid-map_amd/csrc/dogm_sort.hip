@@ -14,18 +14,10 @@
 namespace dogm_hip
 {
 
-constexpr int RADIX_BITS = 8;
-constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int SORT_WAVES = BLOCK / WAVE;
 constexpr int SORT_Q = SORT_TILE / UNIT;  // units per sort tile
-
-__device__ __forceinline__ uint64_t lanemask_lt()
-{
-    const int lane = threadIdx.x & (WAVE - 1);
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
 
 // Lanes of the wave holding the same digit (among `active` lanes).  Nearly sorted keys usually
 // give one digit per wave: that case costs one ballot.
@@ -45,26 +37,6 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, uint64_t active)
         peers &= bit ? m : ~m;
     }
     return peers;
-}
-
-// Adds one to s_h[d] for every active lane.  A wave whose keys share the digit (the usual case
-// for the upper digits: resampled particles arrive in cell order) adds once; otherwise lanes add
-// individually.
-__device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid)
-{
-    const uint64_t active = __ballot(valid);
-    if (active == 0)
-        return;
-    const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
-    if ((__ballot(d == first) & active) == active)
-    {
-        if (valid && (active & lanemask_lt()) == 0)
-            atomicAdd(&s_h[d], (unsigned)__popcll(active));
-    }
-    else if (valid)
-    {
-        atomicAdd(&s_h[d], 1u);
-    }
 }
 
 // Histogram of every pass's digit over all keys, per vehicle: ghist[(v * 4 + pass) * 256 + digit].
@@ -298,17 +270,20 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
 
 static_assert(BLOCK == RADIX, "k_sort_downsweep assigns one digit per thread");
 
-void run_sort(Engine* E, int pph)
+void run_sort(Engine* E, int pph, bool hist_done)
 {
     const int tiles = E->tiles(SP_P, SORT_Q);
     const int passes = E->passes;
     if (tiles <= 0)
         return;
-    run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
-    launch_begin(E, K_SORT_HIST);
-    hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, passes,
-                       E->sort_ghist);
-    launch_end(E, K_SORT_HIST);
+    if (!hist_done)
+    {
+        run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
+        launch_begin(E, K_SORT_HIST);
+        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, passes,
+                           E->sort_ghist);
+        launch_end(E, K_SORT_HIST);
+    }
 
     for (int ps = 0; ps < passes; ++ps)
     {
