@@ -73,72 +73,6 @@ int run_one(Engine* E, Vehicle* v, int stages, const char* stage)
     return finish(E, stage);
 }
 
-int copy_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device)
-{
-    HIP_TRY(E, hipMemcpyAsync(v->meas, m, sizeof(dogm_measurement_cell) * (size_t)v->C,
-                              is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, E->stream));
-    // dogm.cu:205-215: the first measurement initialises the particles.
-    if (!v->first_meas)
-    {
-        v->init_pending = true;
-        v->first_meas = true;
-    }
-    return DOGM_OK;
-}
-
-void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
-{
-    auto shift_ptr = [&](void** field) {
-        run_shift_field(E, v, *field, v->field_tmp, ix, iy);
-        void* old = *field;
-        *field = v->field_tmp;
-        v->field_tmp = static_cast<float*>(old);
-    };
-    shift_ptr(reinterpret_cast<void**>(&v->grid.start));
-    shift_ptr(reinterpret_cast<void**>(&v->grid.end));
-    for (int f = 0; f < GF_COUNT; ++f)
-        shift_ptr(reinterpret_cast<void**>(&v->grid.f[f]));
-}
-
-// dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
-// occupancy update (free mass, the only grid field read before being rewritten in a frame).
-void update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
-{
-    if (!v->first_pose)
-    {
-        v->pos_x = new_x;
-        v->pos_y = new_y;
-        v->yaw = new_yaw;
-        v->first_pose = true;
-        return;
-    }
-    const float x_diff = new_x - v->pos_x;
-    const float y_diff = new_y - v->pos_y;
-    const float res = v->params.resolution;
-    if (std::fabs(x_diff) > res || std::fabs(y_diff) > res)
-    {
-        const int x_move = -static_cast<int>(x_diff / res);
-        const int y_move = -static_cast<int>(y_diff / res);
-        if (lazy)
-        {
-            v->pend_part_shift = true;
-            v->pend_px = (float)x_move;
-            v->pend_py = (float)y_move;
-            v->pend_grid_shift = true;
-            v->pend_gx = x_move;
-            v->pend_gy = y_move;
-        }
-        else
-        {
-            run_move_particles(E, v, (float)x_move, (float)y_move);
-            shift_grid_eager(E, v, x_move, y_move);
-        }
-        v->pos_x = new_x;
-        v->pos_y = new_y;
-        v->yaw = new_yaw;
-    }
-}
-
 int copy_out(Engine* E, void* dst, const void* src, size_t bytes)
 {
     if (bytes == 0)
@@ -295,7 +229,7 @@ int dogm_create_ex(const dogm_params* params, uint64_t seed, uint32_t instance, 
     dogm_handle* h = new (std::nothrow) dogm_handle();
     if (!h)
         return DOGM_ERR_OUT_OF_MEMORY;
-    if ((st = engine_init(&h->eng, &why)) != DOGM_OK || (st = vehicle_alloc(&h->eng, &h->veh, params, seed, instance)))
+    if ((st = engine_init(&h->eng, &why)) != DOGM_OK || (st = vehicle_alloc(&h->veh, params, seed, instance, &h->eng.last_error)))
     {
         release(h);
         return st;
@@ -329,10 +263,10 @@ int dogm_update_grid(dogm_handle* h, const dogm_measurement_cell* meas, float ne
     Vehicle* v = &h->veh;
     if (!meas)
         return fail(E, DOGM_ERR_INVALID_ARGUMENT, "measurement_grid is null");
-    int st = copy_measurement(E, v, meas, is_device);
+    int st = vehicle_set_measurement(E, v, meas, is_device);
     if (st)
         return st;
-    update_pose(E, v, new_x, new_y, new_yaw, true);
+    vehicle_update_pose(E, v, new_x, new_y, new_yaw, true);
     v->dt = dt;
     return run_one(E, v, ST_ALL, "updateGrid");
 }
@@ -342,7 +276,7 @@ int dogm_update_measurement_grid(dogm_handle* h, const dogm_measurement_cell* me
     NEED_H(h);
     if (!meas)
         return fail(&h->eng, DOGM_ERR_INVALID_ARGUMENT, "measurement_grid is null");
-    int st = copy_measurement(&h->eng, &h->veh, meas, is_device);
+    int st = vehicle_set_measurement(&h->eng, &h->veh, meas, is_device);
     if (st)
         return st;
     if (h->veh.init_pending)
@@ -353,7 +287,7 @@ int dogm_update_measurement_grid(dogm_handle* h, const dogm_measurement_cell* me
 int dogm_update_pose(dogm_handle* h, float new_x, float new_y, float new_yaw)
 {
     NEED_H(h);
-    update_pose(&h->eng, &h->veh, new_x, new_y, new_yaw, false);
+    vehicle_update_pose(&h->eng, &h->veh, new_x, new_y, new_yaw, false);
     return finish(&h->eng, "updatePose");
 }
 
@@ -624,7 +558,7 @@ int dogm_batch_add_vehicle(dogm_batch* b, const dogm_params* params, uint64_t se
     std::unique_ptr<Vehicle> v(new (std::nothrow) Vehicle());
     if (!v)
         return DOGM_ERR_OUT_OF_MEMORY;
-    int st = vehicle_alloc(&b->eng, v.get(), params, seed, instance);
+    int st = vehicle_alloc(v.get(), params, seed, instance, &b->eng.last_error);
     if (st != DOGM_OK)
     {
         vehicle_free(v.get());
@@ -694,31 +628,10 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
         for (int j = i + 1; j < count; ++j)
             if (list[i] == list[j])
                 return fail(E, DOGM_ERR_INVALID_ARGUMENT, "a vehicle appears twice in one step");
+    std::vector<FrameIn> in((size_t)count);
     for (int i = 0; i < count; ++i)
-    {
-        Vehicle* v = list[i];
-        const dogm_measurement_cell* m = meas ? meas[i] : nullptr;
-        if (m)
-        {
-            int st = copy_measurement(E, v, m, is_device);
-            if (st)
-                return st;
-        }
-        else if (!v->first_meas)
-        {
-            v->init_pending = true;
-            v->first_meas = true;
-        }
-        update_pose(E, v, x[i], y[i], yaw[i], true);
-        v->dt = dt[i];
-    }
-    int st = engine_run(E, list.data(), count, ST_ALL);
-    if (st != DOGM_OK)
-    {
-        (void)hipStreamSynchronize(E->stream);
-        return st;
-    }
-    return finish(E, "batchUpdate");
+        in[i] = FrameIn{meas ? meas[i] : nullptr, is_device, x[i], y[i], yaw[i], dt[i]};
+    return engine_frame(E, list.data(), in.data(), count);
 }
 
 int dogm_batch_get_grid_cells(dogm_batch* b, int vehicle_id, dogm_grid_cell* out)

@@ -462,7 +462,108 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     return DOGM_OK;
 }
 
+int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m)
+{
+    for (int i = 0; i < m; ++i)
+    {
+        Vehicle* v = veh[i];
+        if (in[i].meas)
+        {
+            int st = vehicle_set_measurement(E, v, in[i].meas, in[i].is_device);
+            if (st != DOGM_OK)
+            {
+                (void)hipStreamSynchronize(E->stream);
+                return st;
+            }
+        }
+        else if (!v->first_meas)
+        {
+            v->init_pending = true;
+            v->first_meas = true;
+        }
+        vehicle_update_pose(E, v, in[i].x, in[i].y, in[i].yaw, true);
+        v->dt = in[i].dt;
+    }
+    int st = engine_run(E, veh, m, ST_ALL);
+    const hipError_t e = hipStreamSynchronize(E->stream);
+    if (st != DOGM_OK)
+        return st;
+    if (e != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
+    engine_collect_timing(E);
+    return DOGM_OK;
+}
+
 // ---- vehicles ----------------------------------------------------------------------------------
+
+int vehicle_set_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device)
+{
+    const hipError_t e = hipMemcpyAsync(v->meas, m, sizeof(dogm_measurement_cell) * (size_t)v->C,
+                                        is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, E->stream);
+    if (e != hipSuccess)
+        return set_err(&E->last_error, DOGM_ERR_HIP, std::string("copying the measurement grid: ") + hipGetErrorString(e));
+    // dogm.cu:205-215: the first measurement initialises the particles.
+    if (!v->first_meas)
+    {
+        v->init_pending = true;
+        v->first_meas = true;
+    }
+    return DOGM_OK;
+}
+
+static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
+{
+    auto shift_ptr = [&](void** field) {
+        run_shift_field(E, v, *field, v->field_tmp, ix, iy);
+        void* old = *field;
+        *field = v->field_tmp;
+        v->field_tmp = static_cast<float*>(old);
+    };
+    shift_ptr(reinterpret_cast<void**>(&v->grid.start));
+    shift_ptr(reinterpret_cast<void**>(&v->grid.end));
+    for (int f = 0; f < GF_COUNT; ++f)
+        shift_ptr(reinterpret_cast<void**>(&v->grid.f[f]));
+}
+
+// dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
+// occupancy update (free mass, the only grid field read before being rewritten in a frame).
+void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
+{
+    if (!v->first_pose)
+    {
+        v->pos_x = new_x;
+        v->pos_y = new_y;
+        v->yaw = new_yaw;
+        v->first_pose = true;
+        return;
+    }
+    const float x_diff = new_x - v->pos_x;
+    const float y_diff = new_y - v->pos_y;
+    const float res = v->params.resolution;
+    if (std::fabs(x_diff) > res || std::fabs(y_diff) > res)
+    {
+        const int x_move = -static_cast<int>(x_diff / res);
+        const int y_move = -static_cast<int>(y_diff / res);
+        if (lazy)
+        {
+            v->pend_part_shift = true;
+            v->pend_px = (float)x_move;
+            v->pend_py = (float)y_move;
+            v->pend_grid_shift = true;
+            v->pend_gx = x_move;
+            v->pend_gy = y_move;
+        }
+        else
+        {
+            run_move_particles(E, v, (float)x_move, (float)y_move);
+            shift_grid_eager(E, v, x_move, y_move);
+        }
+        v->pos_x = new_x;
+        v->pos_y = new_y;
+        v->yaw = new_yaw;
+    }
+}
+
 
 int vehicle_validate(const dogm_params* p, std::string* why)
 {
@@ -493,12 +594,11 @@ size_t vehicle_footprint(const dogm_params* p)
                     (N + B) * 16);
 }
 
-int vehicle_alloc(Engine* E, Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance)
+int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance, std::string* why)
 {
-    std::string why;
-    int st = vehicle_validate(params, &why);
+    int st = vehicle_validate(params, why);
     if (st != DOGM_OK)
-        return set_err(&E->last_error, st, why);
+        return st;
     v->params = *params;
     v->gs = static_cast<int>(params->size / params->resolution);  // dogm.cu:34
     v->C = v->gs * v->gs;
@@ -510,46 +610,46 @@ int vehicle_alloc(Engine* E, Vehicle* v, const dogm_params* params, uint64_t see
     const int C = v->C, N = v->N, B = v->B;
     size_t* by = &v->bytes;
 
-    if ((st = alloc_particles(v->part[0], N, &why, by)) || (st = alloc_particles(v->part[1], N, &why, by)) ||
-        (st = alloc_particles(v->birth, B, &why, by)))
-        return set_err(&E->last_error, st, why);
-    if ((st = dalloc(&v->grid.start, C, "grid", &why, by)) || (st = dalloc(&v->grid.end, C, "grid", &why, by)))
-        return set_err(&E->last_error, st, why);
+    if ((st = alloc_particles(v->part[0], N, why, by)) || (st = alloc_particles(v->part[1], N, why, by)) ||
+        (st = alloc_particles(v->birth, B, why, by)))
+        return st;
+    if ((st = dalloc(&v->grid.start, C, "grid", why, by)) || (st = dalloc(&v->grid.end, C, "grid", why, by)))
+        return st;
     for (int f = 0; f < GF_COUNT; ++f)
-        if ((st = dalloc(&v->grid.f[f], C, "grid", &why, by)))
-            return set_err(&E->last_error, st, why);
-    if ((st = dalloc(&v->free_alt, C, "grid", &why, by)) || (st = dalloc(&v->field_tmp, C, "grid", &why, by)) ||
-        (st = dalloc(&v->meas, C, "measurement grid", &why, by)) ||
-        (st = dalloc(&v->weight_array, N, "weights", &why, by)) || (st = dalloc(&v->born, C, "born masses", &why, by)) ||
-        (st = dalloc(&v->m_raw, C, "cell masses", &why, by)) || (st = dalloc(&v->pend, C, "particle orders", &why, by)) ||
-        (st = dalloc(&v->ancestors, N, "ancestors", &why, by)) ||
-        (st = dalloc(&v->sort_keys[0], N, "sort", &why, by)) || (st = dalloc(&v->sort_keys[1], N, "sort", &why, by)) ||
-        (st = dalloc(&v->sort_vals[0], N, "sort", &why, by)) || (st = dalloc(&v->sort_vals[1], N, "sort", &why, by)) ||
-        (st = dalloc(&v->cdf, (size_t)N + B, "cdf", &why, by)))
-        return set_err(&E->last_error, st, why);
+        if ((st = dalloc(&v->grid.f[f], C, "grid", why, by)))
+            return st;
+    if ((st = dalloc(&v->free_alt, C, "grid", why, by)) || (st = dalloc(&v->field_tmp, C, "grid", why, by)) ||
+        (st = dalloc(&v->meas, C, "measurement grid", why, by)) ||
+        (st = dalloc(&v->weight_array, N, "weights", why, by)) || (st = dalloc(&v->born, C, "born masses", why, by)) ||
+        (st = dalloc(&v->m_raw, C, "cell masses", why, by)) || (st = dalloc(&v->pend, C, "particle orders", why, by)) ||
+        (st = dalloc(&v->ancestors, N, "ancestors", why, by)) ||
+        (st = dalloc(&v->sort_keys[0], N, "sort", why, by)) || (st = dalloc(&v->sort_keys[1], N, "sort", why, by)) ||
+        (st = dalloc(&v->sort_vals[0], N, "sort", why, by)) || (st = dalloc(&v->sort_vals[1], N, "sort", why, by)) ||
+        (st = dalloc(&v->cdf, (size_t)N + B, "cdf", why, by)))
+        return st;
 
     // initGridCellsKernel (init.cu:69-84): start = end = -1, masses 0, measurement (0, 0, 1, 1);
     // the remaining grid fields and buffers start at zero.
     if (hipMemset(v->grid.start, 0xff, sizeof(int) * C) != hipSuccess ||
         hipMemset(v->grid.end, 0xff, sizeof(int) * C) != hipSuccess)
-        return set_err(&E->last_error, DOGM_ERR_HIP, "clearing the grid");
+        return set_err(why, DOGM_ERR_HIP, "clearing the grid");
     for (int f = 0; f < GF_COUNT; ++f)
         if (hipMemset(v->grid.f[f], 0, sizeof(float) * C) != hipSuccess)
-            return set_err(&E->last_error, DOGM_ERR_HIP, "clearing the grid");
+            return set_err(why, DOGM_ERR_HIP, "clearing the grid");
     float* zbufs[] = {v->free_alt, v->weight_array, v->born, v->m_raw};
     const size_t zcnt[] = {(size_t)C, (size_t)N, (size_t)C, (size_t)C};
     for (int i = 0; i < 4; ++i)
         if (zcnt[i] && hipMemset(zbufs[i], 0, sizeof(float) * zcnt[i]) != hipSuccess)
-            return set_err(&E->last_error, DOGM_ERR_HIP, "clearing buffers");
+            return set_err(why, DOGM_ERR_HIP, "clearing buffers");
     if (hipMemset(v->ancestors, 0, sizeof(int) * N) != hipSuccess)
-        return set_err(&E->last_error, DOGM_ERR_HIP, "clearing buffers");
+        return set_err(why, DOGM_ERR_HIP, "clearing buffers");
     {
         std::vector<dogm_measurement_cell> m0((size_t)C, dogm_measurement_cell{0.0f, 0.0f, 1.0f, 1.0f});
         if (hipMemcpy(v->meas, m0.data(), sizeof(dogm_measurement_cell) * C, hipMemcpyHostToDevice) != hipSuccess)
-            return set_err(&E->last_error, DOGM_ERR_HIP, "initialising the measurement grid");
+            return set_err(why, DOGM_ERR_HIP, "initialising the measurement grid");
     }
     if (hipDeviceSynchronize() != hipSuccess)
-        return set_err(&E->last_error, DOGM_ERR_HIP, "hipDeviceSynchronize");
+        return set_err(why, DOGM_ERR_HIP, "hipDeviceSynchronize");
     return DOGM_OK;
 }
 

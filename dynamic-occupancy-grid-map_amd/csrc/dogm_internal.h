@@ -318,8 +318,24 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages);
 void engine_collect_timing(Engine* E);
 int vehicle_validate(const dogm_params* p, std::string* why);
 size_t vehicle_footprint(const dogm_params* p);  // device bytes vehicle_alloc takes
-int vehicle_alloc(Engine* E, Vehicle* v, const dogm_params* p, uint64_t seed, uint32_t instance);
+int vehicle_alloc(Vehicle* v, const dogm_params* p, uint64_t seed, uint32_t instance, std::string* why);
 void vehicle_free(Vehicle* v);
+// Queues the copy of a frame's measurement grid (dogm.cu:205-215); the first one arms the particle
+// initialisation of the next step.
+int vehicle_set_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device);
+// dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
+// occupancy update (free mass); otherwise at once.
+void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy);
+
+// One updateGrid frame for each of `m` distinct vehicles in one step.
+struct FrameIn
+{
+    const dogm_measurement_cell* meas;  // nullptr: the grid already sits in the vehicle's buffer
+    int is_device;
+    float x, y, yaw, dt;
+};
+// Measurement copies, lazy pose updates, the full stage sequence, and a wait for the stream.
+int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m);
 
 // ---- stage launchers (dogm_kernels.hip, dogm_sort.hip, dogm_resample.hip) ----
 // `pph` is the index (0 / 1) of the current particle buffer of the step's descriptors.

@@ -1,0 +1,482 @@
+// Serving: vehicles admitted against a device-memory budget keep their filter state resident;
+// frames from any of them are queued with their arrival times, and each step runs the oldest queued
+// frame of every vehicle that has one (up to max_batch) in one batched update (continuous
+// batching: frames that arrive while a step runs join the next one).  No counterpart in the
+// reference, which runs one dogm::DOGM per grid, one blocking updateGrid per frame
+// (dogm/src/dogm.cu:115-131).
+//
+// Two ways to drive it:
+//  - dogm_server_step(now): one step on the caller's thread, stamped on the caller's clock (start =
+//    now, done = now + the step's measured duration), for virtual-time simulation and tests;
+//  - dogm_server_start(): a worker thread that steps whenever frames are queued, stamped with
+//    dogm_server_clock().
+#include "dogm_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <set>
+#include <thread>
+
+using namespace dogm_hip;
+
+struct dogm_server
+{
+    Engine eng;
+    std::mutex eng_mu;  // serialises steps and read-backs on the engine's stream
+    std::mutex mu;      // guards everything below
+    std::condition_variable cv_work, cv_done;
+    std::map<int, std::unique_ptr<Vehicle>> veh;
+    std::map<int, int> inflight;  // queued + running frames per vehicle
+    uint64_t budget = 0, resident = 0;
+    int max_batch = 0;
+    int next_id = 1;
+    uint64_t next_ticket = 1;
+    struct Frame
+    {
+        uint64_t ticket;
+        int vid;
+        FrameIn in;
+        double t_arrival;
+    };
+    std::deque<Frame> queue;
+    std::vector<dogm_frame_result> done;
+    std::thread worker;
+    bool running = false, stopping = false;
+    std::string last_error;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    long long steps = 0, frames = 0;
+};
+
+namespace
+{
+
+int sfail(dogm_server* s, int code, const std::string& msg)
+{
+    std::lock_guard<std::mutex> g(s->mu);
+    s->last_error = msg;
+    return code;
+}
+
+double clock_now(const dogm_server* s)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+// Takes the next step's frames from the queue (caller holds s->mu).
+std::vector<dogm_server::Frame> take_frames(dogm_server* s)
+{
+    std::vector<dogm_server::Frame> out;
+    std::set<int> seen;
+    for (auto it = s->queue.begin(); it != s->queue.end();)
+    {
+        if (s->max_batch > 0 && (int)out.size() >= s->max_batch)
+            break;
+        if (seen.insert(it->vid).second)
+        {
+            out.push_back(*it);
+            it = s->queue.erase(it);
+        }
+        else
+        {
+            ++it;
+        }
+    }
+    return out;
+}
+
+// Runs one step over `fr`; wall duration in *secs.
+int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* secs)
+{
+    std::vector<Vehicle*> list(fr.size());
+    std::vector<FrameIn> in(fr.size());
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (size_t i = 0; i < fr.size(); ++i)
+        {
+            list[i] = s->veh.at(fr[i].vid).get();
+            in[i] = fr[i].in;
+        }
+    }
+    std::lock_guard<std::mutex> ge(s->eng_mu);
+    const auto a = std::chrono::steady_clock::now();
+    const int st = engine_frame(&s->eng, list.data(), in.data(), (int)fr.size());
+    *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    if (st != DOGM_OK)
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->last_error = s->eng.last_error;
+    }
+    return st;
+}
+
+void record(dogm_server* s, const std::vector<dogm_server::Frame>& fr, int st, double t_start, double t_done)
+{
+    std::lock_guard<std::mutex> g(s->mu);
+    for (const auto& f : fr)
+    {
+        dogm_frame_result r{};
+        r.ticket = f.ticket;
+        r.vehicle_id = f.vid;
+        r.status = st;
+        r.batch_size = (int)fr.size();
+        r.t_arrival = f.t_arrival;
+        r.t_start = t_start;
+        r.t_done = t_done;
+        s->done.push_back(r);
+        if (--s->inflight[f.vid] == 0)
+            s->inflight.erase(f.vid);
+    }
+    s->steps += 1;
+    s->frames += (long long)fr.size();
+    s->cv_done.notify_all();
+}
+
+void worker_loop(dogm_server* s)
+{
+    for (;;)
+    {
+        std::vector<dogm_server::Frame> fr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stopping || !s->queue.empty(); });
+            if (s->queue.empty())
+                return;  // stopping and drained
+            fr = take_frames(s);
+        }
+        const double t_start = clock_now(s);
+        double secs = 0.0;
+        const int st = run_step(s, fr, &secs);
+        record(s, fr, st, t_start, clock_now(s));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dogm_server_create(const dogm_server_config* cfg, dogm_server** out)
+{
+    if (!cfg || !out)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (cfg->max_batch < 0)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    dogm_server* s = new (std::nothrow) dogm_server();
+    if (!s)
+        return DOGM_ERR_OUT_OF_MEMORY;
+    std::string why;
+    const int st = engine_init(&s->eng, &why);
+    if (st != DOGM_OK)
+    {
+        engine_release(&s->eng);
+        delete s;
+        return st;
+    }
+    s->budget = cfg->hbm_budget_bytes;
+    s->max_batch = cfg->max_batch;
+    *out = s;
+    return DOGM_OK;
+}
+
+void dogm_server_destroy(dogm_server* s)
+{
+    if (!s)
+        return;
+    (void)dogm_server_stop(s);
+    {
+        std::lock_guard<std::mutex> ge(s->eng_mu);
+        for (auto& kv : s->veh)
+            vehicle_free(kv.second.get());
+        s->veh.clear();
+        engine_release(&s->eng);
+    }
+    delete s;
+}
+
+const char* dogm_server_last_error(const dogm_server* s)
+{
+    return s ? s->last_error.c_str() : "null server";
+}
+
+double dogm_server_clock(const dogm_server* s)
+{
+    return s ? clock_now(s) : 0.0;
+}
+
+int dogm_server_admit(dogm_server* s, const dogm_params* params, uint64_t seed, uint32_t instance, int* vehicle_id)
+{
+    if (!s || !vehicle_id)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    *vehicle_id = 0;
+    std::string why;
+    int st = vehicle_validate(params, &why);
+    if (st != DOGM_OK)
+        return sfail(s, st, why);
+    const uint64_t fp = (uint64_t)vehicle_footprint(params);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->resident + fp > s->budget)
+        {
+            s->last_error = "admission would exceed the device-memory budget (" + std::to_string(s->resident) +
+                            " + " + std::to_string(fp) + " > " + std::to_string(s->budget) + " bytes)";
+            return DOGM_ERR_OUT_OF_MEMORY;
+        }
+        s->resident += fp;  // reserved while allocating
+    }
+    std::unique_ptr<Vehicle> v(new (std::nothrow) Vehicle());
+    if (!v)
+        st = DOGM_ERR_OUT_OF_MEMORY;
+    else
+        st = vehicle_alloc(v.get(), params, seed, instance, &why);
+    std::lock_guard<std::mutex> g(s->mu);
+    if (st != DOGM_OK)
+    {
+        if (v)
+            vehicle_free(v.get());
+        s->resident -= fp;
+        s->last_error = why;
+        return st;
+    }
+    const int id = s->next_id++;
+    s->veh[id] = std::move(v);
+    *vehicle_id = id;
+    return DOGM_OK;
+}
+
+int dogm_server_release(dogm_server* s, int vehicle_id)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::unique_ptr<Vehicle> v;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->veh.find(vehicle_id);
+        if (it == s->veh.end())
+        {
+            s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        if (s->inflight.count(vehicle_id))
+        {
+            if (!s->running)
+            {
+                s->last_error = "vehicle " + std::to_string(vehicle_id) + " has queued frames";
+                return DOGM_ERR_INVALID_ARGUMENT;
+            }
+            s->cv_done.wait(lk, [&] { return !s->inflight.count(vehicle_id); });
+        }
+        v = std::move(it->second);
+        s->veh.erase(it);
+        s->resident -= (uint64_t)vehicle_footprint(&v->params);
+    }
+    std::lock_guard<std::mutex> ge(s->eng_mu);
+    vehicle_free(v.get());
+    return DOGM_OK;
+}
+
+uint64_t dogm_server_resident_bytes(const dogm_server* s)
+{
+    if (!s)
+        return 0;
+    std::lock_guard<std::mutex> g(const_cast<dogm_server*>(s)->mu);
+    return s->resident;
+}
+
+int dogm_server_submit(dogm_server* s, int vehicle_id, const dogm_measurement_cell* meas, int is_device, float x,
+                       float y, float yaw, float dt, double t_arrival, uint64_t* ticket)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->veh.count(vehicle_id))
+    {
+        s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t t = s->next_ticket++;
+    s->queue.push_back(dogm_server::Frame{t, vehicle_id, FrameIn{meas, is_device, x, y, yaw, dt}, t_arrival});
+    s->inflight[vehicle_id] += 1;
+    if (ticket)
+        *ticket = t;
+    s->cv_work.notify_one();
+    return DOGM_OK;
+}
+
+int dogm_server_queued(const dogm_server* s)
+{
+    if (!s)
+        return 0;
+    std::lock_guard<std::mutex> g(const_cast<dogm_server*>(s)->mu);
+    return (int)s->queue.size();
+}
+
+int dogm_server_step(dogm_server* s, double now, int* frames_run, double* step_seconds)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::vector<dogm_server::Frame> fr;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->running)
+        {
+            s->last_error = "the worker thread is running";
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        fr = take_frames(s);
+    }
+    if (frames_run)
+        *frames_run = (int)fr.size();
+    if (step_seconds)
+        *step_seconds = 0.0;
+    if (fr.empty())
+        return DOGM_OK;
+    double secs = 0.0;
+    const int st = run_step(s, fr, &secs);
+    record(s, fr, st, now, now + secs);
+    if (step_seconds)
+        *step_seconds = secs;
+    return st;
+}
+
+int dogm_server_start(dogm_server* s)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->running)
+        return DOGM_OK;
+    s->stopping = false;
+    s->running = true;
+    s->worker = std::thread(worker_loop, s);
+    return DOGM_OK;
+}
+
+int dogm_server_stop(dogm_server* s)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->running)
+            return DOGM_OK;
+        s->stopping = true;
+        s->cv_work.notify_all();
+    }
+    s->worker.join();
+    std::lock_guard<std::mutex> g(s->mu);
+    s->running = false;
+    s->stopping = false;
+    return DOGM_OK;
+}
+
+int dogm_server_poll(dogm_server* s, dogm_frame_result* out, int max_results, int* count)
+{
+    if (!s || !count || (max_results > 0 && !out))
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(s->mu);
+    const int n = std::min<int>(max_results, (int)s->done.size());
+    std::copy(s->done.begin(), s->done.begin() + n, out);
+    s->done.erase(s->done.begin(), s->done.begin() + n);
+    *count = n;
+    return DOGM_OK;
+}
+
+int dogm_server_wait_idle(dogm_server* s)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->running && !s->queue.empty())
+    {
+        s->last_error = "frames are queued and no worker is running";
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    s->cv_done.wait(lk, [&] { return s->inflight.empty(); });
+    return DOGM_OK;
+}
+
+int dogm_server_get_grid_cells(dogm_server* s, int vehicle_id, dogm_grid_cell* out)
+{
+    if (!s || !out)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = nullptr;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->veh.find(vehicle_id);
+        if (it == s->veh.end())
+        {
+            s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        v = it->second.get();
+    }
+    std::lock_guard<std::mutex> ge(s->eng_mu);
+    Engine* E = &s->eng;
+    if (!v->aos_tmp && hipMalloc(reinterpret_cast<void**>(&v->aos_tmp), sizeof(dogm_grid_cell) * (size_t)v->C) !=
+                           hipSuccess)
+    {
+        v->aos_tmp = nullptr;
+        return sfail(s, DOGM_ERR_OUT_OF_MEMORY, "allocating the grid packing buffer");
+    }
+    run_pack_grid(E, v);
+    if (hipMemcpyAsync(out, v->aos_tmp, sizeof(dogm_grid_cell) * (size_t)v->C, hipMemcpyDeviceToHost, E->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(E->stream) != hipSuccess)
+        return sfail(s, DOGM_ERR_HIP, "reading back the grid");
+    return DOGM_OK;
+}
+
+int dogm_server_get_particles(dogm_server* s, int vehicle_id, dogm_vec4* state, int* grid_cell_idx, float* weight,
+                              uint8_t* associated)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = nullptr;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->veh.find(vehicle_id);
+        if (it == s->veh.end())
+        {
+            s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        v = it->second.get();
+    }
+    std::lock_guard<std::mutex> ge(s->eng_mu);
+    Engine* E = &s->eng;
+    const DevParticles& p = v->part[0];
+    const size_t n = (size_t)v->N;
+    struct Part
+    {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    } parts[] = {{state, p.state, 16 * n}, {grid_cell_idx, p.idx, 4 * n}, {weight, p.weight, 4 * n},
+                 {associated, p.assoc, n}};
+    for (const Part& q : parts)
+        if (q.dst && hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost, E->stream) != hipSuccess)
+            return sfail(s, DOGM_ERR_HIP, "reading back particles");
+    if (hipStreamSynchronize(E->stream) != hipSuccess)
+        return sfail(s, DOGM_ERR_HIP, "reading back particles");
+    return DOGM_OK;
+}
+
+int dogm_server_stats(const dogm_server* s, long long* steps, long long* frames)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(const_cast<dogm_server*>(s)->mu);
+    if (steps)
+        *steps = s->steps;
+    if (frames)
+        *frames = s->frames;
+    return DOGM_OK;
+}
+
+}  // extern "C"

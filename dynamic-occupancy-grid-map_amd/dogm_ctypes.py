@@ -43,6 +43,15 @@ class dogm_params(C.Structure):
     ]
 
 
+class dogm_server_config(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_int)]
+
+
+class dogm_frame_result(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("vehicle_id", C.c_int), ("status", C.c_int), ("batch_size", C.c_int),
+                ("pad", C.c_int), ("t_arrival", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
 GRID_CELL_DTYPE = np.dtype([
     ("start_idx", "<i4"), ("end_idx", "<i4"), ("new_born_occ_mass", "<f4"), ("pers_occ_mass", "<f4"),
     ("free_mass", "<f4"), ("occ_mass", "<f4"), ("pred_occ_mass", "<f4"), ("mu_A", "<f4"), ("mu_UA", "<f4"),
@@ -118,6 +127,23 @@ _SIGS = {
     "dogm_batch_set_kernel_timing": (_i, [_h, _i, _i]),
     "dogm_batch_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "dogm_batch_reset_kernel_timing": (_i, [_h]),
+    "dogm_server_create": (_i, [C.POINTER(dogm_server_config), C.POINTER(_h)]),
+    "dogm_server_destroy": (None, [_h]),
+    "dogm_server_last_error": (C.c_char_p, [_h]),
+    "dogm_server_clock": (C.c_double, [_h]),
+    "dogm_server_admit": (_i, [_h, C.POINTER(dogm_params), C.c_uint64, C.c_uint32, C.POINTER(_i)]),
+    "dogm_server_release": (_i, [_h, _i]),
+    "dogm_server_resident_bytes": (C.c_uint64, [_h]),
+    "dogm_server_submit": (_i, [_h, _i, _vp, _i, _f, _f, _f, _f, C.c_double, C.POINTER(C.c_uint64)]),
+    "dogm_server_queued": (_i, [_h]),
+    "dogm_server_step": (_i, [_h, C.c_double, C.POINTER(_i), C.POINTER(C.c_double)]),
+    "dogm_server_start": (_i, [_h]),
+    "dogm_server_stop": (_i, [_h]),
+    "dogm_server_wait_idle": (_i, [_h]),
+    "dogm_server_poll": (_i, [_h, C.POINTER(dogm_frame_result), _i, C.POINTER(_i)]),
+    "dogm_server_get_grid_cells": (_i, [_h, _i, _vp]),
+    "dogm_server_get_particles": (_i, [_h, _i, _vp, _vp, _vp, _vp]),
+    "dogm_server_stats": (_i, [_h, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -471,6 +497,127 @@ class Batch:
 
     def reset_kernel_timing(self) -> None:
         self._check(lib.dogm_batch_reset_kernel_timing(self.h), "timing")
+
+
+class Server:
+    """Continuous-batching server over resident vehicles (dogm_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_batch: int = 0):
+        cfg = dogm_server_config(hbm_budget_bytes, max_batch)
+        h = C.c_void_p()
+        st = lib.dogm_server_create(C.byref(cfg), C.byref(h))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_server_create")
+        self.h = h
+        self.vehicles: dict[int, Params] = {}
+        self._keep: dict[int, object] = {}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.dogm_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != DOGM_OK:
+            raise DogmError(st, f"{what}: {lib.dogm_server_last_error(self.h).decode()}")
+
+    def clock(self) -> float:
+        return lib.dogm_server_clock(self.h)
+
+    def admit(self, params: Params, seed: int = DOGM_DEFAULT_SEED, instance: int = 0) -> int | None:
+        """Vehicle id, or None when the footprint does not fit the remaining budget."""
+        cp = params.to_c()
+        vid = C.c_int()
+        st = lib.dogm_server_admit(self.h, C.byref(cp), seed, instance, C.byref(vid))
+        if st == 2:  # DOGM_ERR_OUT_OF_MEMORY: over budget
+            return None
+        self._check(st, "admit")
+        self.vehicles[vid.value] = params
+        return vid.value
+
+    def release(self, vid: int) -> None:
+        self._check(lib.dogm_server_release(self.h, vid), "release")
+        self.vehicles.pop(vid, None)
+
+    def resident_bytes(self) -> int:
+        return int(lib.dogm_server_resident_bytes(self.h))
+
+    def submit(self, vid: int, meas, pose, dt: float, t_arrival: float) -> int:
+        """meas: DeviceBuffer or host structured array (kept alive until the frame completes)."""
+        x, y, yaw = pose
+        t = C.c_uint64()
+        if isinstance(meas, DeviceBuffer):
+            st = lib.dogm_server_submit(self.h, vid, C.c_void_p(meas.ptr), 1, x, y, yaw, dt, t_arrival, C.byref(t))
+            keep = meas
+        else:
+            keep = np.ascontiguousarray(meas, dtype=MEAS_CELL_DTYPE)
+            st = lib.dogm_server_submit(self.h, vid, _ptr(keep), 0, x, y, yaw, dt, t_arrival, C.byref(t))
+        self._check(st, "submit")
+        self._keep[t.value] = keep
+        return t.value
+
+    def queued(self) -> int:
+        return lib.dogm_server_queued(self.h)
+
+    def step(self, now: float) -> tuple[int, float]:
+        n = C.c_int()
+        secs = C.c_double()
+        self._check(lib.dogm_server_step(self.h, now, C.byref(n), C.byref(secs)), "step")
+        return n.value, secs.value
+
+    def start(self) -> None:
+        self._check(lib.dogm_server_start(self.h), "start")
+
+    def stop(self) -> None:
+        self._check(lib.dogm_server_stop(self.h), "stop")
+
+    def wait_idle(self) -> None:
+        self._check(lib.dogm_server_wait_idle(self.h), "wait_idle")
+
+    def poll(self, max_results: int = 4096) -> list[dict]:
+        buf = (dogm_frame_result * max_results)()
+        n = C.c_int()
+        self._check(lib.dogm_server_poll(self.h, buf, max_results, C.byref(n)), "poll")
+        out = []
+        for i in range(n.value):
+            r = buf[i]
+            self._keep.pop(r.ticket, None)
+            out.append({"ticket": r.ticket, "vehicle_id": r.vehicle_id, "status": r.status,
+                        "batch_size": r.batch_size, "t_arrival": r.t_arrival, "t_start": r.t_start,
+                        "t_done": r.t_done})
+        return out
+
+    def get_grid_cells(self, vid: int) -> np.ndarray:
+        p = self.vehicles[vid]
+        gs = int(np.float32(p.size) / np.float32(p.resolution))
+        out = np.empty(gs * gs, dtype=GRID_CELL_DTYPE)
+        self._check(lib.dogm_server_get_grid_cells(self.h, vid, _ptr(out)), "getGridCells")
+        return out
+
+    def get_particles(self, vid: int) -> dict:
+        n = self.vehicles[vid].particle_count
+        state = np.empty((n, 4), np.float32)
+        idx = np.empty(n, np.int32)
+        w = np.empty(n, np.float32)
+        a = np.empty(n, np.uint8)
+        self._check(lib.dogm_server_get_particles(self.h, vid, _ptr(state), _ptr(idx), _ptr(w), _ptr(a)),
+                    "getParticles")
+        return {"state": state, "grid_cell_idx": idx, "weight": w, "associated": a}
+
+    def stats(self) -> tuple[int, int]:
+        a, b = C.c_longlong(), C.c_longlong()
+        self._check(lib.dogm_server_stats(self.h, C.byref(a), C.byref(b)), "stats")
+        return a.value, b.value
 
 
 def _as_particle_arrays(n, state, idx, weight, assoc):

@@ -221,6 +221,62 @@ int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable);
 int dogm_batch_get_kernel_timing(dogm_batch* b, int kernel_id, double* total_ms, long long* launches);
 int dogm_batch_reset_kernel_timing(dogm_batch* b);
 
+/* ----------------------------------------------------------------------------------------------
+ * Serving: continuous batching of frames from many vehicles.  No counterpart in the reference.
+ *
+ * Vehicles are admitted against a device-memory budget (their footprint, dogm_vehicle_footprint,
+ * is reserved until release) and keep all filter state resident.  Frames are queued with their
+ * arrival time; a step runs the oldest queued frame of every vehicle that has one (at most
+ * max_batch frames, 0 = no limit) in one batched update, so frames that arrive while a step is on
+ * the GPU join the next step.  A vehicle's frames run in submission order, and every result is
+ * bit-identical to the vehicle's frames run alone through dogm_update_grid.
+ * The measurement grid passed to dogm_server_submit must stay valid until the frame's result is
+ * reported.  Drive the queue either with dogm_server_step (caller's thread and clock) or with the
+ * worker thread (dogm_server_start / dogm_server_stop, stamped with dogm_server_clock).
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct dogm_server dogm_server;
+
+typedef struct dogm_server_config
+{
+    uint64_t hbm_budget_bytes; /* admission budget for resident vehicle state */
+    int max_batch;             /* frames per step, 0 = every vehicle with a queued frame */
+} dogm_server_config;
+
+typedef struct dogm_frame_result
+{
+    uint64_t ticket;
+    int vehicle_id;
+    int status;     /* DOGM_OK or the step's error */
+    int batch_size; /* frames in the step that ran this one */
+    int pad;
+    double t_arrival; /* as submitted */
+    double t_start;   /* step start */
+    double t_done;    /* step end (result ready) */
+} dogm_frame_result;
+
+int dogm_server_create(const dogm_server_config* cfg, dogm_server** out);
+void dogm_server_destroy(dogm_server* s);
+const char* dogm_server_last_error(const dogm_server* s);
+double dogm_server_clock(const dogm_server* s); /* seconds since creation (steady clock) */
+/* DOGM_ERR_OUT_OF_MEMORY when the vehicle's footprint does not fit the remaining budget. */
+int dogm_server_admit(dogm_server* s, const dogm_params* params, uint64_t seed, uint32_t instance, int* vehicle_id);
+/* Waits for the vehicle's in-flight frames when the worker runs; fails if frames are queued otherwise. */
+int dogm_server_release(dogm_server* s, int vehicle_id);
+uint64_t dogm_server_resident_bytes(const dogm_server* s);
+int dogm_server_submit(dogm_server* s, int vehicle_id, const dogm_measurement_cell* meas, int is_device, float x,
+                       float y, float yaw, float dt, double t_arrival, uint64_t* ticket);
+int dogm_server_queued(const dogm_server* s);
+/* One step on the caller's thread: start = now, done = now + the step's measured duration. */
+int dogm_server_step(dogm_server* s, double now, int* frames_run, double* step_seconds);
+int dogm_server_start(dogm_server* s);
+int dogm_server_stop(dogm_server* s); /* runs the queue empty, then joins the worker */
+int dogm_server_wait_idle(dogm_server* s);
+int dogm_server_poll(dogm_server* s, dogm_frame_result* out, int max_results, int* count);
+int dogm_server_get_grid_cells(dogm_server* s, int vehicle_id, dogm_grid_cell* out);
+int dogm_server_get_particles(dogm_server* s, int vehicle_id, dogm_vec4* state, int* grid_cell_idx, float* weight,
+                              uint8_t* associated);
+int dogm_server_stats(const dogm_server* s, long long* steps, long long* frames);
+
 #ifdef __cplusplus
 }
 #endif
