@@ -272,6 +272,140 @@ def run_serving(args) -> int:
     return 0
 
 
+PRECISION_PARAMS = dict(size=204.8, resolution=0.1, particle_count=8_000_000, new_born_particle_count=800_000,
+                        persistence_prob=0.99, stddev_process_noise_position=0.1,
+                        stddev_process_noise_velocity=2.0, birth_prob=0.02, stddev_velocity=30.0,
+                        init_max_velocity=15.0, freespace_discount=0.01)
+
+
+def run_precision(args) -> int:
+    """Precision form: 2048 x 2048 cells at 0.1 m, 8e6 + 8e5 particles, f32 state.  Every frame runs
+    the stages up to the moments, then the joint weights are replaced by 10^(-9 U) (U from a Philox
+    stream, so the CDF spans ~9 decades) and resampled with the exact, FP64 and FP32 CDF scans.
+    Reported per mode: ancestor agreement with a sequential FP64 CDF of the same weights and draws,
+    the relative error of the CDF total, and the resampling stage time.  On two frames the per-cell
+    masses of the f32 path are compared with the f64 checker run from the same state."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle_ctypes  # noqa: E402  (checker: FP64 reference only)
+    import scene
+    from dogm_ctypes import DOGM, Params, lib
+
+    say = lambda m: print(f"[precision] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    p = Params(**PRECISION_PARAMS)
+    N, B = p.particle_count, p.new_born_particle_count
+    cfg = scene.SceneConfig(grid_size=2048, resolution=0.1, n_boxes=60, n_beams=1440, max_range=100.0,
+                            ego_speed=10.0, ego_yaw_rate=0.1, seed=args.seed)
+    sc = scene.Scene(cfg)
+    frames = args.precision_frames
+    modes = ["exact", "fp64", "fp32"]
+    handles = {}
+    for mo in modes:
+        g = DOGM(p, seed=123456, instance=0)
+        g.set_cdf_mode(mo)
+        g.set_kernel_timing("rs_merge", True)
+        g.set_kernel_timing("cdf_write", True)
+        handles[mo] = g
+    rng = np.random.Generator(np.random.Philox(args.seed))
+    agree = {mo: [] for mo in modes}
+    tot_err = {mo: [] for mo in modes}
+    rs_ms = {mo: [] for mo in modes}
+    mass = []
+    for f in range(frames):
+        t0 = time.perf_counter()
+        m = sc.measurement(f)
+        x, y, yaw = sc.ego_pose(f)
+        check_mass = f in (2, frames - 1)
+        for mo, g in handles.items():
+            g.update_measurement_grid(m)
+            g.update_pose(x, y, yaw)
+            g.particle_prediction(0.1)
+            g.particle_assignment()
+            twin = _fp64_twin(g, m, oracle_ctypes, p) if (check_mass and mo == "exact") else None
+            g.grid_cell_occupancy_update(0.1)
+            g.update_persistent_particles()
+            if twin is not None:
+                twin.grid_cell_occupancy_update(0.1)
+                twin.update_persistent_particles()
+                mass.append(_compare_masses(f, g.get_grid_cells(), twin.get_grid_cells()))
+                say(f"frame {f} masses vs f64: " + ", ".join(f"{k} {mass[-1][k]['max_rel']:.2e}" for k in MASS_FIELDS))
+                twin.close()
+            g.initialize_new_particles()
+            g.statistical_moments()
+        w = (10.0 ** (-9.0 * rng.random(N))).astype(np.float32)
+        bw = (10.0 ** (-9.0 * rng.random(B))).astype(np.float32)
+        cdf = np.cumsum(np.concatenate([w, bw]).astype(np.float64))
+        total = cdf[-1]
+        for mo, g in handles.items():
+            g.set_weight_array(w)
+            g.set_birth_particles(weight=bw)
+            g.reset_kernel_timing()
+            epoch = _handle_epoch(g)
+            g.resampling()
+            ms = sum(g.kernel_timing(k)[0] for k in ("rs_merge", "cdf_write"))
+            rs_ms[mo].append(ms)
+            u = oracle_ctypes.spacing_fractions(123456, 0, epoch, N)
+            ref = np.minimum(np.searchsorted(cdf, total * u, side="left"), N + B - 1)
+            anc = g.get_resampled_indices()
+            agree[mo].append(float(np.mean(anc == ref)))
+            nw = g.get_particles()["weight"][0]
+            tot_err[mo].append(abs(float(nw) * N - total) / total)
+        say(f"frame {f}: agreement " + ", ".join(f"{mo} {agree[mo][-1]:.6f}" for mo in modes) +
+            f" ({time.perf_counter() - t0:.1f} s)")
+    out = {
+        "metric": "precision: resampling ancestor agreement with a sequential FP64 CDF, 9-decade weights",
+        "value": round(min(agree["exact"]), 6),
+        "unit": "fraction of ancestors equal (exact mode, worst frame)",
+        "n_gpus": 1,
+        "dtype": "f32 state and weights; CDF exact / f64 / f32",
+        "data": "synthetic",
+        "config": {"workload": "2048x2048 cells at 0.1 m, 8e6 particles + 8e5 birth, 60 boxes, 1440-beam 100 m lidar; "
+                               "joint weights replaced by 10^(-9 U) before every resampling",
+                   "frames": frames},
+        "modes": {mo: {"agreement_mean": round(float(np.mean(agree[mo])), 6),
+                       "agreement_min": round(float(np.min(agree[mo])), 6),
+                       "new_weight_total_rel_error_max": float(np.max(tot_err[mo])),
+                       "cdf_and_merge_ms_mean": round(float(np.mean(rs_ms[mo])), 4)} for mo in modes},
+        "masses_vs_fp64_checker": mass,
+    }
+    print(json.dumps(out))
+    for g in handles.values():
+        g.close()
+    return 0
+
+
+def _handle_epoch(g) -> int:
+    """Philox epoch of the handle's next random stage: the resampling draws use it."""
+    from dogm_ctypes import lib
+    return int(lib.dogm_get_epoch(g.h))
+
+
+def _fp64_twin(g, meas, oracle_ctypes, p):
+    """The f64 checker loaded with the GPU handle's current state (after the assignment stage)."""
+    o = oracle_ctypes.OracleDOGM(p, seed=123456, fp64=True)
+    o.set_flags(True, True)
+    o.update_measurement_grid(meas)
+    parts = g.get_particles()
+    o.set_particles(state=parts["state"], grid_cell_idx=parts["grid_cell_idx"], weight=parts["weight"],
+                    associated=parts["associated"])
+    o.set_grid_cells(g.get_grid_cells())
+    o.set_weight_array(g.get_weight_array())
+    return o
+
+
+MASS_FIELDS = ("occ_mass", "free_mass", "pers_occ_mass", "new_born_occ_mass", "pred_occ_mass", "mu_A", "mu_UA")
+
+
+def _compare_masses(frame, gcells, ocells):
+    """Relative error of the f32 path against the f64 checker per mass field: |g - o| / max(|o|, 1e-6)."""
+    out = {"frame": frame}
+    for k in MASS_FIELDS:
+        gv = gcells[k].astype(np.float64)
+        ov = np.asarray(ocells[k], np.float64)
+        rel = np.abs(gv - ov) / np.maximum(np.abs(ov), 1e-6)
+        out[k] = {"max_rel": float(rel.max()), "within_1e-5": float(np.mean(rel <= 1e-5))}
+    return out
+
+
 def _summary(r):
     fps = r["frames"] / r["elapsed"]
     return {"frames_per_s": round(fps, 3), "ms_per_step": round(1e3 * r["elapsed"] * len(r["lat"]) ** -1, 4),
@@ -289,7 +423,8 @@ def main() -> int:
                     help="vehicles per batched step (0: plain form only); the batched form is the headline")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--form", default="both", choices=["both", "plain", "serving"],
+    ap.add_argument("--precision-frames", type=int, default=30, help="frames of the precision form")
+    ap.add_argument("--form", default="both", choices=["both", "plain", "serving", "precision"],
                     help="both: plain + batched (default); serving: continuous batching of a vehicle fleet")
     ap.add_argument("--serve-vehicle", default="fleet", choices=["fleet", "full"],
                     help="fleet: mixed grid sides; full: every vehicle at 1200x1200 / 2e6 particles")
@@ -310,6 +445,8 @@ def main() -> int:
     assert args.warmup >= 2, "--warmup must be at least 2"
     if args.form == "serving":
         return run_serving(args)
+    if args.form == "precision":
+        return run_precision(args)
     if args.form == "plain":
         args.batch = 0
 

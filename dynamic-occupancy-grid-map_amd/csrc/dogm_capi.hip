@@ -426,6 +426,18 @@ int dogm_get_resampled_indices(dogm_handle* h, int* out)
     return copy_out(&h->eng, out, h->veh.ancestors, sizeof(int) * (size_t)h->veh.N);
 }
 
+int dogm_set_cdf_mode(dogm_handle* h, int mode)
+{
+    NEED_H(h);
+    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32)
+        return fail(&h->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
+    h->veh.cdf_mode = mode;
+    return DOGM_OK;
+}
+
+int dogm_get_cdf_mode(const dogm_handle* h) { return h ? h->veh.cdf_mode : -1; }
+uint32_t dogm_get_epoch(const dogm_handle* h) { return h ? h->veh.epoch : 0u; }
+
 int dogm_get_grid_size(const dogm_handle* h) { return h ? h->veh.gs : 0; }
 int dogm_get_particle_count(const dogm_handle* h) { return h ? h->veh.N : 0; }
 int dogm_get_new_born_particle_count(const dogm_handle* h) { return h ? h->veh.B : 0; }
@@ -663,6 +675,19 @@ int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out)
     if (!v)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
     return copy_out(&b->eng, out, v->ancestors, sizeof(int) * (size_t)v->N);
+}
+
+int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = find_vehicle(b, vehicle_id);
+    if (!v)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
+    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
+    v->cdf_mode = mode;
+    return DOGM_OK;
 }
 
 int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable)

@@ -47,6 +47,48 @@ __device__ __forceinline__ fx_t wave_inclusive_scan_fx(fx_t x)
     return x;
 }
 
+// DPP moves and the same wave scan for f32 / f64 values (float CDF modes).  The association is
+// fixed by the six DPP steps: value + shifted value, rows of 16 first, then the row totals.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_val(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false));
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_val(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan_val(T x)
+{
+    x = x + dpp_val<0x111, 0xf>(x);
+    x = x + dpp_val<0x112, 0xf>(x);
+    x = x + dpp_val<0x114, 0xf>(x);
+    x = x + dpp_val<0x118, 0xf>(x);
+    x = x + dpp_val<0x142, 0xa>(x);
+    x = x + dpp_val<0x143, 0xc>(x);
+    return x;
+}
+
+// Running maximum over the lanes of a wave for non-negative values (max is exact in any order).
+template <class T>
+__device__ __forceinline__ T wave_inclusive_max_val(T x)
+{
+    { const T y = dpp_val<0x111, 0xf>(x); x = y > x ? y : x; }
+    { const T y = dpp_val<0x112, 0xf>(x); x = y > x ? y : x; }
+    { const T y = dpp_val<0x114, 0xf>(x); x = y > x ? y : x; }
+    { const T y = dpp_val<0x118, 0xf>(x); x = y > x ? y : x; }
+    { const T y = dpp_val<0x142, 0xa>(x); x = y > x ? y : x; }
+    { const T y = dpp_val<0x143, 0xc>(x); x = y > x ? y : x; }
+    return x;
+}
+
 // Exclusive block scan of one fx value per thread (BLOCK threads); returns the exclusive prefix
 // and writes the block total.
 __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
@@ -558,17 +600,30 @@ enum ScanMode
     SCAN_EXP = 3,       // exponential spacings: total into exp_total
 };
 
+// Vehicle filters of the exact scans.
+enum ScanOnly
+{
+    ONLY_ALL = 0,
+    ONLY_INIT = 1,       // vehicles initialising their particles in this step
+    ONLY_CDF_EXACT = 2,  // vehicles resampling with the exact CDF
+};
+
+__device__ __forceinline__ bool scan_takes(const VehDesc& d, int only)
+{
+    return only == ONLY_ALL || (only == ONLY_INIT && d.init) || (only == ONLY_CDF_EXACT && d.cdf_mode == CDF_EXACT);
+}
+
 // One workgroup per vehicle: exclusive prefix of the vehicle's tile totals (each thread scans a
 // contiguous chunk), optional inclusive tile ends, and the scalars derived from the vehicle's
-// total.  `init_only`: vehicles without first-frame initialisation are skipped.
+// total.  Vehicles the filter `only` excludes are skipped.
 template <int Unused = 0>
 __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ descs, int space,
                                                     const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
                                                     fx_t* __restrict__ tile_incl, DevScalars* __restrict__ scs,
-                                                    int mode, int init_only)
+                                                    int mode, int only)
 {
     const VehDesc& d = descs[blockIdx.x];
-    if (init_only && !d.init)
+    if (!scan_takes(d, only))
         return;
     DevScalars* sc = scs + blockIdx.x;
     const int t0 = d.u0[space] / SCAN_Q;
@@ -655,7 +710,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, const 
 
 template <class Src, class Sink>
 inline void run_exact_scan(Engine* E, int space, const Src& src, const Sink& sink, int mode, int kid_apply,
-                           bool init_only, fx_t* tile_incl = nullptr)
+                           int only, fx_t* tile_incl = nullptr)
 {
     const int tiles = E->tiles(space, SCAN_Q);
     if (tiles <= 0)
@@ -666,7 +721,7 @@ inline void run_exact_scan(Engine* E, int space, const Src& src, const Sink& sin
     launch_end(E, K_SCAN_TILES);
     launch_begin(E, K_SCAN_TOP);
     hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, space, E->scan_tile_sum,
-                       E->scan_tile_pre, tile_incl, E->scalars, mode, init_only ? 1 : 0);
+                       E->scan_tile_pre, tile_incl, E->scalars, mode, only);
     launch_end(E, K_SCAN_TOP);
     launch_begin(E, kid_apply);
     hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, E->stream, src, sink, E->d_desc,

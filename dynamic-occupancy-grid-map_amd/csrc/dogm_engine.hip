@@ -362,6 +362,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.shifted = (stages & ST_OCC) && v->pend_grid_shift ? 1 : 0;
         d.gx = v->pend_gx;
         d.gy = v->pend_gy;
+        d.cdf_mode = v->cdf_mode;
         d.part[0] = v->part[0];
         d.part[1] = v->part[1];
         d.birth = v->birth;
@@ -432,7 +433,10 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         run_moments(E, pph, pers_mom, (zero_empty & 2) != 0);
     if (stages & ST_RESAMPLE)
     {
-        run_resampling(E, pph);
+        int modes = 0;
+        for (int i = 0; i < m; ++i)
+            modes |= 1 << veh[i]->cdf_mode;
+        run_resampling(E, pph, modes);
         pph ^= 1;
     }
     // Host state: the descriptors already hold this step's buffers, so swaps take effect for the

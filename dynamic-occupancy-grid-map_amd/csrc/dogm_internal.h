@@ -101,6 +101,7 @@ struct VehDesc
     float mx, my;
     int shifted;             // pending grid shift (fused into the occupancy update)
     int gx, gy;
+    int cdf_mode;            // resampling CDF arithmetic (dogm_num::CdfMode)
     int u0[SP_COUNT];        // first table unit of this vehicle per space
     int nu[SP_COUNT];        // its units per space (multiple of UNIT_ALIGN)
     DevParticles part[2];
@@ -127,6 +128,7 @@ struct DevScalars
     fx_t exp_total;    // resampling: sum of the N + 1 exponentials
     dogm_num::Spacing spacing;  // resampling: draw mapping derived from exp_total and total
     double scale;      // particle-order scale v / total
+    double ftotal;     // resampling, float CDF modes: the CDF total (envelope maximum)
     float joint_max;   // resampling: f32 of the CDF total
     float new_weight;  // resampling: joint_max / N
     int pad[2];
@@ -204,6 +206,7 @@ struct Vehicle
     uint32_t epoch = 0;  // Philox epoch: incremented by every random-consuming stage
     uint64_t uid = 0;    // unique per allocation (work tables are reused while the list is unchanged)
     bool init_pending = false;  // initialise particles in the next step that includes ST_INIT
+    int cdf_mode = 0;           // resampling CDF arithmetic (dogm_num::CdfMode)
     float dt = 0.0f;            // time step of the next step
 
     DevParticles part[2]{};  // part[0] is current
@@ -352,7 +355,8 @@ void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize);  /
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
 void run_births(Engine* E);
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
-void run_resampling(Engine* E, int pph);           // reads pph, writes pph ^ 1
+// modes: bit m set when a vehicle of the step resamples with CDF mode m (dogm_num::CdfMode).
+void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 1
 void run_sort(Engine* E, int pph, bool hist_done); // per vehicle: sorted keys / permutation
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);

@@ -607,7 +607,7 @@ void run_births(Engine* E)
 {
     // With B = 0 the cell pass still runs (every cell gets n_j = 0 and w_A = w_UA = 0) and the
     // stage still consumes its RNG epoch, so the stream of later stages does not depend on B.
-    run_exact_scan(E, SP_C, SrcBorn{0}, SinkBirth{}, SCAN_ORDERS_B, K_BIRTH_CELLS, false);
+    run_exact_scan(E, SP_C, SrcBorn{0}, SinkBirth{}, SCAN_ORDERS_B, K_BIRTH_CELLS, ONLY_ALL);
     const int blocks = E->tiles(SP_B, 1);
     if (blocks <= 0)
         return;
@@ -659,7 +659,7 @@ void run_init_particles(Engine* E)
     launch_begin(E, K_COPY_MASSES);
     hipLaunchKernelGGL(k_copy_masses, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
     launch_end(E, K_COPY_MASSES);
-    run_exact_scan(E, SP_C, SrcBorn{1}, SinkOrders{}, SCAN_ORDERS_N, K_INIT_CELLS, true);
+    run_exact_scan(E, SP_C, SrcBorn{1}, SinkOrders{}, SCAN_ORDERS_N, K_INIT_CELLS, ONLY_INIT);
     launch_begin(E, K_INIT_PARTICLES);
     hipLaunchKernelGGL(k_init_particles, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc,
                        E->tab(SP_P));

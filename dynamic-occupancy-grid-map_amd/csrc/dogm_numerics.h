@@ -474,12 +474,40 @@ DOGM_HD Spacing spacing_setup(fx_t s_n, fx_t cdf_total)
     return sp;
 }
 
-DOGM_HD fx_t spacing_draw(fx_t s_k, const Spacing& sp)
+// Fraction u_k = S_k / S_N in [0, 1) of the spacing construction (the same for every CDF mode).
+DOGM_HD double spacing_fraction(fx_t s_k, const Spacing& sp)
 {
     double u = (double)(uint64_t)(s_k >> sp.shift) * sp.inv;
     if (!(u < 1.0))
         u = 0x1.fffffffffffffp-1;
-    return fx_from_f64(sp.total * u);
+    return u;
+}
+
+DOGM_HD fx_t spacing_draw(fx_t s_k, const Spacing& sp)
+{
+    return fx_from_f64(sp.total * spacing_fraction(s_k, sp));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Floating-point CDF modes of the resampling scan (precision study).  The exact mode above is the
+// default.  In the float modes the CDF is a scan of the non-negative joint weights in f64 or f32
+// with one fixed association (4-element sequential folds per thread, a 64-lane wave scan, and
+// sequential carries across waves and tiles; the checker restates it), its running maximum (the
+// monotone envelope E), total = E[J - 1], draws r_k = total * u_k in the mode's precision, and the
+// ancestor is the first j with E[j] >= r_k.
+// ---------------------------------------------------------------------------------------------
+enum CdfMode
+{
+    CDF_EXACT = 0,
+    CDF_FP64 = 1,
+    CDF_FP32 = 2,
+};
+
+DOGM_HD double cdf_float_draw(double total, double u, int mode)
+{
+    if (mode == CDF_FP32)
+        return (double)((float)total * (float)u);
+    return total * u;
 }
 
 // CDF contribution of a joint weight: negative values (from rounding in rho_p = occ - rho_b) and

@@ -26,7 +26,7 @@ static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-sc
 
 struct SrcJointPos
 {
-    __device__ bool active(const VehDesc&) const { return true; }
+    __device__ bool active(const VehDesc& d) const { return d.cdf_mode == CDF_EXACT; }
     __device__ int len(const VehDesc& d) const { return d.N + d.B; }
     __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const
     {
@@ -80,8 +80,224 @@ struct SrcExp
     }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Float CDF modes (precision study).  Per tile of 1024 joint weights: 4-element sequential folds
+// per thread, the 64-lane DPP wave scan, sequential carries across the 4 waves, and a sequential
+// prefix over the vehicle's tiles; element = ((tile prefix + wave carry) + thread's exclusive wave
+// prefix) + thread-local prefix.  The running maximum (monotone envelope) is what the ancestor
+// search sees.  The CDF is stored as doubles in the vehicle's CDF buffer; per tile, the coarse
+// index holds (envelope at the tile end, envelope before the tile) as two doubles.
+// ---------------------------------------------------------------------------------------------
+template <class T>
+struct CdfFloatMode;
+template <>
+struct CdfFloatMode<double>
+{
+    static constexpr int mode = CDF_FP64;
+};
+template <>
+struct CdfFloatMode<float>
+{
+    static constexpr int mode = CDF_FP32;
+};
+
+// Thread fold, wave scan and wave carries of one tile: the thread's local prefixes, its wave
+// carry (sum of the earlier waves' totals), its exclusive prefix inside the wave, and the tile total
+// (on every thread).
+template <class T>
+__device__ __forceinline__ void tile_float_scan(const VehDesc& d, int p0, int cnt, T (&loc)[4], T* carry_out,
+                                                T* excl_out, T* tile_total)
+{
+    __shared__ T s_w[BLOCK / WAVE];
+    float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (cnt > 0)
+        SrcJointPos{}.load4(d, p0, cnt, f);
+    T run = (T)0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        run = run + (T)(i < cnt ? f[i] : 0.0f);
+        loc[i] = run;
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const T incl = wave_inclusive_scan_val(run);
+    T excl = __shfl_up(incl, 1, WAVE);
+    if (lane == 0)
+        excl = (T)0;
+    if (lane == WAVE - 1)
+        s_w[wid] = incl;
+    __syncthreads();
+    T carry = (T)0;
+    for (int w = 0; w < wid; ++w)
+        carry = carry + s_w[w];
+    T tot = (T)0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / WAVE; ++w)
+        tot = tot + s_w[w];
+    __syncthreads();
+    *carry_out = carry;
+    *excl_out = excl;
+    *tile_total = tot;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cdff_tiles(const VehDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tab, double* __restrict__ tile_sum)
+{
+    const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    if (d.cdf_mode != CdfFloatMode<T>::mode)
+        return;
+    const int n = d.N + d.B;
+    const int p0 = tr.base + threadIdx.x * SCAN_ITEMS;
+    const int cnt = max(0, min(SCAN_ITEMS, n - p0));
+    T loc[4];
+    T carry, ex, tot;
+    tile_float_scan<T>(d, p0, cnt, loc, &carry, &ex, &tot);
+    if (threadIdx.x == 0)
+        tile_sum[blockIdx.x] = (double)tot;
+}
+
+// One thread per vehicle: sequential prefix over the vehicle's tile totals.
+template <class T>
+__global__ void k_cdff_top(const VehDesc* __restrict__ descs, int m, const double* __restrict__ tile_sum,
+                           double* __restrict__ tile_pre)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m)
+        return;
+    const VehDesc& d = descs[v];
+    if (d.cdf_mode != CdfFloatMode<T>::mode)
+        return;
+    const int t0 = d.u0[SP_J] / SCAN_Q;
+    const int nt = d.nu[SP_J] / SCAN_Q;
+    T run = (T)0;
+    for (int t = 0; t < nt; ++t)
+    {
+        tile_pre[t0 + t] = (double)run;
+        run = run + (T)tile_sum[t0 + t];
+    }
+}
+
+// Values, the envelope inside the tile, and the tile maximum (into env[2 t]).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cdff_apply(const VehDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tab,
+                                                      const double* __restrict__ tile_pre, double* __restrict__ env)
+{
+    __shared__ T s_m[BLOCK / WAVE];
+    const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    if (d.cdf_mode != CdfFloatMode<T>::mode)
+        return;
+    const int n = d.N + d.B;
+    const int p0 = tr.base + threadIdx.x * SCAN_ITEMS;
+    const int cnt = max(0, min(SCAN_ITEMS, n - p0));
+    T loc[4];
+    T carry, ex, tot;
+    tile_float_scan<T>(d, p0, cnt, loc, &carry, &ex, &tot);
+    const T tp = (T)tile_pre[blockIdx.x];
+    const int wid = threadIdx.x / WAVE;
+    const T base = (tp + carry) + ex;
+    T val[4];
+    T mx = (T)0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        val[i] = base + loc[i];
+        if (i < cnt && val[i] > mx)
+            mx = val[i];
+    }
+    // Exclusive running maximum over the earlier threads of the tile.
+    const T wmax = wave_inclusive_max_val(mx);
+    T before = __shfl_up(wmax, 1, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        before = (T)0;
+    if ((threadIdx.x & (WAVE - 1)) == WAVE - 1)
+        s_m[wid] = wmax;
+    __syncthreads();
+    for (int w = 0; w < wid; ++w)
+        before = s_m[w] > before ? s_m[w] : before;
+    T tmax = (T)0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / WAVE; ++w)
+        tmax = s_m[w] > tmax ? s_m[w] : tmax;
+    double* cdf = reinterpret_cast<double*>(d.cdf);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        if (i < cnt)
+        {
+            const T e = val[i] > before ? val[i] : before;
+            cdf[p0 + i] = (double)e;
+            before = e;
+        }
+    }
+    if (threadIdx.x == 0)
+        env[2 * (size_t)blockIdx.x] = (double)tmax;
+}
+
+// One thread per vehicle: the envelope across tiles, the total and the new particle weight.
+__global__ void k_cdff_env(const VehDesc* __restrict__ descs, int m, double* __restrict__ env,
+                           DevScalars* __restrict__ scs)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m)
+        return;
+    const VehDesc& d = descs[v];
+    if (d.cdf_mode == CDF_EXACT)
+        return;
+    const int t0 = d.u0[SP_J] / SCAN_Q;
+    const int nt = d.nu[SP_J] / SCAN_Q;
+    double run = 0.0;
+    for (int t = 0; t < nt; ++t)
+    {
+        const double tm = env[2 * (size_t)(t0 + t)];
+        env[2 * (size_t)(t0 + t) + 1] = run;
+        run = tm > run ? tm : run;
+        env[2 * (size_t)(t0 + t)] = run;
+    }
+    DevScalars* sc = scs + v;
+    sc->ftotal = run;
+    const float jm = (float)run;
+    sc->joint_max = jm;
+    sc->new_weight = jm / (float)d.N;
+}
+
+// Ancestor-search traits: exact 128-bit CDF or float-mode envelope (doubles).
+struct CdfExactTr
+{
+    using V = fx_t;
+    __device__ static bool takes(const VehDesc& d) { return d.cdf_mode == CDF_EXACT; }
+    __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars*, int) { return spacing_draw(s_k, sp); }
+    __device__ static V incl(const void* ci, int t0, int t) { return static_cast<const fx_t*>(ci)[t0 + t]; }
+    __device__ static V value(const VehDesc& d, const void*, int, int j) { return d.cdf[j]; }
+};
+
+struct CdfFloatTr
+{
+    using V = double;
+    __device__ static bool takes(const VehDesc& d) { return d.cdf_mode != CDF_EXACT; }
+    __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars* sc, int mode)
+    {
+        return cdf_float_draw(sc->ftotal, spacing_fraction(s_k, sp), mode);
+    }
+    __device__ static V incl(const void* ci, int t0, int t)
+    {
+        return static_cast<const double*>(ci)[2 * (size_t)(t0 + t)];
+    }
+    __device__ static V value(const VehDesc& d, const void* ci, int t0, int j)
+    {
+        const double c = reinterpret_cast<const double*>(d.cdf)[j];
+        const double b = static_cast<const double*>(ci)[2 * (size_t)(t0 + j / SCAN_TILE) + 1];
+        return c > b ? c : b;
+    }
+};
+
 // Index of the first CDF tile whose inclusive end is >= r (T - 1 if none): two-level k-ary search.
-__device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* s_cnt)
+template <class Tr>
+__device__ int find_cdf_tile(const void* __restrict__ ci, int t0, int T, typename Tr::V r, int* s_cnt)
 {
     const int G = (T + BLOCK - 1) / BLOCK;  // tiles per group; at most BLOCK groups
     const int ng = (T + G - 1) / G;
@@ -89,13 +305,13 @@ __device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* 
         s_cnt[0] = 0;
     __syncthreads();
     const int i = threadIdx.x;
-    const bool below = i < ng && incl[min(T - 1, (i + 1) * G - 1)] < r;
+    const bool below = i < ng && Tr::incl(ci, t0, min(T - 1, (i + 1) * G - 1)) < r;
     const int groups_below = __syncthreads_count(below);
     const int g = min(groups_below, ng - 1);
     for (int j = threadIdx.x; j < G; j += BLOCK)
     {
         const int t = g * G + j;
-        if (t < T && incl[t] < r)
+        if (t < T && Tr::incl(ci, t0, t) < r)
             atomicAdd(&s_cnt[0], 1);
     }
     __syncthreads();
@@ -104,14 +320,16 @@ __device__ int find_cdf_tile(const fx_t* __restrict__ incl, int T, fx_t r, int* 
     return t;
 }
 
+template <class Tr>
 __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ descs,
                                                     const uint32_t* __restrict__ tab, int pph,
                                                     const fx_t* __restrict__ exp_tile_pre,
-                                                    const fx_t* __restrict__ cdf_incl,
+                                                    const void* __restrict__ cdf_incl,
                                                     const DevScalars* __restrict__ scs)
 {
-    __shared__ fx_t s_cdf[RS_SLICE];
-    __shared__ fx_t s_rb[2];
+    using V = typename Tr::V;
+    __shared__ V s_cdf[RS_SLICE];
+    __shared__ V s_rb[2];
     __shared__ int s_cnt[1];
     __shared__ int s_kmin;
 
@@ -119,11 +337,10 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
     const VehDesc& d = descs[tr.v];
     const int n = d.N;
     const int base = tr.base;
-    if (base >= n)
+    if (base >= n || !Tr::takes(d))
         return;
     const DevScalars* sc = scs + tr.v;
     const int J = n + d.B;
-    const fx_t* cdf = d.cdf;
     const int p0 = base + threadIdx.x * RS_ITEMS;
     fx_t e[4];
     exp4(d, p0, n, e);
@@ -131,12 +348,12 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
     fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) +
                exp_tile_pre[d.u0[SP_E] / SCAN_Q + base / RS_TILE];
     const Spacing sp = sc->spacing;
-    fx_t r[RS_ITEMS];
+    V r[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
     {
         run += e[i];
-        r[i] = spacing_draw(run, sp);
+        r[i] = Tr::draw(run, sp, sc, d.cdf_mode);
     }
     const int last = min(base + RS_TILE, n) - 1;
     if (p0 == base)
@@ -147,9 +364,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
             s_rb[1] = r[i];
     __syncthreads();
     const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
-    const fx_t* incl = cdf_incl + d.u0[SP_J] / SCAN_Q;
-    const int t_lo = find_cdf_tile(incl, T, s_rb[0], s_cnt);
-    const int t_hi = find_cdf_tile(incl, T, s_rb[1], s_cnt);
+    const int t0 = d.u0[SP_J] / SCAN_Q;
+    const int t_lo = find_cdf_tile<Tr>(cdf_incl, t0, T, s_rb[0], s_cnt);
+    const int t_hi = find_cdf_tile<Tr>(cdf_incl, t0, T, s_rb[1], s_cnt);
     // Walk the CDF tiles [t_lo, t_hi] in LDS-sized chunks; a draw is resolved in the first chunk
     // whose last entry is >= its value (the CDF is non-decreasing, so its lower bound is there).
     // A thread's four draws are consecutive, so after one binary search the next lower bounds are
@@ -166,9 +383,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
         const int clen = min(J, (tc_end + 1) * SCAN_TILE) - c0;
         __syncthreads();
         for (int i = threadIdx.x; i < clen; i += BLOCK)
-            s_cdf[i] = cdf[c0 + i];
+            s_cdf[i] = Tr::value(d, cdf_incl, t0, c0 + i);
         __syncthreads();
-        const fx_t cmax = s_cdf[clen - 1];
+        const V cmax = s_cdf[clen - 1];
         int pos = -1;  // lower bound of the previous draw inside this chunk
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
@@ -229,12 +446,12 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
             if (p0 + i == kmin)
                 s_rb[0] = r[i];
         __syncthreads();
-        const fx_t rmin = s_rb[0];
+        const V rmin = s_rb[0];
         const int next_end = min(tc_end + RS_SLICE_TILES, t_hi);
-        if (!(incl[next_end] < rmin))
+        if (!(Tr::incl(cdf_incl, t0, next_end) < rmin))
             tc = tc_end + 1;
         else
-            tc = max(tc_end + 1, find_cdf_tile(incl, T, rmin, s_cnt));
+            tc = max(tc_end + 1, find_cdf_tile<Tr>(cdf_incl, t0, T, rmin, s_cnt));
     }
     // Transpose through LDS so the gather and the stores run wave-contiguous.
     __syncthreads();
@@ -268,9 +485,32 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
     }
 }
 
-void run_resampling(Engine* E, int pph)
+void run_resampling(Engine* E, int pph, int modes)
 {
-    run_exact_scan(E, SP_J, SrcJointPos{}, SinkCdf{}, SCAN_RESAMPLE, K_CDF_WRITE, false, E->cdf_tile_incl);
+    const bool exact = modes & (1 << CDF_EXACT);
+    if (exact)
+        run_exact_scan(E, SP_J, SrcJointPos{}, SinkCdf{}, SCAN_RESAMPLE, K_CDF_WRITE, ONLY_CDF_EXACT,
+                       E->cdf_tile_incl);
+    const int jtiles = E->tiles(SP_J, SCAN_Q);
+    double* tsum = reinterpret_cast<double*>(E->scan_tile_sum);
+    double* tpre = reinterpret_cast<double*>(E->scan_tile_pre);
+    double* env = reinterpret_cast<double*>(E->cdf_tile_incl);
+    const int vb = div_up(E->m, 64);
+    auto float_mode = [&](auto tag) {
+        using T = decltype(tag);
+        launch_begin(E, K_CDF_WRITE);
+        hipLaunchKernelGGL(k_cdff_tiles<T>, dim3(jtiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_J), tsum);
+        hipLaunchKernelGGL(k_cdff_top<T>, dim3(vb), dim3(64), 0, E->stream, E->d_desc, E->m, tsum, tpre);
+        hipLaunchKernelGGL(k_cdff_apply<T>, dim3(jtiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_J), tpre,
+                           env);
+        launch_end(E, K_CDF_WRITE);
+    };
+    if (modes & (1 << CDF_FP64))
+        float_mode(0.0);
+    if (modes & (1 << CDF_FP32))
+        float_mode(0.0f);
+    if (modes & ~(1 << CDF_EXACT))
+        hipLaunchKernelGGL(k_cdff_env, dim3(vb), dim3(64), 0, E->stream, E->d_desc, E->m, env, E->scalars);
 
     const int etiles = E->tiles(SP_E, SCAN_Q);
     launch_begin(E, K_EXP_TILES);
@@ -279,12 +519,17 @@ void run_resampling(Engine* E, int pph)
     launch_end(E, K_EXP_TILES);
     launch_begin(E, K_SCAN_TOP);
     hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_E, E->scan_tile_sum,
-                       E->scan_tile_pre, (fx_t*)nullptr, E->scalars, (int)SCAN_EXP, 0);
+                       E->scan_tile_pre, (fx_t*)nullptr, E->scalars, (int)SCAN_EXP, (int)ONLY_ALL);
     launch_end(E, K_SCAN_TOP);
 
+    const int mt = E->tiles(SP_P, RS_TILE / UNIT);
     launch_begin(E, K_RS_MERGE);
-    hipLaunchKernelGGL(k_rs_merge, dim3(E->tiles(SP_P, RS_TILE / UNIT)), dim3(BLOCK), 0, E->stream, E->d_desc,
-                       E->tab(SP_P), pph, E->scan_tile_pre, E->cdf_tile_incl, E->scalars);
+    if (exact)
+        hipLaunchKernelGGL(k_rs_merge<CdfExactTr>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                           E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+    if (modes & ~(1 << CDF_EXACT))
+        hipLaunchKernelGGL(k_rs_merge<CdfFloatTr>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                           E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
     launch_end(E, K_RS_MERGE);
 }
 

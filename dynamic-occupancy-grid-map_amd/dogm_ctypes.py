@@ -25,6 +25,7 @@ lib = C.CDLL(str(LIB_PATH))
 
 DOGM_OK = 0
 DOGM_DEFAULT_SEED = 123456
+CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2}
 
 
 class dogm_params(C.Structure):
@@ -111,6 +112,10 @@ _SIGS = {
     "dogm_set_kernel_timing": (_i, [_h, _i, _i]),
     "dogm_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "dogm_reset_kernel_timing": (_i, [_h]),
+    "dogm_set_cdf_mode": (_i, [_h, _i]),
+    "dogm_get_cdf_mode": (_i, [_h]),
+    "dogm_get_epoch": (C.c_uint32, [_h]),
+    "dogm_batch_set_cdf_mode": (_i, [_h, _i, _i]),
     "dogm_batch_create": (_i, [C.POINTER(_h)]),
     "dogm_batch_destroy": (None, [_h]),
     "dogm_batch_last_error": (C.c_char_p, [_h]),
@@ -353,6 +358,10 @@ class DOGM:
         self._check(lib.dogm_get_resampled_indices(self.h, _ptr(out)), "getResampledIndices")
         return out
 
+    def set_cdf_mode(self, mode: str) -> None:
+        """Resampling CDF arithmetic: 'exact' (default), 'fp64' or 'fp32'."""
+        self._check(lib.dogm_set_cdf_mode(self.h, CDF_MODES[mode]), "setCdfMode")
+
     def position(self) -> tuple[float, float, float]:
         return (lib.dogm_get_position_x(self.h), lib.dogm_get_position_y(self.h), lib.dogm_get_yaw(self.h))
 
@@ -483,6 +492,9 @@ class Batch:
         out = np.empty(self.vehicles[vid].particle_count, np.int32)
         self._check(lib.dogm_batch_get_resampled_indices(self.h, vid, _ptr(out)), "getResampledIndices")
         return out
+
+    def set_cdf_mode(self, vid: int, mode: str) -> None:
+        self._check(lib.dogm_batch_set_cdf_mode(self.h, vid, CDF_MODES[mode]), "setCdfMode")
 
     def set_kernel_timing(self, name: str, enable: bool = True) -> None:
         self._check(lib.dogm_batch_set_kernel_timing(self.h, DOGM.kernel_names().index(name), int(enable)),

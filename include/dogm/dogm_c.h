@@ -182,6 +182,28 @@ int dogm_get_kernel_timing(dogm_handle* h, int kernel_id, double* total_ms, long
 int dogm_reset_kernel_timing(dogm_handle* h);
 
 /* ----------------------------------------------------------------------------------------------
+ * Precision of the resampling CDF (the cumulative-weight scan behind resampling.cu:34-47 and the
+ * new weight total / N, dogm.cu:386-423).  Particle state and weights stay f32 in every mode.
+ *  DOGM_CDF_EXACT (default): exact 128-bit fixed-point prefix sums; the draws and the ancestors
+ *    do not depend on any summation order.
+ *  DOGM_CDF_FP64 / DOGM_CDF_FP32: an f64 or f32 scan with one fixed association (4-element
+ *    sequential folds, 64-lane wave scans, sequential carries across waves and 1024-entry tiles),
+ *    ancestor = first entry of its running maximum >= the draw.  FP32 is the reference's class of
+ *    scan (thrust::inclusive_scan in f32, dogm.cu:405-407).
+ * ---------------------------------------------------------------------------------------------- */
+enum
+{
+    DOGM_CDF_EXACT = 0,
+    DOGM_CDF_FP64 = 1,
+    DOGM_CDF_FP32 = 2
+};
+int dogm_set_cdf_mode(dogm_handle* h, int mode);
+int dogm_get_cdf_mode(const dogm_handle* h);
+/* Philox epoch of the handle's next random stage (the draws of a resampling called now use it);
+ * lets a host reference rebuild the same uniforms. */
+uint32_t dogm_get_epoch(const dogm_handle* h);
+
+/* ----------------------------------------------------------------------------------------------
  * Batched update of many independent grids ("vehicles").  No counterpart in the reference, which
  * runs one dogm::DOGM per grid, one frame per updateGrid call (dogm/src/dogm.cu:115-131).
  *
@@ -217,6 +239,7 @@ int dogm_batch_get_grid_cells(dogm_batch* b, int vehicle_id, dogm_grid_cell* out
 int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, int* grid_cell_idx, float* weight,
                              uint8_t* associated);
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
+int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
 int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable);
 int dogm_batch_get_kernel_timing(dogm_batch* b, int kernel_id, double* total_ms, long long* launches);
 int dogm_batch_reset_kernel_timing(dogm_batch* b);

@@ -48,6 +48,8 @@ def lib():
             "oracle_position_y": (f, [vp]),
             "oracle_yaw": (f, [vp]),
             "oracle_set_flags": (None, [vp, i, i]),
+            "oracle_set_cdf_mode": (None, [vp, i]),
+            "oracle_spacing_fractions": (None, [C.c_uint64, C.c_uint32, C.c_uint32, i, vp]),
             "oracle_philox": (None, [C.c_uint32] * 6 + [vp]),
             "oracle_log_f32": (f, [f]),
             "oracle_sincos_2pi": (None, [f, C.POINTER(f), C.POINTER(f)]),
@@ -242,6 +244,10 @@ class OracleDOGM:
     def set_flags(self, first_pose: bool, first_meas: bool) -> None:
         lib().oracle_set_flags(self.h, int(first_pose), int(first_meas))
 
+    def set_cdf_mode(self, mode: str) -> None:
+        """Resampling CDF arithmetic of the f32 checker: 'exact', 'fp64' or 'fp32'."""
+        lib().oracle_set_cdf_mode(self.h, {"exact": 0, "fp64": 1, "fp32": 2}[mode])
+
     def position(self):
         return (lib().oracle_position_x(self.h), lib().oracle_position_y(self.h), lib().oracle_yaw(self.h))
 
@@ -317,3 +323,10 @@ def fx_scale_u64(total: int, u: int) -> int:
 
 def order_end(prefix: int, total: int, v: int) -> int:
     return lib().oracle_order_end(*_split(prefix), *_split(total), v)
+
+
+def spacing_fractions(seed: int, instance: int, epoch: int, n: int) -> np.ndarray:
+    """The n draw fractions u_k in [0, 1) of one resampling (shared by every CDF mode)."""
+    out = np.empty(n, np.float64)
+    lib().oracle_spacing_fractions(seed, instance, epoch, n, out.ctypes.data)
+    return out

@@ -84,7 +84,93 @@ struct OracleBase
     float pos_x = 0.0f, pos_y = 0.0f, yaw = 0.0f;
     bool first_pose = false, first_meas = false;
     uint32_t epoch = 0;
+    int cdf_mode = 0;  // CdfMode of the resampling scan (f32 checker only; the f64 twin is exact)
 };
+
+// Float CDF modes: restatement of the device association (dogm_resample.hip, k_cdff_*): per tile of
+// 1024 entries, 4-element sequential folds per thread, the 64-lane wave scan of the thread totals as
+// six simultaneous steps (row shifts 1, 2, 4, 8 inside 16-lane rows, then lane 15 of rows 0 / 2 into
+// rows 1 / 3 and lane 31 into rows 2 / 3; every step adds the moved value), sequential carries
+// across the 4 waves and across tiles; element = ((tile prefix + wave carry) + exclusive wave
+// prefix) + thread-local prefix; E = running maximum.
+template <typename F>
+static void float_cdf(const std::vector<float>& w, std::vector<double>& E)
+{
+    const int J = (int)w.size();
+    const int ntiles = (J + 1023) / 1024;
+    F tp = (F)0;
+    double emax = 0.0;
+    E.assign(J, 0.0);
+    for (int k = 0; k < ntiles; ++k)
+    {
+        F loc[256][4];
+        F incl[256];
+        for (int t = 0; t < 256; ++t)
+        {
+            F run = (F)0;
+            for (int i = 0; i < 4; ++i)
+            {
+                const int j = k * 1024 + 4 * t + i;
+                run = run + (F)(j < J ? w[j] : 0.0f);
+                loc[t][i] = run;
+            }
+            incl[t] = run;
+        }
+        F W[4];
+        for (int wv = 0; wv < 4; ++wv)
+        {
+            F* x = incl + 64 * wv;
+            for (int d = 1; d <= 8; d <<= 1)
+            {
+                F y[64];
+                for (int l = 0; l < 64; ++l)
+                    y[l] = (l % 16 >= d) ? x[l] + x[l - d] : x[l];
+                for (int l = 0; l < 64; ++l)
+                    x[l] = y[l];
+            }
+            {
+                F y[64];
+                for (int l = 0; l < 64; ++l)
+                {
+                    const int row = l / 16;
+                    y[l] = (row == 1 || row == 3) ? x[l] + x[16 * row - 1] : x[l];
+                }
+                for (int l = 0; l < 64; ++l)
+                    x[l] = y[l];
+            }
+            {
+                F y[64];
+                for (int l = 0; l < 64; ++l)
+                    y[l] = (l / 16 >= 2) ? x[l] + x[31] : x[l];
+                for (int l = 0; l < 64; ++l)
+                    x[l] = y[l];
+            }
+            W[wv] = x[63];
+        }
+        F tot = (F)0;
+        for (int wv = 0; wv < 4; ++wv)
+            tot = tot + W[wv];
+        for (int t = 0; t < 256; ++t)
+        {
+            const int wv = t / 64, l = t % 64;
+            F carry = (F)0;
+            for (int u = 0; u < wv; ++u)
+                carry = carry + W[u];
+            const F excl = l == 0 ? (F)0 : incl[t - 1];
+            const F base = (tp + carry) + excl;
+            for (int i = 0; i < 4; ++i)
+            {
+                const int j = k * 1024 + 4 * t + i;
+                if (j >= J)
+                    continue;
+                const double v = (double)(base + loc[t][i]);
+                emax = v > emax ? v : emax;
+                E[j] = emax;
+            }
+        }
+        tp = tp + tot;
+    }
+}
 
 // Field ids for oracle_get / oracle_set (double transport) and oracle_get_i32 / oracle_set_i32.
 enum Field
@@ -541,6 +627,11 @@ struct Oracle final : OracleBase
 
     void resampling() override
     {
+        if (cdf_mode != CDF_EXACT && sizeof(T) == sizeof(float))
+        {
+            resampling_float();
+            return;
+        }
         const int J = N + B;
         std::vector<fx_t> cdf(J);
         fx_t s = 0;
@@ -590,6 +681,60 @@ struct Oracle final : OracleBase
             next.idx[k] = src.idx[o];
             next.assoc[k] = src.assoc[o];
             next.w[k] = new_weight;
+        }
+        part = std::move(next);
+        ++epoch;
+    }
+
+    // Float CDF modes (precision study): same draws u_k, CDF from float_cdf, ancestor = first j
+    // with E[j] >= r_k, r_k = total * u_k in the mode's precision (dogm_numerics.h, cdf_float_draw).
+    void resampling_float()
+    {
+        const int J = N + B;
+        std::vector<float> wj(J);
+        for (int k = 0; k < N; ++k)
+            wj[k] = cdf_weight((float)weight_array[k]);
+        for (int k = 0; k < B; ++k)
+            wj[N + k] = cdf_weight((float)birth.w[k]);
+        std::vector<double> E;
+        if (cdf_mode == CDF_FP64)
+            float_cdf<double>(wj, E);
+        else
+            float_cdf<float>(wj, E);
+        const double total = J > 0 ? E[J - 1] : 0.0;
+        std::vector<fx_t> spac(N + 1);
+        fx_t e = 0;
+        float e4[4];
+        for (int k = 0; k <= N; ++k)
+        {
+            if ((k & 3) == 0)
+                resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
+            e += fx_from_f32(e4[k & 3]);
+            spac[k] = e;
+        }
+        const Spacing sp = spacing_setup(spac[N], 0);
+        for (int k = 0; k < N; ++k)
+        {
+            const double rv = cdf_float_draw(total, spacing_fraction(spac[k], sp), cdf_mode);
+            const int lo = (int)(std::lower_bound(E.begin(), E.end(), rv) - E.begin());
+            ancestors[k] = lo < J ? lo : J - 1;
+        }
+        const float joint_max = (float)total;
+        const float new_weight = joint_max / (float)N;
+        ParticleSet<T> next;
+        next.resize(N);
+        for (int k = 0; k < N; ++k)
+        {
+            const int a = ancestors[k];
+            const ParticleSet<T>& src = a < N ? part : birth;
+            const int o = a < N ? a : a - N;
+            next.x[k] = src.x[o];
+            next.y[k] = src.y[o];
+            next.vx[k] = src.vx[o];
+            next.vy[k] = src.vy[o];
+            next.idx[k] = src.idx[o];
+            next.assoc[k] = src.assoc[o];
+            next.w[k] = (T)new_weight;
         }
         part = std::move(next);
         ++epoch;
@@ -752,12 +897,32 @@ void oracle_set_epoch(void* h, uint32_t e) { O->epoch = e; }
 float oracle_position_x(void* h) { return O->pos_x; }
 float oracle_position_y(void* h) { return O->pos_y; }
 float oracle_yaw(void* h) { return O->yaw; }
+void oracle_set_cdf_mode(void* h, int mode) { O->cdf_mode = mode; }
 void oracle_set_flags(void* h, int first_pose, int first_meas)
 {
     O->first_pose = first_pose != 0;
     O->first_meas = first_meas != 0;
 }
 #undef O
+
+// The N draw fractions u_k of one resampling (seed, instance, epoch): exponential spacings with
+// the exact prefix, as every CDF mode uses them.
+void oracle_spacing_fractions(uint64_t seed, uint32_t instance, uint32_t epoch, int n, double* out)
+{
+    std::vector<fx_t> spac((size_t)n + 1);
+    fx_t e = 0;
+    float e4[4];
+    for (int k = 0; k <= n; ++k)
+    {
+        if ((k & 3) == 0)
+            resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
+        e += fx_from_f32(e4[k & 3]);
+        spac[k] = e;
+    }
+    const Spacing sp = spacing_setup(spac[n], 0);
+    for (int k = 0; k < n; ++k)
+        out[k] = spacing_fraction(spac[k], sp);
+}
 
 // Numerics entry points for the known-answer tests of dogm_numerics.h.
 void oracle_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t* out)
