@@ -24,6 +24,7 @@ import time
 from pathlib import Path
 
 import numpy as np
+from ctypes import c_void_p as C_void
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "dynamic-occupancy-grid-map_amd"))
@@ -43,26 +44,22 @@ WORKLOAD_TEXT = {
 
 
 def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None:
-    """Algorithmic bytes one launch must move (read once what it consumes, write once what it hands
-    on); gathers through cell indices count their payload once."""
+    """Algorithmic bytes one launch must move per vehicle (read once what it consumes, write once what
+    it hands on); gathers through cell indices count their payload once."""
     table = {
         "predict": 44 * N,                       # state+weight in, state+weight+idx out
-        "sort_hist": 4 * N,                      # keys
-        "sort_upsweep": 4 * N,
-        "sort_downsweep": 16 * N,                # keys+values in and out (first pass reads keys only)
-        "sort_gather": 58 * N,                   # perm+keys, 21 B payload in; 21 B payload + idx + weight copy out
+        "sort_upsweep": 4 * N,                   # keys
+        # per pass keys+values in and out; the last pass instead gathers the 21 B payload and writes the
+        # sorted particles (25 B) plus the weight copy: averaged over the passes of one sort
+        "sort_downsweep": (16 * N * (passes - 1) + (8 + 21 + 25 + 4) * N) / passes,
         "seg_occ": 8 * N,                        # keys + weights
         "mass_update": 40 * C,                   # start, free, meas occ/free in; 6 fields out
-        "normalize": 4 * N,                      # cell index
         "seg_pers": 12 * N,                      # keys + weights in, likelihood-weighted weights out
-        "post_weights": 16 * N,                  # idx, weight, lw in; posterior out
-        "seg_moments": 16 * N,                   # keys, posterior, vx, vy
-        "cdf_write": 20 * (N + B),               # joint weights in, exact CDF out
-        "rs_merge": 50 * N,                      # CDF slice in, 21 B ancestor payload in, 25 B + ancestor out
+        "seg_moments": 24 * N,                   # keys, lik-weighted and prior weights, vx, vy in; posterior out
+        "rs_merge": 50 * N + 4 * B,              # joint weights, 21 B ancestor payload in; 25 B particles out
         "birth_particles": 25 * B,
         "birth_cells": 20 * C,
         "reset_bounds": 8 * C,
-        "empty_cells": 24 * C,
         "scan_tiles": 4 * C,
     }
     return table.get(name)
@@ -272,6 +269,9 @@ def run_serving(args) -> int:
     return 0
 
 
+# Every kernel a resampling stage launches (CDF tile totals and top scan, float-mode CDF, spacings, merge).
+RESAMPLING_KERNELS = ("scan_tiles", "scan_top", "cdf_write", "exp_tiles", "rs_merge")
+
 PRECISION_PARAMS = dict(size=204.8, resolution=0.1, particle_count=8_000_000, new_born_particle_count=800_000,
                         persistence_prob=0.99, stddev_process_noise_position=0.1,
                         stddev_process_noise_velocity=2.0, birth_prob=0.02, stddev_velocity=30.0,
@@ -302,8 +302,8 @@ def run_precision(args) -> int:
     for mo in modes:
         g = DOGM(p, seed=123456, instance=0)
         g.set_cdf_mode(mo)
-        g.set_kernel_timing("rs_merge", True)
-        g.set_kernel_timing("cdf_write", True)
+        for kn in RESAMPLING_KERNELS:
+            g.set_kernel_timing(kn, True)
         handles[mo] = g
     rng = np.random.Generator(np.random.Philox(args.seed))
     agree = {mo: [] for mo in modes}
@@ -341,7 +341,7 @@ def run_precision(args) -> int:
             g.reset_kernel_timing()
             epoch = _handle_epoch(g)
             g.resampling()
-            ms = sum(g.kernel_timing(k)[0] for k in ("rs_merge", "cdf_write"))
+            ms = sum(g.kernel_timing(k)[0] for k in RESAMPLING_KERNELS)
             rs_ms[mo].append(ms)
             u = oracle_ctypes.spacing_fractions(123456, 0, epoch, N)
             ref = np.minimum(np.searchsorted(cdf, total * u, side="left"), N + B - 1)
@@ -364,7 +364,7 @@ def run_precision(args) -> int:
         "modes": {mo: {"agreement_mean": round(float(np.mean(agree[mo])), 6),
                        "agreement_min": round(float(np.min(agree[mo])), 6),
                        "new_weight_total_rel_error_max": float(np.max(tot_err[mo])),
-                       "cdf_and_merge_ms_mean": round(float(np.mean(rs_ms[mo])), 4)} for mo in modes},
+                       "resampling_ms_mean": round(float(np.mean(rs_ms[mo])), 4)} for mo in modes},
         "masses_vs_fp64_checker": mass,
     }
     print(json.dumps(out))
@@ -470,10 +470,11 @@ def main() -> int:
         bufs.append(d)
 
     if args.copy_calibration_mb > 0:
+        # One copy kernel dispatch that reads and writes exactly this many bytes (the reference for
+        # scaling the FETCH_SIZE / WRITE_SIZE counters of a profiling run).
         nb = args.copy_calibration_mb << 20
         src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
-        lib.dogm_memcpy(dst.ptr, src.ptr, nb, 2)
-        lib.dogm_device_synchronize()
+        assert lib.dogm_device_copy(C_void(dst.ptr), C_void(src.ptr), nb) == 0
         src.free()
         dst.free()
 

@@ -195,6 +195,18 @@ Vehicle* find_vehicle(dogm_batch* b, int id)
 
 }  // namespace
 
+// Single-dispatch device-to-device copy (16-byte vectors, grid-stride), on the null stream.
+__global__ __launch_bounds__(BLOCK) void k_copy_bytes(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      size_t n16, const uint8_t* __restrict__ src_tail,
+                                                      uint8_t* __restrict__ dst_tail, int tail)
+{
+    const size_t stride = (size_t)gridDim.x * BLOCK;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail)
+        dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 // ================================================================================================
 extern "C" {
 
@@ -474,6 +486,23 @@ int dogm_memcpy(void* dst, const void* src, size_t bytes, int kind)
     default: return DOGM_ERR_INVALID_ARGUMENT;
     }
     return hipMemcpy(dst, src, bytes, k) == hipSuccess ? DOGM_OK : DOGM_ERR_HIP;
+}
+
+int dogm_device_copy(void* dst, const void* src, size_t bytes)
+{
+    if ((!dst || !src) && bytes)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    if (((uintptr_t)dst | (uintptr_t)src) & 15u)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    const size_t n16 = bytes / 16;
+    const int tail = (int)(bytes % 16);
+    const int blocks = (int)std::min<size_t>(std::max<size_t>(1, (n16 + BLOCK - 1) / BLOCK), 16384);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(blocks), dim3(BLOCK), 0, nullptr, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16, static_cast<const uint8_t*>(src) + n16 * 16,
+                       static_cast<uint8_t*>(dst) + n16 * 16, tail);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return DOGM_ERR_HIP;
+    return DOGM_OK;
 }
 
 int dogm_device_synchronize(void)

@@ -115,23 +115,47 @@ __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Vector access to a thread's ITEMS consecutive elements (16-byte loads and stores when the run
-// is complete; ITEMS is a multiple of 4 and runs start at multiples of ITEMS).
+// Vector access to a thread's ITEMS consecutive elements of 32-bit values: 16-byte (ITEMS a
+// multiple of 4) or 8-byte (ITEMS even) accesses when the run is complete; runs start at multiples
+// of ITEMS.
 // ---------------------------------------------------------------------------------------------
+template <int IT>
+constexpr int run_width()
+{
+    return IT % 4 == 0 ? 4 : (IT % 2 == 0 ? 2 : 1);
+}
+
+template <int IT, class T>
+__device__ __forceinline__ void load_full_run(const T* __restrict__ a, int p0, T (&v)[IT])
+{
+    static_assert(sizeof(T) == 4, "32-bit elements");
+    constexpr int W = run_width<IT>();
+#pragma unroll
+    for (int i = 0; i < IT; i += W)
+    {
+        if constexpr (W == 4)
+        {
+            const int4 q = *reinterpret_cast<const int4*>(a + p0 + i);
+            __builtin_memcpy(&v[i], &q, 16);
+        }
+        else if constexpr (W == 2)
+        {
+            const int2 q = *reinterpret_cast<const int2*>(a + p0 + i);
+            __builtin_memcpy(&v[i], &q, 8);
+        }
+        else
+        {
+            v[i] = a[p0 + i];
+        }
+    }
+}
+
 template <int IT>
 __device__ __forceinline__ void load_run(const float* __restrict__ a, int p0, int cnt, float (&v)[IT])
 {
     if (cnt == IT)
     {
-#pragma unroll
-        for (int i = 0; i < IT; i += 4)
-        {
-            const float4 q = *reinterpret_cast<const float4*>(a + p0 + i);
-            v[i] = q.x;
-            v[i + 1] = q.y;
-            v[i + 2] = q.z;
-            v[i + 3] = q.w;
-        }
+        load_full_run<IT, float>(a, p0, v);
     }
     else
     {
@@ -146,15 +170,7 @@ __device__ __forceinline__ void load_run(const int* __restrict__ a, int p0, int 
 {
     if (cnt == IT)
     {
-#pragma unroll
-        for (int i = 0; i < IT; i += 4)
-        {
-            const int4 q = *reinterpret_cast<const int4*>(a + p0 + i);
-            v[i] = q.x;
-            v[i + 1] = q.y;
-            v[i + 2] = q.z;
-            v[i + 3] = q.w;
-        }
+        load_full_run<IT, int>(a, p0, v);
     }
     else
     {
@@ -168,14 +184,28 @@ template <int IT, class T>
 __device__ __forceinline__ void store_run(T* __restrict__ a, int p0, int cnt, const T (&v)[IT])
 {
     static_assert(sizeof(T) == 4, "32-bit elements");
+    constexpr int W = run_width<IT>();
     if (cnt == IT)
     {
 #pragma unroll
-        for (int i = 0; i < IT; i += 4)
+        for (int i = 0; i < IT; i += W)
         {
-            float4 q;
-            __builtin_memcpy(&q, &v[i], 16);
-            *reinterpret_cast<float4*>(a + p0 + i) = q;
+            if constexpr (W == 4)
+            {
+                int4 q;
+                __builtin_memcpy(&q, &v[i], 16);
+                *reinterpret_cast<int4*>(a + p0 + i) = q;
+            }
+            else if constexpr (W == 2)
+            {
+                int2 q;
+                __builtin_memcpy(&q, &v[i], 8);
+                *reinterpret_cast<int2*>(a + p0 + i) = q;
+            }
+            else
+            {
+                a[p0 + i] = v[i];
+            }
         }
     }
     else
@@ -279,8 +309,10 @@ __device__ __forceinline__ TileRef tile_ref(const uint32_t* __restrict__ tab, in
 // tile ends every segment, so no carry crosses vehicles.
 //
 // Op interface:
-//   static constexpr int NCH, ITEMS;                                    // ITEMS % 4 == 0
+//   static constexpr int NCH, ITEMS;                                    // ITEMS even
 //   __device__ const int* keys(const VehDesc& d) const;                 // cell index per particle
+//   __device__ void head(const VehDesc& d, int key, int p) const;       // p starts the segment of key
+//   __device__ void end(const VehDesc& d, int key, int p) const;        // p ends the segment of key
 //   __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS],
 //                        float (&v)[ITEMS][NCH]) const;
 //   __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[NCH]) const;
@@ -292,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const VehDesc* __res
     constexpr int NCH = Op::NCH;
     constexpr int IT = Op::ITEMS;
     constexpr int TILE = BLOCK * IT;
-    static_assert(IT % 4 == 0, "runs are loaded with 16-byte accesses");
+    static_assert(IT % 2 == 0 && BLOCK * IT % UNIT == 0, "runs are loaded with 8- or 16-byte accesses");
     constexpr int QCAP = BLOCK;  // queued segment ends per tile
     __shared__ int s_wh[BLOCK / WAVE];
     __shared__ fx_t s_wb[BLOCK / WAVE][NCH];
@@ -413,9 +445,12 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const VehDesc* __res
             for (int c = 0; c < NCH; ++c)
                 base[c] = ex[c] + (i == 0 ? (fx_t)0 : pre[i - 1][c]);
             in_tile = true;
+            op.head(d, key[i], p);
         }
         const int nk = i + 1 < IT ? key[i + 1] : knext;
         const bool is_end = (p == n - 1) || nk != key[i];
+        if (is_end)
+            op.end(d, key[i], p);
         if (is_end || p == tlast)
         {
             fx_t s[NCH];

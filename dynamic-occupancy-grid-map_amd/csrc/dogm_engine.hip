@@ -288,7 +288,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
             total += tot[s];
         }
         const long long scan_tiles = std::max({tot[SP_C], tot[SP_J], tot[SP_E]}) / 4;
-        const long long seg_tiles = tot[SP_P] / 4;
+        const long long seg_tiles = tot[SP_P] / 2;  // segmented reductions take 512 or more particles per tile
         const long long sort_tiles = tot[SP_P] / 8;
         if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
             return set_err(&E->last_error, st, why);
@@ -412,6 +412,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     int pph = 0;
     const bool hist = (stages & ST_PREDICT) && (stages & ST_ASSIGN);
     const bool occ_pers = (stages & ST_OCC) && (stages & ST_PERS);
+    const bool assign_occ = (stages & ST_ASSIGN) && (stages & ST_OCC);
     const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
     const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
     if (any_init)
@@ -420,11 +421,11 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         run_predict(E, pph, hist);
     if (stages & ST_ASSIGN)
     {
-        run_assignment(E, pph, hist);
+        run_assignment(E, pph, hist, assign_occ);
         pph ^= 1;
     }
     if (stages & ST_OCC)
-        run_occupancy(E, pph, zero_empty, occ_pers);
+        run_occupancy(E, pph, zero_empty, occ_pers, assign_occ);
     if (stages & ST_PERS)
         run_persistent(E, pph, occ_pers, (zero_empty & 1) != 0, pers_mom);
     if (stages & ST_BIRTH)

@@ -350,14 +350,16 @@ void run_build_tables(Engine* E);
 // the moments' load.  Each stage run alone keeps the reference's separate passes.
 void run_init_particles(Engine* E);                // pph 0
 void run_predict(Engine* E, int pph, bool hist);
-void run_assignment(Engine* E, int pph, bool hist_done);  // reads pph, writes pph ^ 1
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize);  // free[0] -> free[1]
+// defer_bounds: the last sort pass writes the particles directly and the occupancy reduction records
+// the cell bounds (bounds = true there).
+void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds);  // reads pph, writes pph ^ 1
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds);  // free[0] -> free[1]
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
 void run_births(Engine* E);
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
 // modes: bit m set when a vehicle of the step resamples with CDF mode m (dogm_num::CdfMode).
 void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 1
-void run_sort(Engine* E, int pph, bool hist_done); // per vehicle: sorted keys / permutation
+void run_sort(Engine* E, int pph, bool hist_done, bool gather);  // gather: last pass writes the particles
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);

@@ -278,12 +278,14 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
         d.end[key] = k;
 }
 
-void run_assignment(Engine* E, int pph, bool hist_done)
+void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds)
 {
     launch_begin(E, K_RESET_BOUNDS);
     hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
     launch_end(E, K_RESET_BOUNDS);
-    run_sort(E, pph, hist_done);
+    run_sort(E, pph, hist_done, defer_bounds);
+    if (defer_bounds)
+        return;  // the last sort pass wrote the particles; the occupancy reduction records the bounds
     launch_begin(E, K_SORT_GATHER);
     hipLaunchKernelGGL(k_sort_gather, dim3(E->tiles(SP_P, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
                        pph, (E->passes - 1) & 1);
@@ -298,7 +300,18 @@ struct OccOp
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
     int pph;
+    int bounds;  // record start_idx / end_idx (dogm.cu:262-281) when the assignment deferred them
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void head(const VehDesc& d, int key, int p) const
+    {
+        if (bounds)
+            d.start[key] = p;
+    }
+    __device__ void end(const VehDesc& d, int key, int p) const
+    {
+        if (bounds)
+            d.end[key] = p;
+    }
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];
@@ -391,9 +404,9 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
     }
 }
 
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize)
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds)
 {
-    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph});
+    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph, bounds ? 1 : 0});
     launch_begin(E, K_MASS_UPDATE);
     hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
                        zero_empty);
@@ -420,6 +433,8 @@ struct PersOp
     int pph;
     int norm;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void head(const VehDesc&, int, int) const {}
+    __device__ void end(const VehDesc&, int, int) const {}
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS], lw[ITEMS];
@@ -558,11 +573,14 @@ struct SinkBirth
                 e[i] = order_end_at(j, pre[i], sc, C, v);
                 const int s = j == 0 ? 0 : order_end_at(j - 1, pre[i] - own[i], sc, C, v);
                 const int n = e[i] - s;
-                const float pA = d.meas[j].p_A;
-                const int nu_A = (int)roundf((float)n * pA);
-                const int nu_UA = n - nu_A;
-                a[i] = nu_A > 0 ? (pA * bm[i]) / (float)nu_A : 0.0f;
-                u[i] = nu_UA > 0 ? ((1.0f - pA) * bm[i]) / (float)nu_UA : 0.0f;
+                if (n > 0)  // cells without birth slots keep w_A = w_UA = 0 and skip the measurement read
+                {
+                    const float pA = d.meas[j].p_A;
+                    const int nu_A = (int)roundf((float)n * pA);
+                    const int nu_UA = n - nu_A;
+                    a[i] = nu_A > 0 ? (pA * bm[i]) / (float)nu_A : 0.0f;
+                    u[i] = nu_UA > 0 ? ((1.0f - pA) * bm[i]) / (float)nu_UA : 0.0f;
+                }
             }
         }
         store_run<4>(d.pend, p0, cnt, e);
@@ -679,6 +697,8 @@ struct MomOp
     int pph;
     int post;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
+    __device__ void head(const VehDesc&, int, int) const {}
+    __device__ void end(const VehDesc&, int, int) const {}
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];

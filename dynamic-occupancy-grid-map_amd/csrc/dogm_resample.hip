@@ -1,15 +1,17 @@
 // Multinomial resampling on gfx950 (dogm/src/dogm.cu:386-423, resampling.cu:17-68).
 //
-// 1. Exact CDF over [posterior weights | birth weights] (reduce, scan of tile totals, rescan); the
-//    inclusive CDF value at the end of every 1024-entry tile is kept as a coarse index.
+// 1. Exact CDF over [posterior weights | birth weights]: tile totals and their scan give the
+//    inclusive CDF value at the end of every 1024-entry tile (the coarse index); entries are not
+//    stored but recomputed per staged tile in the merge.
 // 2. Sorted uniforms by exponential spacings: E_k for k = 0..N, exact prefix S_k, draw k is the
 //    fraction S_k / S_N of the CDF total (spacing_draw in dogm_numerics.h).  Only the tile totals of E are
 //    stored; the merge kernel regenerates E from the counter-based RNG (four E per Philox call).
 // 3. Merge: a workgroup owns 1024 consecutive (hence sorted) draws.  Its ancestors lie between the
 //    CDF tiles holding its first and last draw; those are found by a two-level k-ary search over
 //    the coarse index (two dependent loads instead of a 22-level binary search), the CDF entries of
-//    those tiles are staged in LDS, every draw finds its ancestor there, and the next particle is
-//    written directly (the reference's lower_bound and resamplingKernel gather in one pass).
+//    those tiles are rebuilt in LDS from the weights (exact block scan + tile prefix), every draw
+//    finds its ancestor there, and the next particle is written directly (the reference's
+//    lower_bound and resamplingKernel gather in one pass).
 // No atomics on global memory and no global sort: the result is a pure function of the inputs.
 // Every vehicle of a step resamples from its own CDF with its own draws.
 #include "dogm_device.h"
@@ -19,8 +21,12 @@ namespace dogm_hip
 
 constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = BLOCK * RS_ITEMS;  // draws per workgroup
-constexpr int RS_SLICE_TILES = 2;          // CDF tiles staged in LDS
-constexpr int RS_SLICE = RS_SLICE_TILES * SCAN_TILE;
+// CDF tiles staged in LDS per chunk: 2 for launches of up to RS_WIDE_MAX_TILES workgroups (fewer chunks
+// per workgroup shortens the critical path), 1 for larger launches (half the LDS, twice the
+// workgroups per CU).  Measured on 2e6-particle grids: two tiles win for one grid per launch (1954
+// workgroups), one tile for eight (15.6k workgroups).
+constexpr int RS_SLICE_MAX_TILES = 2;
+constexpr int RS_WIDE_MAX_TILES = 4096;
 
 static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-scan tile size");
 
@@ -44,18 +50,6 @@ struct SrcJointPos
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             v[i] = cdf_weight(v[i]);
-    }
-};
-
-struct SinkCdf
-{
-    __device__ void apply4(const VehDesc& d, const DevScalars*, int p0, int cnt, const fx_t (&pre)[4],
-                           const fx_t (&)[4]) const
-    {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (i < cnt)
-                d.cdf[p0 + i] = pre[i];
     }
 };
 
@@ -266,9 +260,13 @@ __global__ void k_cdff_env(const VehDesc* __restrict__ descs, int m, double* __r
 }
 
 // Ancestor-search traits: exact 128-bit CDF or float-mode envelope (doubles).
+// The exact CDF is not stored: a chunk's entries are recomputed from the 4-byte joint weights (exact
+// block scan plus the tile's prefix from the coarse index), which is cheaper than writing and
+// re-reading 16-byte entries.
 struct CdfExactTr
 {
     using V = fx_t;
+    static constexpr bool kRecompute = true;
     __device__ static bool takes(const VehDesc& d) { return d.cdf_mode == CDF_EXACT; }
     __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars*, int) { return spacing_draw(s_k, sp); }
     __device__ static V incl(const void* ci, int t0, int t) { return static_cast<const fx_t*>(ci)[t0 + t]; }
@@ -278,6 +276,7 @@ struct CdfExactTr
 struct CdfFloatTr
 {
     using V = double;
+    static constexpr bool kRecompute = false;
     __device__ static bool takes(const VehDesc& d) { return d.cdf_mode != CDF_EXACT; }
     __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars* sc, int mode)
     {
@@ -320,7 +319,7 @@ __device__ int find_cdf_tile(const void* __restrict__ ci, int t0, int T, typenam
     return t;
 }
 
-template <class Tr>
+template <class Tr, int RS_SLICE_TILES>
 __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ descs,
                                                     const uint32_t* __restrict__ tab, int pph,
                                                     const fx_t* __restrict__ exp_tile_pre,
@@ -328,6 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
                                                     const DevScalars* __restrict__ scs)
 {
     using V = typename Tr::V;
+    constexpr int RS_SLICE = RS_SLICE_TILES * SCAN_TILE;
     __shared__ V s_cdf[RS_SLICE];
     __shared__ V s_rb[2];
     __shared__ int s_cnt[1];
@@ -382,8 +382,40 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
         const int c0 = tc * SCAN_TILE;
         const int clen = min(J, (tc_end + 1) * SCAN_TILE) - c0;
         __syncthreads();
-        for (int i = threadIdx.x; i < clen; i += BLOCK)
-            s_cdf[i] = Tr::value(d, cdf_incl, t0, c0 + i);
+        if constexpr (Tr::kRecompute)
+        {
+            for (int tt = tc; tt <= tc_end; ++tt)
+            {
+                const int q0 = tt * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+                const int cnt = max(0, min(SCAN_ITEMS, J - q0));
+                float f[SCAN_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (cnt > 0)
+                    SrcJointPos{}.load4(d, q0, cnt, f);
+                fx_t v[SCAN_ITEMS];
+                fx_t sum = 0;
+#pragma unroll
+                for (int i = 0; i < SCAN_ITEMS; ++i)
+                {
+                    v[i] = i < cnt ? fx_from_f32(f[i]) : (fx_t)0;
+                    sum += v[i];
+                }
+                fx_t ttot;
+                fx_t run = block_exclusive_scan_fx(sum, &ttot) + (tt > 0 ? Tr::incl(cdf_incl, t0, tt - 1) : (fx_t)0);
+                fx_t* dst = &s_cdf[(tt - tc) * SCAN_TILE + threadIdx.x * SCAN_ITEMS];
+#pragma unroll
+                for (int i = 0; i < SCAN_ITEMS; ++i)
+                {
+                    run += v[i];
+                    if (i < cnt)
+                        dst[i] = run;
+                }
+            }
+        }
+        else
+        {
+            for (int i = threadIdx.x; i < clen; i += BLOCK)
+                s_cdf[i] = Tr::value(d, cdf_incl, t0, c0 + i);
+        }
         __syncthreads();
         const V cmax = s_cdf[clen - 1];
         int pos = -1;  // lower bound of the previous draw inside this chunk
@@ -488,10 +520,20 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
 void run_resampling(Engine* E, int pph, int modes)
 {
     const bool exact = modes & (1 << CDF_EXACT);
-    if (exact)
-        run_exact_scan(E, SP_J, SrcJointPos{}, SinkCdf{}, SCAN_RESAMPLE, K_CDF_WRITE, ONLY_CDF_EXACT,
-                       E->cdf_tile_incl);
     const int jtiles = E->tiles(SP_J, SCAN_Q);
+    if (exact)
+    {
+        // Tile totals and the coarse index (inclusive tile ends) only; the merge recomputes entries.
+        launch_begin(E, K_SCAN_TILES);
+        hipLaunchKernelGGL(k_scan_tiles<SrcJointPos>, dim3(jtiles), dim3(BLOCK), 0, E->stream, SrcJointPos{},
+                           E->d_desc, E->tab(SP_J), E->scan_tile_sum);
+        launch_end(E, K_SCAN_TILES);
+        launch_begin(E, K_SCAN_TOP);
+        hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_J,
+                           E->scan_tile_sum, E->scan_tile_pre, E->cdf_tile_incl, E->scalars, (int)SCAN_RESAMPLE,
+                           (int)ONLY_CDF_EXACT);
+        launch_end(E, K_SCAN_TOP);
+    }
     double* tsum = reinterpret_cast<double*>(E->scan_tile_sum);
     double* tpre = reinterpret_cast<double*>(E->scan_tile_pre);
     double* env = reinterpret_cast<double*>(E->cdf_tile_incl);
@@ -523,13 +565,21 @@ void run_resampling(Engine* E, int pph, int modes)
     launch_end(E, K_SCAN_TOP);
 
     const int mt = E->tiles(SP_P, RS_TILE / UNIT);
+    const bool wide = mt <= RS_WIDE_MAX_TILES;
     launch_begin(E, K_RS_MERGE);
     if (exact)
-        hipLaunchKernelGGL(k_rs_merge<CdfExactTr>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+    {
+        if (wide)
+            hipLaunchKernelGGL((k_rs_merge<CdfExactTr, RS_SLICE_MAX_TILES>), dim3(mt), dim3(BLOCK), 0, E->stream,
+                               E->d_desc, E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl,
+                               E->scalars);
+        else
+            hipLaunchKernelGGL((k_rs_merge<CdfExactTr, 1>), dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc,
+                               E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+    }
     if (modes & ~(1 << CDF_EXACT))
-        hipLaunchKernelGGL(k_rs_merge<CdfFloatTr>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+        hipLaunchKernelGGL((k_rs_merge<CdfFloatTr, RS_SLICE_MAX_TILES>), dim3(mt), dim3(BLOCK), 0, E->stream,
+                           E->d_desc, E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
     launch_end(E, K_RS_MERGE);
 }
 

@@ -159,6 +159,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
 
 // Stable ranking and scatter of one tile.  Pass 0 ranks the particles' cell indices with identity
 // values; later passes read the previous pass's output.
+// GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
+// sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
+// weight_array; the cell bounds are then recorded by the next segmented reduction.
+template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
                                                           const unsigned* __restrict__ goff)
@@ -263,14 +267,29 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         const int k = s_keys[s];
         const unsigned d = ((unsigned)k >> shift) & (RADIX - 1);
         const unsigned g = goff[(size_t)d * tiles + tile] + ((unsigned)s - s_digoff[d]);
-        keys_out[g] = k;
-        vals_out[g] = s_vals[s];
+        if constexpr (GATHER)
+        {
+            const DevParticles src = dv.part[pph];
+            const DevParticles dst = dv.part[pph ^ 1];
+            const int o = s_vals[s];
+            const float w = src.weight[o];
+            dst.state[g] = src.state[o];
+            dst.weight[g] = w;
+            dst.assoc[g] = src.assoc[o];
+            dst.idx[g] = k;
+            dv.weight_array[g] = w;
+        }
+        else
+        {
+            keys_out[g] = k;
+            vals_out[g] = s_vals[s];
+        }
     }
 }
 
 static_assert(BLOCK == RADIX, "k_sort_downsweep assigns one digit per thread");
 
-void run_sort(Engine* E, int pph, bool hist_done)
+void run_sort(Engine* E, int pph, bool hist_done, bool gather)
 {
     const int tiles = E->tiles(SP_P, SORT_Q);
     const int passes = E->passes;
@@ -296,8 +315,12 @@ void run_sort(Engine* E, int pph, bool hist_done)
                            E->sort_ghist, ps);
         launch_end(E, K_SORT_SCAN);
         launch_begin(E, K_SORT_DOWNSWEEP);
-        hipLaunchKernelGGL(k_sort_downsweep, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, ps,
-                           tiles, E->sort_thist);
+        if (gather && ps == passes - 1)
+            hipLaunchKernelGGL(k_sort_downsweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
+                               pph, ps, tiles, E->sort_thist);
+        else
+            hipLaunchKernelGGL(k_sort_downsweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
+                               E->tab(SP_P), pph, ps, tiles, E->sort_thist);
         launch_end(E, K_SORT_DOWNSWEEP);
     }
 }

@@ -106,6 +106,7 @@ _SIGS = {
     "dogm_device_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dogm_device_free": (_i, [_vp]),
     "dogm_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),
+    "dogm_device_copy": (_i, [_vp, _vp, C.c_size_t]),
     "dogm_device_synchronize": (_i, []),
     "dogm_kernel_count": (_i, []),
     "dogm_kernel_name": (C.c_char_p, [_i]),

@@ -168,6 +168,9 @@ int dogm_set_born_masses(dogm_handle* h, const float* in);
 int dogm_device_malloc(void** ptr, size_t bytes);
 int dogm_device_free(void* ptr);
 int dogm_memcpy(void* dst, const void* src, size_t bytes, int kind);
+/* Device-to-device copy in one kernel dispatch (16-byte aligned pointers); blocking.  Also the
+ * reference transfer of known size for scaling memory-traffic counters. */
+int dogm_device_copy(void* dst, const void* src, size_t bytes);
 int dogm_device_synchronize(void);
 
 /* ----------------------------------------------------------------------------------------------
