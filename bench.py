@@ -65,6 +65,25 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
     return table.get(name)
 
 
+# Profiled kernel names (rocprofv3) of the per-kernel timers.
+PROFILED_NAME = {
+    "predict": "k_predict", "sort_upsweep": "k_sort_upsweep", "sort_downsweep": "k_sort_downsweep",
+    "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>", "seg_moments": "k_segreduce<MomOp>",
+    "mass_update": "k_mass_update", "rs_merge": "k_rs_merge", "birth_cells": "k_scan_apply<SrcBorn,SinkBirth>",
+}
+
+
+def measured_traffic(kernel: str, workload: str, vehicles: int) -> tuple[int | None, str | None]:
+    """HBM bytes per launch of `kernel` from the committed counter summary of this workload and
+    batch size (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), if one exists."""
+    import profiling
+    path = REPO / "profiles" / f"hbm_traffic_{workload}_batch{vehicles}.md"
+    name = PROFILED_NAME.get(kernel)
+    if name is None or not path.exists():
+        return None, None
+    return profiling.lookup_traffic(str(path), name), str(path.relative_to(REPO))
+
+
 def frame_bytes(N: int, B: int, C: int, shifted: bool) -> int:
     """Stage-boundary algorithmic bytes of one frame: 172 N + 29 B + 128 C (+16 N + 8 C with a
     grid shift)."""
@@ -508,6 +527,7 @@ def main() -> int:
 
     shifted = args.workload == "plain_full"
     fb = frame_bytes(N, B, C, shifted)
+    traffic, traffic_src = measured_traffic(dominant, args.workload, kveh)
     cpu = None
     if args.cpu_frames > 0:
         cpu = cpu_baseline(args.workload, min(args.cpu_frames, total), meas, poses, dt)
@@ -549,7 +569,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_launch_us": round(dom_avg_s * 1e6, 3),
             "algorithmic_bytes_per_launch": dbytes,
         },
