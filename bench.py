@@ -193,9 +193,72 @@ def run_batched(args, k, p, meas_bufs, poses, dt):
     lib.dogm_device_synchronize()
     elapsed = time.perf_counter() - t_start
     dom_ms, dom_n = b.kernel_timing(dominant)
+    b.set_kernel_timing(dominant, False)
+    extraction = run_extraction(args, b, ids, p, step, total) if args.extract_steps > 0 else None
     b.close()
     return {"elapsed": elapsed, "frames": args.steps * k, "lat": lat, "per_frame": per_step, "dominant": dominant,
-            "dom_avg_s": dom_ms / max(1, dom_n) / 1e3, "vehicles": k}
+            "dom_avg_s": dom_ms / max(1, dom_n) / 1e3, "vehicles": k, "extraction": extraction}
+
+
+EXTRACT_KERNELS = ("extract_mark", "extract_scan", "extract_compact")
+
+
+def run_extraction(args, b, ids, p, step, first_frame):
+    """Result extraction inside the batched steps (after the timed frames): the demo's per-frame
+    post-processing -- 8-bit occupancy image and the cells passing the velocity test at the demo's
+    thresholds 0.7 / 4.0 (image_creation.cpp:19-128, main.cpp:51-52) -- for every vehicle of the step.
+    Reports each extraction kernel's average launch time against its algorithmic bytes, and the
+    read-back size of the compact results against a full getGridCells."""
+    from dogm_ctypes import EXTRACT_IMAGE, EXTRACT_VELOCITY, lib
+    min_occ, min_vel = 0.7, 4.0
+    b.set_extraction(EXTRACT_IMAGE | EXTRACT_VELOCITY, min_occ, min_vel)
+    step(first_frame - 1)  # allocates the extraction buffers
+    for n in EXTRACT_KERNELS:
+        b.set_kernel_timing(n, True)
+    lib.dogm_device_synchronize()
+    t0 = time.perf_counter()
+    for f in range(args.extract_steps):
+        step(first_frame - 1 - (f % 2))
+    lib.dogm_device_synchronize()
+    ms_step = (time.perf_counter() - t0) / args.extract_steps * 1e3
+    timing = {n: b.kernel_timing(n) for n in EXTRACT_KERNELS}
+    for n in EXTRACT_KERNELS:
+        b.set_kernel_timing(n, False)
+    gs = int(np.float32(p.size) / np.float32(p.resolution))
+    C = gs * gs
+    units = -(-C // 256)
+    units += (-units) % 8
+    counts = []
+    occupied = 0
+    for vid in ids:
+        _, cnt = b.get_cells_with_velocity(vid, capacity=0)
+        counts.append(cnt)
+        occupied += int(np.count_nonzero(b.get_occupancy(vid) >= np.float32(min_occ)))
+    kv = len(ids)
+    passing = sum(counts)
+    # mark: occ + free of every cell, the five moment fields of cells at or above the occupancy
+    # threshold; writes the image byte, one 64-bit ballot per 64 cells and one count per unit.
+    # scan: counts in, offsets out.  compact: counts, offsets and ballots of every unit, 4 bytes per
+    # passing cell.
+    algo = {
+        "extract_mark": kv * (8 * C + 1 * C + 8 * (units * 4) + 4 * units) + 20 * occupied,
+        "extract_scan": kv * (4 * units + 4 * (units + 1)),
+        "extract_compact": kv * (4 * units + 4 * units + 8 * units * 4) + 4 * passing,
+    }
+    kernels = {}
+    for n in EXTRACT_KERNELS:
+        ms, launches = timing[n]
+        avg_s = ms / max(1, launches) / 1e3
+        gbs = algo[n] / avg_s / 1e9 if avg_s > 0 else None
+        kernels[n] = {"avg_launch_us": round(avg_s * 1e6, 3), "launches": launches, "algorithmic_bytes": algo[n],
+                      "achieved_GBs": round(gbs, 1) if gbs else None,
+                      "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    readback = C + 72 * (passing / kv)
+    return {"what": f"8-bit occupancy image + cells with v^T S^-1 v >= {min_vel} at occupancy >= {min_occ}",
+            "ms_per_step_with_extraction": round(ms_step, 4), "velocity_cells_per_vehicle": counts,
+            "cells_at_or_above_occupancy": occupied, "kernels": kernels,
+            "readback_bytes_per_vehicle": int(readback), "full_grid_readback_bytes": 64 * C,
+            "readback_ratio": round(64 * C / readback, 1)}
 
 
 def run_serving(args) -> int:
@@ -453,6 +516,8 @@ def main() -> int:
     ap.add_argument("--sweep-seconds", type=float, default=10.0, help="simulated seconds per sweep point")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device-memory admission budget")
     ap.add_argument("--max-batch", type=int, default=0, help="frames per serving step (0: no limit)")
+    ap.add_argument("--extract-steps", type=int, default=5,
+                    help="batched steps with result extraction after the timed frames (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the frames (scales HBM counters)")
     args = ap.parse_args()
@@ -559,6 +624,7 @@ def main() -> int:
         "plain": _summary(plain),
         "batched": _summary(batched) if batched else None,
         "batched_vs_plain": round(fps / (plain["frames"] / plain["elapsed"]), 3) if batched else None,
+        "extraction": batched.get("extraction") if batched else None,
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),

@@ -19,6 +19,7 @@ struct dogm_batch
     std::map<int, std::unique_ptr<Vehicle>> veh;
     int next_id = 1;
     uint64_t bytes = 0;
+    ExtractCfg ext;  // extraction run inside every step (what = 0: none)
 };
 
 namespace
@@ -194,6 +195,62 @@ Vehicle* find_vehicle(dogm_batch* b, int id)
 }
 
 }  // namespace
+
+// ---- result extraction getters -------------------------------------------------------------------
+namespace dogm_hip
+{
+
+int extract_ensure(Engine* E, Vehicle* v, const ExtractCfg& want, bool reuse)
+{
+    const bool vel = (want.what & DOGM_EXTRACT_VELOCITY) != 0;
+    if (reuse && (v->ext_what & want.what) == want.what && v->ext_source == want.source &&
+        (!vel || (v->ext_min_occ == want.min_occ && v->ext_min_vel == want.min_vel)))
+        return DOGM_OK;
+    E->ext = want;
+    return run_one(E, v, ST_EXTRACT, "extraction");
+}
+
+int extract_read_occupancy(Engine* E, Vehicle* v, float* out)
+{
+    return copy_out(E, out, v->x_occ, sizeof(float) * (size_t)v->C);
+}
+
+int extract_read_image(Engine* E, Vehicle* v, uint8_t* out)
+{
+    return copy_out(E, out, v->x_img, (size_t)v->C);
+}
+
+int extract_read_velocity(Engine* E, Vehicle* v, dogm_velocity_cell* out, int capacity, int* count)
+{
+    if (!count)
+        return fail(E, DOGM_ERR_INVALID_ARGUMENT, "count is null");
+    if (capacity < 0 || (capacity > 0 && !out))
+        return fail(E, DOGM_ERR_INVALID_ARGUMENT, "capacity must be >= 0, with an output buffer when > 0");
+    int total = 0;
+    int st = copy_out(E, &total, v->x_off + units_for(v->C), sizeof(int));
+    if (st)
+        return st;
+    if (total < 0 || total > v->C)
+        return fail(E, DOGM_ERR_DEVICE_CHECK, "velocity cell count out of range");
+    *count = total;
+    const int n = std::min(total, capacity);
+    for (int r0 = 0; r0 < n; r0 += (int)v->x_rec_cap)
+    {
+        const int k = std::min(n - r0, (int)v->x_rec_cap);
+        Vehicle chunk = {};
+        chunk.grid = v->grid;
+        chunk.gs = v->gs;
+        chunk.x_idx = v->x_idx + r0;
+        chunk.x_rec = v->x_rec;
+        run_extract_gather(E, &chunk, k);
+        if ((st = check_launch(E, "cells with velocity")) ||
+            (st = copy_out(E, out + r0, v->x_rec, sizeof(dogm_velocity_cell) * (size_t)k)))
+            return st;
+    }
+    return DOGM_OK;
+}
+
+}  // namespace dogm_hip
 
 // Single-dispatch device-to-device copy (16-byte vectors, grid-stride), on the null stream.
 __global__ __launch_bounds__(BLOCK) void k_copy_bytes(const uint4* __restrict__ src, uint4* __restrict__ dst,
@@ -406,6 +463,29 @@ int dogm_set_grid_cells(dogm_handle* h, const dogm_grid_cell* cells)
         return st;
     run_unpack_grid(&h->eng, &h->veh);
     return finish(&h->eng, "setGridCells");
+}
+
+int dogm_get_occupancy(dogm_handle* h, int source, float* out)
+{
+    NEED_H(h);
+    int st = extract_ensure(&h->eng, &h->veh, ExtractCfg{DOGM_EXTRACT_OCCUPANCY, source, 0.0f, 0.0f}, false);
+    return st ? st : extract_read_occupancy(&h->eng, &h->veh, out);
+}
+
+int dogm_get_occupancy_image(dogm_handle* h, int source, uint8_t* out)
+{
+    NEED_H(h);
+    int st = extract_ensure(&h->eng, &h->veh, ExtractCfg{DOGM_EXTRACT_IMAGE, source, 0.0f, 0.0f}, false);
+    return st ? st : extract_read_image(&h->eng, &h->veh, out);
+}
+
+int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_velocity, dogm_velocity_cell* out,
+                                 int capacity, int* count)
+{
+    NEED_H(h);
+    int st = extract_ensure(&h->eng, &h->veh,
+                            ExtractCfg{DOGM_EXTRACT_VELOCITY, DOGM_SOURCE_GRID, min_occupancy, min_velocity}, false);
+    return st ? st : extract_read_velocity(&h->eng, &h->veh, out, capacity, count);
 }
 
 int dogm_get_weight_array(dogm_handle* h, float* out)
@@ -633,7 +713,12 @@ int dogm_batch_vehicle_count(const dogm_batch* b)
 
 uint64_t dogm_batch_device_bytes(const dogm_batch* b)
 {
-    return b ? b->bytes : 0;
+    if (!b)
+        return 0;
+    uint64_t x = 0;
+    for (const auto& kv : b->veh)
+        x += kv.second->x_bytes;
+    return b->bytes + x;
 }
 
 int dogm_batch_measurement_buffer(dogm_batch* b, int vehicle_id, dogm_measurement_cell** device_ptr)
@@ -672,7 +757,8 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
     std::vector<FrameIn> in((size_t)count);
     for (int i = 0; i < count; ++i)
         in[i] = FrameIn{meas ? meas[i] : nullptr, is_device, x[i], y[i], yaw[i], dt[i]};
-    return engine_frame(E, list.data(), in.data(), count);
+    E->ext = b->ext;
+    return engine_frame(E, list.data(), in.data(), count, b->ext.what != 0);
 }
 
 int dogm_batch_get_grid_cells(dogm_batch* b, int vehicle_id, dogm_grid_cell* out)
@@ -717,6 +803,58 @@ int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     v->cdf_mode = mode;
     return DOGM_OK;
+}
+
+int dogm_batch_set_extraction(dogm_batch* b, int what, float min_occupancy, float min_velocity)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    if (what & ~(DOGM_EXTRACT_OCCUPANCY | DOGM_EXTRACT_IMAGE | DOGM_EXTRACT_VELOCITY))
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown extraction flags");
+    b->ext = ExtractCfg{what, DOGM_SOURCE_GRID, min_occupancy, min_velocity};
+    return DOGM_OK;
+}
+
+}  // extern "C"
+
+namespace
+{
+
+int batch_extract(dogm_batch* b, int vehicle_id, int what, Vehicle** out)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = find_vehicle(b, vehicle_id);
+    if (!v)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
+    *out = v;
+    return extract_ensure(&b->eng, v, ExtractCfg{what, DOGM_SOURCE_GRID, b->ext.min_occ, b->ext.min_vel}, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dogm_batch_get_occupancy(dogm_batch* b, int vehicle_id, float* out)
+{
+    Vehicle* v = nullptr;
+    const int st = batch_extract(b, vehicle_id, DOGM_EXTRACT_OCCUPANCY, &v);
+    return st ? st : extract_read_occupancy(&b->eng, v, out);
+}
+
+int dogm_batch_get_occupancy_image(dogm_batch* b, int vehicle_id, uint8_t* out)
+{
+    Vehicle* v = nullptr;
+    const int st = batch_extract(b, vehicle_id, DOGM_EXTRACT_IMAGE, &v);
+    return st ? st : extract_read_image(&b->eng, v, out);
+}
+
+int dogm_batch_get_cells_with_velocity(dogm_batch* b, int vehicle_id, dogm_velocity_cell* out, int capacity,
+                                       int* count)
+{
+    Vehicle* v = nullptr;
+    const int st = batch_extract(b, vehicle_id, DOGM_EXTRACT_VELOCITY, &v);
+    return st ? st : extract_read_velocity(&b->eng, v, out, capacity, count);
 }
 
 int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable)

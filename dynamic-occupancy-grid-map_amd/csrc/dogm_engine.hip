@@ -17,7 +17,8 @@ const char* const kKernelNames[K_COUNT] = {
     "post_weights",   "empty_cells",     "scan_tiles",      "scan_top",        "birth_cells",
     "birth_particles", "init_cells",     "init_particles",  "copy_masses",     "seg_moments",
     "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
-    "unpack_grid",    "build_tables",
+    "unpack_grid",    "build_tables",    "extract_mark",    "extract_scan",    "extract_compact",
+    "extract_gather",
 };
 
 // ---- timing ------------------------------------------------------------------------------------
@@ -75,6 +76,12 @@ void engine_collect_timing(Engine* E)
 
 // ---- allocation helpers ------------------------------------------------------------------------
 
+long long units_for(long long len)
+{
+    const long long u = (len + UNIT - 1) / UNIT;
+    return (u + UNIT_ALIGN - 1) / UNIT_ALIGN * UNIT_ALIGN;
+}
+
 namespace
 {
 
@@ -121,12 +128,6 @@ int bits_for(long long x)  // number of bits needed to represent x >= 0
     while (b < 62 && (1ll << b) <= x)
         ++b;
     return b;
-}
-
-long long units_for(long long len)
-{
-    const long long u = (len + UNIT - 1) / UNIT;
-    return (u + UNIT_ALIGN - 1) / UNIT_ALIGN * UNIT_ALIGN;
 }
 
 long long space_len(const Vehicle* v, int s)
@@ -383,6 +384,12 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.sort_vals[0] = v->sort_vals[0];
         d.sort_vals[1] = v->sort_vals[1];
         d.cdf = v->cdf;
+        d.x_occ = v->x_occ;
+        d.x_img = v->x_img;
+        d.x_mask = v->x_mask;
+        d.x_cnt = v->x_cnt;
+        d.x_off = v->x_off;
+        d.x_idx = v->x_idx;
     }
     // Table offsets per vehicle (units); the prefixes are those the tables were built from.
     {
@@ -403,6 +410,23 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
 
 int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
 {
+    if (stages & ST_EXTRACT)
+    {
+        const ExtractCfg& x = E->ext;
+        if ((x.what & ~(DOGM_EXTRACT_OCCUPANCY | DOGM_EXTRACT_IMAGE | DOGM_EXTRACT_VELOCITY)) || x.what == 0)
+            return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT, "unknown extraction flags");
+        if (x.source != DOGM_SOURCE_GRID && x.source != DOGM_SOURCE_MEASUREMENT)
+            return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT, "unknown extraction source");
+        if ((x.what & DOGM_EXTRACT_VELOCITY) && x.source != DOGM_SOURCE_GRID)
+            return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT, "the velocity test reads grid cells only");
+        for (int i = 0; i < m; ++i)
+        {
+            std::string why;
+            const int st = vehicle_ext_alloc(veh[i], x.what, &why);
+            if (st != DOGM_OK)
+                return set_err(&E->last_error, st, why);
+        }
+    }
     int st = engine_plan(E, veh, m, stages);
     if (st != DOGM_OK)
         return st;
@@ -440,6 +464,8 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         run_resampling(E, pph, modes);
         pph ^= 1;
     }
+    if (stages & ST_EXTRACT)
+        run_extract(E, (stages & ST_OCC) ? 1 : 0);
     // Host state: the descriptors already hold this step's buffers, so swaps take effect for the
     // next step.
     for (int i = 0; i < m; ++i)
@@ -460,6 +486,17 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
             v->pend_part_shift = false;
         if (stages & ST_OCC)
             v->pend_grid_shift = false;
+        if (stages & ST_EXTRACT)
+        {
+            v->ext_what = E->ext.what;
+            v->ext_source = E->ext.source;
+            v->ext_min_occ = E->ext.min_occ;
+            v->ext_min_vel = E->ext.min_vel;
+        }
+        else if (stages & ST_ALL)
+        {
+            v->ext_what = 0;
+        }
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -467,7 +504,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     return DOGM_OK;
 }
 
-int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m)
+int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
 {
     for (int i = 0; i < m; ++i)
     {
@@ -489,7 +526,7 @@ int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m)
         vehicle_update_pose(E, v, in[i].x, in[i].y, in[i].yaw, true);
         v->dt = in[i].dt;
     }
-    int st = engine_run(E, veh, m, ST_ALL);
+    int st = engine_run(E, veh, m, ST_ALL | (extract ? ST_EXTRACT : 0));
     const hipError_t e = hipStreamSynchronize(E->stream);
     if (st != DOGM_OK)
         return st;
@@ -669,10 +706,56 @@ void vehicle_free(Vehicle* v)
         (void)hipFree(v->grid.f[f]);
     void* bufs[] = {v->free_alt, v->field_tmp,    v->meas,         v->weight_array, v->born,
                     v->m_raw,    v->pend,         v->ancestors,    v->sort_keys[0], v->sort_keys[1],
-                    v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp};
+                    v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp,
+                    v->x_occ,    v->x_img,        v->x_mask,       v->x_cnt,        v->x_off,
+                    v->x_idx,    v->x_rec};
     for (void* b : bufs)
         (void)hipFree(b);
     *v = Vehicle{};
+}
+
+size_t extract_footprint(int C, int what)
+{
+    const size_t units = (size_t)units_for(C);
+    size_t b = 0;
+    if (what & DOGM_EXTRACT_OCCUPANCY)
+        b += sizeof(float) * (size_t)C;
+    if (what & DOGM_EXTRACT_IMAGE)
+        b += (size_t)C;
+    if (what & DOGM_EXTRACT_VELOCITY)
+        b += units * (UNIT / WAVE * 8 + 8) + 4 + sizeof(int) * (size_t)C +
+             sizeof(dogm_velocity_cell) * (size_t)std::min(C, EXTRACT_REC_CHUNK);
+    return b;
+}
+
+int vehicle_ext_alloc(Vehicle* v, int what, std::string* why)
+{
+    const long long units = units_for(v->C);
+    int st;
+    if ((what & DOGM_EXTRACT_OCCUPANCY) && !v->x_occ)
+    {
+        if ((st = dalloc(&v->x_occ, (size_t)v->C, "occupancy output", why)))
+            return st;
+        v->x_bytes += extract_footprint(v->C, DOGM_EXTRACT_OCCUPANCY);
+    }
+    if ((what & DOGM_EXTRACT_IMAGE) && !v->x_img)
+    {
+        if ((st = dalloc(&v->x_img, (size_t)v->C, "occupancy image", why)))
+            return st;
+        v->x_bytes += extract_footprint(v->C, DOGM_EXTRACT_IMAGE);
+    }
+    if ((what & DOGM_EXTRACT_VELOCITY) && !v->x_idx)
+    {
+        if ((st = dalloc(&v->x_mask, (size_t)units * (UNIT / WAVE), "velocity ballots", why)) ||
+            (st = dalloc(&v->x_cnt, (size_t)units, "velocity counts", why)) ||
+            (st = dalloc(&v->x_off, (size_t)units + 1, "velocity offsets", why)) ||
+            (st = dalloc(&v->x_idx, (size_t)v->C, "velocity cells", why)) ||
+            (st = dalloc(&v->x_rec, (size_t)std::min(v->C, EXTRACT_REC_CHUNK), "velocity records", why)))
+            return st;
+        v->x_rec_cap = std::min(v->C, EXTRACT_REC_CHUNK);
+        v->x_bytes += extract_footprint(v->C, DOGM_EXTRACT_VELOCITY);
+    }
+    return DOGM_OK;
 }
 
 }  // namespace dogm_hip

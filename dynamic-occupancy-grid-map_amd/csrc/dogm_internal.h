@@ -119,6 +119,13 @@ struct VehDesc
     int* sort_keys[2];
     int* sort_vals[2];
     fx_t* cdf;
+    // result extraction (allocated on first use)
+    float* x_occ;                 // [C] pignistic occupancy
+    uint8_t* x_img;               // [C] 8-bit occupancy image
+    unsigned long long* x_mask;   // [4 per unit] velocity-test ballots of the unit's waves
+    int* x_cnt;                   // [units] passing cells per unit
+    int* x_off;                   // [units + 1] their exclusive prefix; x_off[units] = total
+    int* x_idx;                   // [C] passing cells, row-major
 };
 
 // Scalars produced and consumed on the device within one step, one per vehicle.
@@ -181,6 +188,10 @@ enum KernelId
     K_PACK_GRID,
     K_UNPACK_GRID,
     K_BUILD_TABLES,
+    K_EXTRACT_MARK,
+    K_EXTRACT_SCAN,
+    K_EXTRACT_COMPACT,
+    K_EXTRACT_GATHER,
     K_COUNT
 };
 
@@ -226,6 +237,21 @@ struct Vehicle
     dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes (lazy)
     size_t bytes = 0;                   // device bytes held
 
+    // Result extraction buffers (allocated on first use, not part of the footprint) and what the
+    // vehicle's last step extracted (ext_what = 0 once a later step changed the state).
+    float* x_occ = nullptr;
+    uint8_t* x_img = nullptr;
+    unsigned long long* x_mask = nullptr;
+    int* x_cnt = nullptr;
+    int* x_off = nullptr;
+    int* x_idx = nullptr;
+    dogm_velocity_cell* x_rec = nullptr;
+    long long x_rec_cap = 0;
+    size_t x_bytes = 0;
+    int ext_what = 0;
+    int ext_source = 0;
+    float ext_min_occ = 0.0f, ext_min_vel = 0.0f;
+
     // pose (dogm/src/dogm.cu:161-203)
     bool first_pose = false;
     bool first_meas = false;
@@ -241,7 +267,8 @@ struct Vehicle
 // Stage selection of one step.
 enum StageBits
 {
-    ST_ALL = 0xff,
+    ST_ALL = 0xff,         // the eight stages of updateGrid
+    ST_EXTRACT = 1 << 8,   // result extraction after the stages (Engine::ext), not part of ST_ALL
     ST_INIT = 1 << 0,
     ST_PREDICT = 1 << 1,
     ST_ASSIGN = 1 << 2,
@@ -250,6 +277,14 @@ enum StageBits
     ST_BIRTH = 1 << 5,
     ST_MOMENTS = 1 << 6,
     ST_RESAMPLE = 1 << 7,
+};
+
+// What a step with ST_EXTRACT extracts (dogm/demo/utils/image_creation.cpp:19-66).
+struct ExtractCfg
+{
+    int what = 0;    // DOGM_EXTRACT_* bits
+    int source = 0;  // DOGM_SOURCE_GRID / DOGM_SOURCE_MEASUREMENT
+    float min_occ = 0.0f, min_vel = 0.0f;
 };
 
 // Execution context on one device and stream: step descriptors, work tables and the shared
@@ -294,6 +329,8 @@ struct Engine
     std::vector<TimedLaunch> timed_pending;
     std::vector<hipEvent_t> event_pool;
 
+    ExtractCfg ext;  // read by steps with ST_EXTRACT
+
     std::string last_error;
 
     const uint32_t* tab(int s) const { return d_tab + tab_off[s]; }
@@ -310,6 +347,7 @@ inline int div_up(long long a, long long b)
 }
 
 // ---- engine (dogm_engine.hip) ----
+long long units_for(long long len);  // table units of a space of `len` elements (padded)
 int engine_init(Engine* E, std::string* why);
 void engine_release(Engine* E);
 // Builds descriptors and tables for `veh` (reusing the tables while the vehicle list is
@@ -337,8 +375,23 @@ struct FrameIn
     int is_device;
     float x, y, yaw, dt;
 };
-// Measurement copies, lazy pose updates, the full stage sequence, and a wait for the stream.
-int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m);
+// Measurement copies, lazy pose updates, the full stage sequence (plus ST_EXTRACT with
+// E->ext when `extract`), and a wait for the stream.
+int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract = false);
+// Allocates the extraction buffers `what` needs on a vehicle.
+int vehicle_ext_alloc(Vehicle* v, int what, std::string* why);
+// Velocity records are packed and read back in chunks of at most this many.
+constexpr int EXTRACT_REC_CHUNK = 1 << 16;
+// Device bytes the extraction buffers for `what` take on a vehicle of C cells (record staging included).
+size_t extract_footprint(int C, int what);
+
+// ---- result getters shared by the plain, batch and server entry points (dogm_capi.hip) ----
+// Runs the extraction `want` on v and waits, unless `reuse` and v's last step already extracted it
+// (same source and, for the velocity test, the same thresholds).
+int extract_ensure(Engine* E, Vehicle* v, const ExtractCfg& want, bool reuse);
+int extract_read_occupancy(Engine* E, Vehicle* v, float* out);
+int extract_read_image(Engine* E, Vehicle* v, uint8_t* out);
+int extract_read_velocity(Engine* E, Vehicle* v, dogm_velocity_cell* out, int capacity, int* count);
 
 // ---- stage launchers (dogm_kernels.hip, dogm_sort.hip, dogm_resample.hip) ----
 // `pph` is the index (0 / 1) of the current particle buffer of the step's descriptors.
@@ -365,6 +418,11 @@ void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);
 void run_pack_grid(Engine* E, const Vehicle* v);
 void run_unpack_grid(Engine* E, const Vehicle* v);
+// Result extraction over the step's vehicles (dogm_extract.hip); fsel selects the current free mass
+// buffer of the step's descriptors (1 after an occupancy update in the same step).
+void run_extract(Engine* E, int fsel);
+// Packs records of the first n cells of v->x_idx into v->x_rec.
+void run_extract_gather(Engine* E, const Vehicle* v, int n);
 
 }  // namespace dogm_hip
 

@@ -33,6 +33,8 @@ struct dogm_server
     std::condition_variable cv_work, cv_done;
     std::map<int, std::unique_ptr<Vehicle>> veh;
     std::map<int, int> inflight;  // queued + running frames per vehicle
+    std::map<int, uint64_t> reserved;  // budget bytes held per vehicle
+    ExtractCfg ext;                    // extraction inside every step (what = 0: none)
     uint64_t budget = 0, resident = 0;
     int max_batch = 0;
     int next_id = 1;
@@ -105,7 +107,8 @@ int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* 
     }
     std::lock_guard<std::mutex> ge(s->eng_mu);
     const auto a = std::chrono::steady_clock::now();
-    const int st = engine_frame(&s->eng, list.data(), in.data(), (int)fr.size());
+    s->eng.ext = s->ext;
+    const int st = engine_frame(&s->eng, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
     *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     if (st != DOGM_OK)
     {
@@ -218,9 +221,16 @@ int dogm_server_admit(dogm_server* s, const dogm_params* params, uint64_t seed, 
     int st = vehicle_validate(params, &why);
     if (st != DOGM_OK)
         return sfail(s, st, why);
-    const uint64_t fp = (uint64_t)vehicle_footprint(params);
+    uint64_t fp = (uint64_t)vehicle_footprint(params);
+    int ext_what = 0;
     {
         std::lock_guard<std::mutex> g(s->mu);
+        ext_what = s->ext.what;
+        if (ext_what)
+        {
+            const int gs = (int)(params->size / params->resolution);
+            fp += (uint64_t)extract_footprint(gs * gs, ext_what);
+        }
         if (s->resident + fp > s->budget)
         {
             s->last_error = "admission would exceed the device-memory budget (" + std::to_string(s->resident) +
@@ -234,6 +244,8 @@ int dogm_server_admit(dogm_server* s, const dogm_params* params, uint64_t seed, 
         st = DOGM_ERR_OUT_OF_MEMORY;
     else
         st = vehicle_alloc(v.get(), params, seed, instance, &why);
+    if (st == DOGM_OK && ext_what)
+        st = vehicle_ext_alloc(v.get(), ext_what, &why);
     std::lock_guard<std::mutex> g(s->mu);
     if (st != DOGM_OK)
     {
@@ -245,6 +257,7 @@ int dogm_server_admit(dogm_server* s, const dogm_params* params, uint64_t seed, 
     }
     const int id = s->next_id++;
     s->veh[id] = std::move(v);
+    s->reserved[id] = fp;
     *vehicle_id = id;
     return DOGM_OK;
 }
@@ -273,7 +286,8 @@ int dogm_server_release(dogm_server* s, int vehicle_id)
         }
         v = std::move(it->second);
         s->veh.erase(it);
-        s->resident -= (uint64_t)vehicle_footprint(&v->params);
+        s->resident -= s->reserved[vehicle_id];
+        s->reserved.erase(vehicle_id);
     }
     std::lock_guard<std::mutex> ge(s->eng_mu);
     vehicle_free(v.get());
@@ -465,6 +479,80 @@ int dogm_server_get_particles(dogm_server* s, int vehicle_id, dogm_vec4* state, 
     if (hipStreamSynchronize(E->stream) != hipSuccess)
         return sfail(s, DOGM_ERR_HIP, "reading back particles");
     return DOGM_OK;
+}
+
+int dogm_server_set_extraction(dogm_server* s, int what, float min_occupancy, float min_velocity)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    if (what & ~(DOGM_EXTRACT_OCCUPANCY | DOGM_EXTRACT_IMAGE | DOGM_EXTRACT_VELOCITY))
+        return sfail(s, DOGM_ERR_INVALID_ARGUMENT, "unknown extraction flags");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->veh.empty())
+    {
+        s->last_error = "set the extraction before admitting vehicles (its buffers count against the budget)";
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    s->ext = ExtractCfg{what, DOGM_SOURCE_GRID, min_occupancy, min_velocity};
+    return DOGM_OK;
+}
+
+}  // extern "C"
+
+namespace
+{
+
+// Runs `read` on the vehicle's extraction results for `what` under the engine lock.
+template <typename Read>
+int server_extract(dogm_server* s, int vehicle_id, int what, Read read)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = nullptr;
+    ExtractCfg want;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->veh.find(vehicle_id);
+        if (it == s->veh.end())
+        {
+            s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        v = it->second.get();
+        want = ExtractCfg{what, DOGM_SOURCE_GRID, s->ext.min_occ, s->ext.min_vel};
+    }
+    std::lock_guard<std::mutex> ge(s->eng_mu);
+    Engine* E = &s->eng;
+    int st = extract_ensure(E, v, want, true);
+    if (st == DOGM_OK)
+        st = read(E, v);
+    if (st != DOGM_OK)
+        return sfail(s, st, E->last_error);
+    return DOGM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dogm_server_get_occupancy(dogm_server* s, int vehicle_id, float* out)
+{
+    return server_extract(s, vehicle_id, DOGM_EXTRACT_OCCUPANCY,
+                          [&](Engine* E, Vehicle* v) { return extract_read_occupancy(E, v, out); });
+}
+
+int dogm_server_get_occupancy_image(dogm_server* s, int vehicle_id, uint8_t* out)
+{
+    return server_extract(s, vehicle_id, DOGM_EXTRACT_IMAGE,
+                          [&](Engine* E, Vehicle* v) { return extract_read_image(E, v, out); });
+}
+
+int dogm_server_get_cells_with_velocity(dogm_server* s, int vehicle_id, dogm_velocity_cell* out, int capacity,
+                                        int* count)
+{
+    return server_extract(s, vehicle_id, DOGM_EXTRACT_VELOCITY, [&](Engine* E, Vehicle* v) {
+        return extract_read_velocity(E, v, out, capacity, count);
+    });
 }
 
 int dogm_server_stats(const dogm_server* s, long long* steps, long long* frames)

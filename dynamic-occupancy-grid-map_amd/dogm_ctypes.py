@@ -60,7 +60,11 @@ GRID_CELL_DTYPE = np.dtype([
     ("var_y_vel", "<f4"), ("covar_xy_vel", "<f4"),
 ])
 MEAS_CELL_DTYPE = np.dtype([("free_mass", "<f4"), ("occ_mass", "<f4"), ("likelihood", "<f4"), ("p_A", "<f4")])
-assert GRID_CELL_DTYPE.itemsize == 64 and MEAS_CELL_DTYPE.itemsize == 16
+VELOCITY_CELL_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("data", GRID_CELL_DTYPE)])
+assert GRID_CELL_DTYPE.itemsize == 64 and MEAS_CELL_DTYPE.itemsize == 16 and VELOCITY_CELL_DTYPE.itemsize == 72
+
+SOURCES = {"grid": 0, "measurement": 1}
+EXTRACT_OCCUPANCY, EXTRACT_IMAGE, EXTRACT_VELOCITY = 1, 2, 4
 
 _vp = C.c_void_p
 _h = C.c_void_p
@@ -103,6 +107,9 @@ _SIGS = {
     "dogm_set_grid_cells": (_i, [_h, _vp]),
     "dogm_set_weight_array": (_i, [_h, _vp]),
     "dogm_set_born_masses": (_i, [_h, _vp]),
+    "dogm_get_occupancy": (_i, [_h, _i, _vp]),
+    "dogm_get_occupancy_image": (_i, [_h, _i, _vp]),
+    "dogm_get_cells_with_velocity": (_i, [_h, _f, _f, _vp, _i, C.POINTER(_i)]),
     "dogm_device_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dogm_device_free": (_i, [_vp]),
     "dogm_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),
@@ -130,6 +137,10 @@ _SIGS = {
     "dogm_batch_get_grid_cells": (_i, [_h, _i, _vp]),
     "dogm_batch_get_particles": (_i, [_h, _i, _vp, _vp, _vp, _vp]),
     "dogm_batch_get_resampled_indices": (_i, [_h, _i, _vp]),
+    "dogm_batch_set_extraction": (_i, [_h, _i, _f, _f]),
+    "dogm_batch_get_occupancy": (_i, [_h, _i, _vp]),
+    "dogm_batch_get_occupancy_image": (_i, [_h, _i, _vp]),
+    "dogm_batch_get_cells_with_velocity": (_i, [_h, _i, _vp, _i, C.POINTER(_i)]),
     "dogm_batch_set_kernel_timing": (_i, [_h, _i, _i]),
     "dogm_batch_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "dogm_batch_reset_kernel_timing": (_i, [_h]),
@@ -150,6 +161,10 @@ _SIGS = {
     "dogm_server_get_grid_cells": (_i, [_h, _i, _vp]),
     "dogm_server_get_particles": (_i, [_h, _i, _vp, _vp, _vp, _vp]),
     "dogm_server_stats": (_i, [_h, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "dogm_server_set_extraction": (_i, [_h, _i, _f, _f]),
+    "dogm_server_get_occupancy": (_i, [_h, _i, _vp]),
+    "dogm_server_get_occupancy_image": (_i, [_h, _i, _vp]),
+    "dogm_server_get_cells_with_velocity": (_i, [_h, _i, _vp, _i, C.POINTER(_i)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -304,6 +319,23 @@ class DOGM:
         out = np.empty(self.grid_cell_count, dtype=MEAS_CELL_DTYPE)
         self._check(lib.dogm_get_measurement_cells(self.h, _ptr(out)), "getMeasurementCells")
         return out
+
+    # ---- result extraction (image_creation.cpp:19-128, on the device) ---------------------------
+    def get_occupancy(self, source: str = "grid") -> np.ndarray:
+        out = np.empty(self.grid_cell_count, np.float32)
+        self._check(lib.dogm_get_occupancy(self.h, SOURCES[source], _ptr(out)), "getOccupancy")
+        return out
+
+    def get_occupancy_image(self, source: str = "grid") -> np.ndarray:
+        out = np.empty(self.grid_cell_count, np.uint8)
+        self._check(lib.dogm_get_occupancy_image(self.h, SOURCES[source], _ptr(out)), "getOccupancyImage")
+        return out
+
+    def get_cells_with_velocity(self, min_occupancy: float = 0.7, min_velocity: float = 4.0,
+                                capacity: int | None = None) -> tuple[np.ndarray, int]:
+        """(records, count): records of the passing cells (at most `capacity`), count of all."""
+        return _velocity_cells(lambda out, cap, cnt: lib.dogm_get_cells_with_velocity(
+            self.h, min_occupancy, min_velocity, out, cap, cnt), self.grid_cell_count, capacity, self._check)
 
     def _get_set(self, fn, n: int, what: str) -> dict:
         state = np.empty((n, 4), np.float32)
@@ -497,6 +529,23 @@ class Batch:
     def set_cdf_mode(self, vid: int, mode: str) -> None:
         self._check(lib.dogm_batch_set_cdf_mode(self.h, vid, CDF_MODES[mode]), "setCdfMode")
 
+    def set_extraction(self, what: int, min_occupancy: float = 0.7, min_velocity: float = 4.0) -> None:
+        self._check(lib.dogm_batch_set_extraction(self.h, what, min_occupancy, min_velocity), "setExtraction")
+
+    def get_occupancy(self, vid: int) -> np.ndarray:
+        out = np.empty(self.grid_cell_count(vid), np.float32)
+        self._check(lib.dogm_batch_get_occupancy(self.h, vid, _ptr(out)), "getOccupancy")
+        return out
+
+    def get_occupancy_image(self, vid: int) -> np.ndarray:
+        out = np.empty(self.grid_cell_count(vid), np.uint8)
+        self._check(lib.dogm_batch_get_occupancy_image(self.h, vid, _ptr(out)), "getOccupancyImage")
+        return out
+
+    def get_cells_with_velocity(self, vid: int, capacity: int | None = None) -> tuple[np.ndarray, int]:
+        return _velocity_cells(lambda out, cap, cnt: lib.dogm_batch_get_cells_with_velocity(
+            self.h, vid, out, cap, cnt), self.grid_cell_count(vid), capacity, self._check)
+
     def set_kernel_timing(self, name: str, enable: bool = True) -> None:
         self._check(lib.dogm_batch_set_kernel_timing(self.h, DOGM.kernel_names().index(name), int(enable)),
                     "timing")
@@ -631,6 +680,36 @@ class Server:
         a, b = C.c_longlong(), C.c_longlong()
         self._check(lib.dogm_server_stats(self.h, C.byref(a), C.byref(b)), "stats")
         return a.value, b.value
+
+    def set_extraction(self, what: int, min_occupancy: float = 0.7, min_velocity: float = 4.0) -> None:
+        self._check(lib.dogm_server_set_extraction(self.h, what, min_occupancy, min_velocity), "setExtraction")
+
+    def _cells(self, vid: int) -> int:
+        p = self.vehicles[vid]
+        gs = int(np.float32(p.size) / np.float32(p.resolution))
+        return gs * gs
+
+    def get_occupancy(self, vid: int) -> np.ndarray:
+        out = np.empty(self._cells(vid), np.float32)
+        self._check(lib.dogm_server_get_occupancy(self.h, vid, _ptr(out)), "getOccupancy")
+        return out
+
+    def get_occupancy_image(self, vid: int) -> np.ndarray:
+        out = np.empty(self._cells(vid), np.uint8)
+        self._check(lib.dogm_server_get_occupancy_image(self.h, vid, _ptr(out)), "getOccupancyImage")
+        return out
+
+    def get_cells_with_velocity(self, vid: int, capacity: int | None = None) -> tuple[np.ndarray, int]:
+        return _velocity_cells(lambda out, cap, cnt: lib.dogm_server_get_cells_with_velocity(
+            self.h, vid, out, cap, cnt), self._cells(vid), capacity, self._check)
+
+
+def _velocity_cells(call, cells: int, capacity: int | None, check) -> tuple[np.ndarray, int]:
+    cap = cells if capacity is None else int(capacity)
+    out = np.empty(max(cap, 1), dtype=VELOCITY_CELL_DTYPE)
+    cnt = C.c_int()
+    check(call(_ptr(out), cap, C.byref(cnt)), "getCellsWithVelocity")
+    return out[:min(cap, cnt.value)].copy(), cnt.value
 
 
 def _as_particle_arrays(n, state, idx, weight, assoc):

@@ -161,6 +161,48 @@ int dogm_set_weight_array(dogm_handle* h, const float* in);
 int dogm_set_born_masses(dogm_handle* h, const float* in);
 
 /* ----------------------------------------------------------------------------------------------
+ * Result extraction on the device.  Replaces the demo's per-frame post-processing, which reads the
+ * whole grid back (getGridCells, 64 bytes per cell) and loops over it on the host
+ * (dogm/demo/utils/image_creation.cpp:19-66 computeCellsWithVelocity, :70-87
+ * compute_measurement_grid_image, :112-128 compute_dogm_image).  Outputs are compact: 4 or 1 bytes
+ * per cell, or one record per cell that passes the velocity test.
+ *
+ *  pignistic occupancy  p = occ + 0.5 * (1 - occ - free)   (f32, image_creation.cpp:19-22)
+ *  image                255 - floor(255 * p), clamped to [0, 255] (image_creation.cpp:80-83, 123-125)
+ *  velocity test        p >= min_occupancy and v^T S^-1 v >= min_velocity, v = (mean_x_vel,
+ *                       mean_y_vel), S = [[var_x_vel, covar_xy_vel], [covar_xy_vel, var_y_vel]]
+ *                       (image_creation.cpp:35-50).  S^-1 is OpenCV's closed 2x2 inverse: determinant
+ *                       in f64, elements f32(element * (1 / det)), all zero when det == 0; both
+ *                       products are 2-term dot products summed in f64 and rounded to f32.
+ * ---------------------------------------------------------------------------------------------- */
+enum
+{
+    DOGM_SOURCE_GRID = 0,        /* the filter's grid cells (occ_mass, free_mass) */
+    DOGM_SOURCE_MEASUREMENT = 1  /* the current measurement grid (occupancy and image only) */
+};
+enum
+{
+    DOGM_EXTRACT_OCCUPANCY = 1, /* f32 pignistic occupancy per cell */
+    DOGM_EXTRACT_IMAGE = 2,     /* 8-bit occupancy image per cell */
+    DOGM_EXTRACT_VELOCITY = 4   /* cells that pass the velocity test */
+};
+
+/* A cell that passed the velocity test: the demo's Point<dogm::GridCell> (dbscan.h:10-17) without
+ * cluster_id.  x, y = column and row of the cell (image_creation.cpp:53-56). */
+typedef struct dogm_velocity_cell
+{
+    float x, y;
+    dogm_grid_cell data;
+} dogm_velocity_cell;
+
+int dogm_get_occupancy(dogm_handle* h, int source, float* out);         /* grid_size^2 floats */
+int dogm_get_occupancy_image(dogm_handle* h, int source, uint8_t* out); /* grid_size^2 bytes */
+/* Passing cells in row-major order (the demo's loop order).  Writes min(*count, capacity) records;
+ * *count = number of passing cells.  out may be NULL when capacity is 0. */
+int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_velocity, dogm_velocity_cell* out,
+                                 int capacity, int* count);
+
+/* ----------------------------------------------------------------------------------------------
  * Device-memory helpers for callers that pass device measurement grids (the reference's callers use
  * cudaMalloc / cudaMemcpy for this, e.g. dogm/demo/simulator/mapping/laser_to_meas_grid.cu:15).
  * kind: 0 host->device, 1 device->host, 2 device->device.
@@ -243,6 +285,15 @@ int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, in
                              uint8_t* associated);
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
 int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
+/* Result extraction of the grid source inside every dogm_batch_update step, for all vehicles of
+ * the step in the same launches; what = OR of DOGM_EXTRACT_* bits (0 = off).  The getters below
+ * return the results of the vehicle's last step, or extract them on demand (with these
+ * thresholds) when that step did not. */
+int dogm_batch_set_extraction(dogm_batch* b, int what, float min_occupancy, float min_velocity);
+int dogm_batch_get_occupancy(dogm_batch* b, int vehicle_id, float* out);
+int dogm_batch_get_occupancy_image(dogm_batch* b, int vehicle_id, uint8_t* out);
+int dogm_batch_get_cells_with_velocity(dogm_batch* b, int vehicle_id, dogm_velocity_cell* out, int capacity,
+                                       int* count);
 int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable);
 int dogm_batch_get_kernel_timing(dogm_batch* b, int kernel_id, double* total_ms, long long* launches);
 int dogm_batch_reset_kernel_timing(dogm_batch* b);
@@ -302,6 +353,13 @@ int dogm_server_get_grid_cells(dogm_server* s, int vehicle_id, dogm_grid_cell* o
 int dogm_server_get_particles(dogm_server* s, int vehicle_id, dogm_vec4* state, int* grid_cell_idx, float* weight,
                               uint8_t* associated);
 int dogm_server_stats(const dogm_server* s, long long* steps, long long* frames);
+/* Extraction inside every step (as dogm_batch_set_extraction); the getters read the vehicle's
+ * latest results, compact instead of the 64-byte-per-cell dogm_server_get_grid_cells. */
+int dogm_server_set_extraction(dogm_server* s, int what, float min_occupancy, float min_velocity);
+int dogm_server_get_occupancy(dogm_server* s, int vehicle_id, float* out);
+int dogm_server_get_occupancy_image(dogm_server* s, int vehicle_id, uint8_t* out);
+int dogm_server_get_cells_with_velocity(dogm_server* s, int vehicle_id, dogm_velocity_cell* out, int capacity,
+                                        int* count);
 
 #ifdef __cplusplus
 }

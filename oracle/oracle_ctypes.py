@@ -63,6 +63,9 @@ def lib():
             "oracle_fx_scale_u64": (None, [C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64),
                                            C.POINTER(C.c_uint64)]),
             "oracle_order_end": (i, [C.c_int64, C.c_uint64, C.c_int64, C.c_uint64, i]),
+            "oracle_occupancy": (None, [vp, vp, i, vp, vp]),
+            "oracle_cells_with_velocity": (i, [i] + [vp] * 7 + [f, f, vp]),
+            "oracle_velocity_normalized": (f, [f, f, f, f, f]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -330,3 +333,29 @@ def spacing_fractions(seed: int, instance: int, epoch: int, n: int) -> np.ndarra
     out = np.empty(n, np.float64)
     lib().oracle_spacing_fractions(seed, instance, epoch, n, out.ctypes.data)
     return out
+
+
+def occupancy(occ, free_mass) -> tuple[np.ndarray, np.ndarray]:
+    """Pignistic occupancy and 8-bit image of the cells (image_creation.cpp:19-22, 123-125)."""
+    o = np.ascontiguousarray(occ, np.float32)
+    fr = np.ascontiguousarray(free_mass, np.float32)
+    p = np.empty(o.size, np.float32)
+    img = np.empty(o.size, np.uint8)
+    lib().oracle_occupancy(o.ctypes.data, fr.ctypes.data, o.size, p.ctypes.data, img.ctypes.data)
+    return p, img
+
+
+def cells_with_velocity(grid_size: int, cells, min_occupancy: float, min_velocity: float) -> np.ndarray:
+    """Row-major indices of the cells computeCellsWithVelocity keeps (image_creation.cpp:24-66);
+    `cells` is a structured array with the dogm_grid_cell field names."""
+    fields = ["occ_mass", "free_mass", "mean_x_vel", "mean_y_vel", "var_x_vel", "var_y_vel", "covar_xy_vel"]
+    arrs = [np.ascontiguousarray(cells[k], np.float32) for k in fields]
+    out = np.empty(max(arrs[0].size, 1), np.int32)
+    n = lib().oracle_cells_with_velocity(grid_size, *[a.ctypes.data for a in arrs], min_occupancy, min_velocity,
+                                         out.ctypes.data)
+    return out[:n].copy()
+
+
+def velocity_normalized(mean_x: float, mean_y: float, var_x: float, var_y: float, covar: float) -> float:
+    """v^T S^-1 v of one cell as the demo evaluates it (image_creation.cpp:35-50)."""
+    return lib().oracle_velocity_normalized(mean_x, mean_y, var_x, var_y, covar)

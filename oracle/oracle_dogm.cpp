@@ -845,6 +845,62 @@ int Oracle<double>::num_assoc(int n, float pA) const
 // ------------------------------------------------------------------------------------------------
 // C ABI used by the tests (ctypes) and the host-core timing leg of bench.py.
 // ------------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
+// Result extraction (dogm/demo/utils/image_creation.cpp), over structure-of-arrays cell fields.
+// ---------------------------------------------------------------------------------------------
+
+// image_creation.cpp:19-22.
+static float pignistic_f32(float free_mass, float occ_mass)
+{
+    const float unknown = 1.0f - occ_mass - free_mass;
+    return occ_mass + 0.5f * unknown;
+}
+
+// OpenCV cv::Mat::inv() (cv::invert, DECOMP_LU) of a 2x2 CV_32F matrix: the closed form of
+// modules/core/src/lapack.cpp for n == 2 -- determinant in double, d = 1 / det, each element
+// stored as float(element * d) at its adjugate position; a zero matrix when det == 0.
+static void cv_invert_2x2_f32(const float m[2][2], float r[2][2])
+{
+    double det = (double)m[0][0] * m[1][1] - (double)m[0][1] * m[1][0];
+    r[0][0] = r[0][1] = r[1][0] = r[1][1] = 0.0f;
+    if (det != 0.0)
+    {
+        const double d = 1.0 / det;
+        r[1][1] = (float)(m[0][0] * d);
+        r[0][0] = (float)(m[1][1] * d);
+        r[0][1] = (float)(-m[0][1] * d);
+        r[1][0] = (float)(-m[1][0] * d);
+    }
+}
+
+// cv::gemm of CV_32F operands: every output element is a dot product accumulated in double from
+// zero and stored as float.  A is rows x inner, B is inner x cols, all row-major.
+static void cv_gemm_f32(const float* a, const float* b, float* c, int rows, int inner, int cols)
+{
+    for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j)
+        {
+            double acc = 0.0;
+            for (int k = 0; k < inner; ++k)
+                acc += (double)a[i * inner + k] * (double)b[k * cols + j];
+            c[i * cols + j] = (float)acc;
+        }
+}
+
+// velocity_mean.t() * velocity_covar.inv() * velocity_mean (image_creation.cpp:35-50), evaluated
+// left to right with a float temporary between the two products.
+static float velocity_normalized_by_variance(float mx, float my, float vxx, float vyy, float cxy)
+{
+    const float mean[2] = {mx, my};
+    const float cov[2][2] = {{vxx, cxy}, {cxy, vyy}};
+    float inv[2][2];
+    cv_invert_2x2_f32(cov, inv);
+    float row[2], out[1];
+    cv_gemm_f32(mean, &inv[0][0], row, 1, 2, 2);
+    cv_gemm_f32(row, mean, out, 1, 2, 1);
+    return out[0];
+}
+
 extern "C" {
 
 void* oracle_create(const dogm_params* params, uint64_t seed, uint32_t instance, int fp64)
@@ -964,6 +1020,52 @@ int oracle_order_end(int64_t hi, uint64_t lo, int64_t thi, uint64_t tlo, int v)
     const fx_t pref = (fx_t)(((ufx_t)(uint64_t)hi << 64) | lo);
     const fx_t tot = (fx_t)(((ufx_t)(uint64_t)thi << 64) | tlo);
     return order_end(pref, order_scale(tot, v), v);
+}
+
+// Pignistic occupancy and the 8-bit image (image_creation.cpp:19-22, 123-125; floor clamped to
+// [0, 255]) of n cells.  Either output may be null.
+void oracle_occupancy(const float* occ, const float* free_mass, int n, float* p, uint8_t* img)
+{
+    for (int c = 0; c < n; ++c)
+    {
+        const float q = pignistic_f32(free_mass[c], occ[c]);
+        if (p)
+            p[c] = q;
+        if (img)
+        {
+            float g = std::floor(q * 255.0f);
+            if (!(g >= 0.0f))
+                g = 0.0f;
+            if (g > 255.0f)
+                g = 255.0f;
+            img[c] = (uint8_t)(255 - (int)g);
+        }
+    }
+}
+
+// computeCellsWithVelocity (image_creation.cpp:24-66): indices of the passing cells in the demo's
+// y-then-x loop order; returns their number.  out_idx holds up to n entries.
+int oracle_cells_with_velocity(int gs, const float* occ, const float* free_mass, const float* mean_x,
+                               const float* mean_y, const float* var_x, const float* var_y, const float* covar,
+                               float min_occ, float min_vel, int* out_idx)
+{
+    int k = 0;
+    for (int y = 0; y < gs; y++)
+        for (int x = 0; x < gs; x++)
+        {
+            const int index = y * gs + x;
+            const float o = pignistic_f32(free_mass[index], occ[index]);
+            const float m = velocity_normalized_by_variance(mean_x[index], mean_y[index], var_x[index], var_y[index],
+                                                           covar[index]);
+            if (o >= min_occ && m >= min_vel)
+                out_idx[k++] = index;
+        }
+    return k;
+}
+
+float oracle_velocity_normalized(float mx, float my, float vxx, float vyy, float cxy)
+{
+    return velocity_normalized_by_variance(mx, my, vxx, vyy, cxy);
 }
 
 }  // extern "C"
