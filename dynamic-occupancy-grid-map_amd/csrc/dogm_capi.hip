@@ -16,6 +16,9 @@ using namespace dogm_hip;
 struct dogm_batch
 {
     Engine eng;
+    Engine aux;             // second lane of steps over two or more vehicles (created on first use)
+    bool aux_ready = false;
+    int lanes = 2;
     std::map<int, std::unique_ptr<Vehicle>> veh;
     int next_id = 1;
     uint64_t bytes = 0;
@@ -649,6 +652,8 @@ void dogm_batch_destroy(dogm_batch* b)
     for (auto& kv : b->veh)
         vehicle_free(kv.second.get());
     b->veh.clear();
+    if (b->aux_ready)
+        engine_release(&b->aux);
     engine_release(&b->eng);
     delete b;
 }
@@ -758,7 +763,23 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
     for (int i = 0; i < count; ++i)
         in[i] = FrameIn{meas ? meas[i] : nullptr, is_device, x[i], y[i], yaw[i], dt[i]};
     E->ext = b->ext;
-    return engine_frame(E, list.data(), in.data(), count, b->ext.what != 0);
+    if (count >= 2 && b->lanes > 1 && !b->aux_ready)
+    {
+        std::string why;
+        const int st = engine_init(&b->aux, &why);
+        if (st != DOGM_OK)
+        {
+            engine_release(&b->aux);
+            b->aux = Engine{};
+            return fail(E, st, why);
+        }
+        for (int k = 0; k < K_COUNT; ++k)
+            b->aux.timing_on[k] = E->timing_on[k];
+        b->aux_ready = true;
+    }
+    Engine* lanes[2] = {E, &b->aux};
+    return engine_frame_lanes(lanes, b->aux_ready && b->lanes > 1 ? 2 : 1, list.data(), in.data(), count,
+                              b->ext.what != 0);
 }
 
 int dogm_batch_get_grid_cells(dogm_batch* b, int vehicle_id, dogm_grid_cell* out)
@@ -802,6 +823,16 @@ int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
     if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     v->cdf_mode = mode;
+    return DOGM_OK;
+}
+
+int dogm_batch_set_lanes(dogm_batch* b, int lanes)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    if (lanes < 1 || lanes > 2)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "lanes must be 1 or 2");
+    b->lanes = lanes;
     return DOGM_OK;
 }
 
@@ -861,14 +892,28 @@ int dogm_batch_set_kernel_timing(dogm_batch* b, int kernel_id, int enable)
 {
     if (!b)
         return DOGM_ERR_INVALID_ARGUMENT;
-    return set_timing(&b->eng, kernel_id, enable);
+    const int st = set_timing(&b->eng, kernel_id, enable);
+    if (st == DOGM_OK && b->aux_ready)
+        (void)set_timing(&b->aux, kernel_id, enable);
+    return st;
 }
 
 int dogm_batch_get_kernel_timing(dogm_batch* b, int kernel_id, double* total_ms, long long* launches)
 {
     if (!b)
         return DOGM_ERR_INVALID_ARGUMENT;
-    return get_timing(&b->eng, kernel_id, total_ms, launches);
+    double ms = 0.0, ms2 = 0.0;
+    long long n = 0, n2 = 0;
+    int st = get_timing(&b->eng, kernel_id, &ms, &n);
+    if (st == DOGM_OK && b->aux_ready)
+        st = get_timing(&b->aux, kernel_id, &ms2, &n2);
+    if (st != DOGM_OK)
+        return st;
+    if (total_ms)
+        *total_ms = ms + ms2;
+    if (launches)
+        *launches = n + n2;
+    return DOGM_OK;
 }
 
 int dogm_batch_reset_kernel_timing(dogm_batch* b)
@@ -876,6 +921,8 @@ int dogm_batch_reset_kernel_timing(dogm_batch* b)
     if (!b)
         return DOGM_ERR_INVALID_ARGUMENT;
     reset_timing(&b->eng);
+    if (b->aux_ready)
+        reset_timing(&b->aux);
     return DOGM_OK;
 }
 

@@ -504,7 +504,11 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     return DOGM_OK;
 }
 
-int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
+namespace
+{
+
+// Measurement copies, lazy pose updates and the stage sequence of one step, without waiting.
+int frame_enqueue(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
 {
     for (int i = 0; i < m; ++i)
     {
@@ -513,10 +517,7 @@ int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool 
         {
             int st = vehicle_set_measurement(E, v, in[i].meas, in[i].is_device);
             if (st != DOGM_OK)
-            {
-                (void)hipStreamSynchronize(E->stream);
                 return st;
-            }
         }
         else if (!v->first_meas)
         {
@@ -526,13 +527,59 @@ int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool 
         vehicle_update_pose(E, v, in[i].x, in[i].y, in[i].yaw, true);
         v->dt = in[i].dt;
     }
-    int st = engine_run(E, veh, m, ST_ALL | (extract ? ST_EXTRACT : 0));
+    return engine_run(E, veh, m, ST_ALL | (extract ? ST_EXTRACT : 0));
+}
+
+int frame_wait(Engine* E, int st)
+{
     const hipError_t e = hipStreamSynchronize(E->stream);
     if (st != DOGM_OK)
         return st;
     if (e != hipSuccess)
         return set_err(&E->last_error, DOGM_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
     engine_collect_timing(E);
+    return DOGM_OK;
+}
+
+}  // namespace
+
+int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
+{
+    return frame_wait(E, frame_enqueue(E, veh, in, m, extract));
+}
+
+int engine_frame_lanes(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
+{
+    if (nl < 2 || m < 2)
+        return engine_frame(lanes[0], veh, in, m, extract);
+    // Each vehicle goes to the lane with less stage-boundary traffic so far (bytes per frame
+    // 172 N + 29 B + 128 C); a given vehicle list always splits the same way, so each lane keeps
+    // its work tables from step to step.
+    std::vector<Vehicle*> lv[2];
+    std::vector<FrameIn> li[2];
+    double load[2] = {0.0, 0.0};
+    for (int i = 0; i < m; ++i)
+    {
+        const int l = load[1] < load[0] ? 1 : 0;
+        load[l] += 172.0 * veh[i]->N + 29.0 * veh[i]->B + 128.0 * veh[i]->C;
+        lv[l].push_back(veh[i]);
+        li[l].push_back(in[i]);
+    }
+    int st[2];
+    for (int l = 0; l < 2; ++l)
+    {
+        lanes[l]->ext = lanes[0]->ext;
+        st[l] = frame_enqueue(lanes[l], lv[l].data(), li[l].data(), (int)lv[l].size(), extract);
+    }
+    for (int l = 0; l < 2; ++l)
+        st[l] = frame_wait(lanes[l], st[l]);
+    if (st[0] != DOGM_OK)
+        return st[0];
+    if (st[1] != DOGM_OK)
+    {
+        lanes[0]->last_error = lanes[1]->last_error;
+        return st[1];
+    }
     return DOGM_OK;
 }
 

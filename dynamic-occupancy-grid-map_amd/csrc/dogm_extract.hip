@@ -4,10 +4,11 @@
 // cells that pass the velocity test.
 //
 // Three launches over the step's cell work table:
-//   mark     one thread per cell: pignistic occupancy (f32 and / or 8-bit image) and the velocity
-//            test; each wave stores its 64-lane ballot of passing cells and each 256-cell unit its
-//            passing count.  The five moment fields are read only for cells whose occupancy
-//            reaches the threshold, so an unoccupied cell costs the 8 bytes of occ / free.
+//   mark     one wave per 256-cell unit, four consecutive cells per lane (16-byte loads of occ /
+//            free, 16-byte occupancy and 4-byte image stores): pignistic occupancy and the velocity
+//            test; each unit stores its four 64-lane ballots of passing cells and its passing
+//            count.  The five moment fields are read only for cells whose occupancy reaches the
+//            threshold, so an unoccupied cell costs the 8 bytes of occ / free.
 //   scan     one workgroup per vehicle: exclusive prefix of the unit counts.
 //   compact  units with passing cells write their cell indices at the unit's offset plus the
 //            cell's rank among the unit's ballots, so the list is in row-major order.
@@ -61,70 +62,135 @@ struct ExtractArgs
     float min_vel;
 };
 
+constexpr int XCELLS = 4;                         // consecutive cells per lane
+constexpr int XUNITS = BLOCK / WAVE;              // units per mark workgroup (one wave per unit)
+static_assert(WAVE * XCELLS == UNIT, "one wave covers one unit");
+
+// Mark: wave w of the workgroup covers unit (tile base / UNIT + w), lane l its cells 4l .. 4l + 3.
+// The unit's four ballots are stored as masks j = 0..3 (bit l of mask j <-> cell 4l + j) with the
+// unit's passing count.
 __global__ __launch_bounds__(BLOCK) void k_extract_mark(const VehDesc* __restrict__ descs,
                                                         const uint32_t* __restrict__ tab, ExtractArgs a)
 {
-    __shared__ int s_cnt[BLOCK / WAVE];
-    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const TileRef tr = tile_ref(tab, XUNITS, blockIdx.x);
     const VehDesc& d = descs[tr.v];
-    const int c = tr.base + threadIdx.x;
-    const bool in = c < d.C;
-    float occ = 0.0f, fr = 0.0f;
-    if (in)
+    const int w = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int u = tr.base / UNIT + w;
+    const int c0 = u * UNIT + lane * XCELLS;
+    float occ[XCELLS], fr[XCELLS];
+    if (a.source == DOGM_SOURCE_GRID)
     {
-        if (a.source == DOGM_SOURCE_GRID)
+        if (c0 + XCELLS <= d.C)
         {
-            occ = d.f[GF_OCC][c];
-            fr = d.free[a.fsel][c];
+            const float4 o = *reinterpret_cast<const float4*>(d.f[GF_OCC] + c0);
+            const float4 f = *reinterpret_cast<const float4*>(d.free[a.fsel] + c0);
+            occ[0] = o.x, occ[1] = o.y, occ[2] = o.z, occ[3] = o.w;
+            fr[0] = f.x, fr[1] = f.y, fr[2] = f.z, fr[3] = f.w;
         }
         else
         {
-            const float2 m = *reinterpret_cast<const float2*>(&d.meas[c].free_mass);
-            fr = m.x;
-            occ = m.y;
+#pragma unroll
+            for (int j = 0; j < XCELLS; ++j)
+            {
+                const bool in = c0 + j < d.C;
+                occ[j] = in ? d.f[GF_OCC][c0 + j] : 0.0f;
+                fr[j] = in ? d.free[a.fsel][c0 + j] : 0.0f;
+            }
         }
     }
-    const float p = pignistic(fr, occ);
-    if (in && (a.what & DOGM_EXTRACT_OCCUPANCY))
-        d.x_occ[c] = p;
-    if (in && (a.what & DOGM_EXTRACT_IMAGE))
-        d.x_img[c] = occupancy_gray(p);
+    else
+    {
+#pragma unroll
+        for (int j = 0; j < XCELLS; ++j)
+        {
+            float2 m = make_float2(0.0f, 0.0f);
+            if (c0 + j < d.C)
+                m = *reinterpret_cast<const float2*>(&d.meas[c0 + j].free_mass);
+            fr[j] = m.x;
+            occ[j] = m.y;
+        }
+    }
+    float p[XCELLS];
+#pragma unroll
+    for (int j = 0; j < XCELLS; ++j)
+        p[j] = pignistic(fr[j], occ[j]);
+    const bool full = c0 + XCELLS <= d.C;
+    if (a.what & DOGM_EXTRACT_OCCUPANCY)
+    {
+        if (full)
+            *reinterpret_cast<float4*>(d.x_occ + c0) = make_float4(p[0], p[1], p[2], p[3]);
+        else
+            for (int j = 0; j < XCELLS; ++j)
+                if (c0 + j < d.C)
+                    d.x_occ[c0 + j] = p[j];
+    }
+    if (a.what & DOGM_EXTRACT_IMAGE)
+    {
+        if (full)
+        {
+            const uint32_t g = (uint32_t)occupancy_gray(p[0]) | ((uint32_t)occupancy_gray(p[1]) << 8) |
+                               ((uint32_t)occupancy_gray(p[2]) << 16) | ((uint32_t)occupancy_gray(p[3]) << 24);
+            *reinterpret_cast<uint32_t*>(d.x_img + c0) = g;
+        }
+        else
+        {
+            for (int j = 0; j < XCELLS; ++j)
+                if (c0 + j < d.C)
+                    d.x_img[c0 + j] = occupancy_gray(p[j]);
+        }
+    }
     if (!(a.what & DOGM_EXTRACT_VELOCITY))
         return;
-    bool pass = false;
-    if (in && p >= a.min_occ)
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < XCELLS; ++j)
     {
-        const float m = mahalanobis2(d.f[GF_MEAN_X][c], d.f[GF_MEAN_Y][c], d.f[GF_VAR_X][c], d.f[GF_COVAR][c],
-                                     d.f[GF_VAR_Y][c]);
-        pass = m >= a.min_vel;
+        const int c = c0 + j;
+        bool pass = false;
+        if (c < d.C && p[j] >= a.min_occ)
+        {
+            const float m = mahalanobis2(d.f[GF_MEAN_X][c], d.f[GF_MEAN_Y][c], d.f[GF_VAR_X][c], d.f[GF_COVAR][c],
+                                         d.f[GF_VAR_Y][c]);
+            pass = m >= a.min_vel;
+        }
+        const uint64_t bal = __ballot(pass);
+        cnt += __popcll(bal);
+        if (lane == 0)
+            d.x_mask[(size_t)u * XCELLS + j] = bal;
     }
-    const uint64_t bal = __ballot(pass);
-    const int w = threadIdx.x / WAVE;
-    if ((threadIdx.x & (WAVE - 1)) == 0)
-    {
-        d.x_mask[(tr.base / WAVE) + w] = bal;
-        s_cnt[w] = __popcll(bal);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        d.x_cnt[tr.base / UNIT] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (lane == 0)
+        d.x_cnt[u] = cnt;
 }
 
+constexpr int XSCAN_BLOCK = 1024;
+constexpr int XSCAN_ITEMS = UNIT_ALIGN;  // consecutive units per thread (a vehicle's units come in 8s)
+
 // One workgroup per vehicle: x_off = exclusive prefix of x_cnt over the vehicle's units,
-// x_off[units] = total.
-__global__ __launch_bounds__(BLOCK) void k_extract_scan(const VehDesc* __restrict__ descs)
+// x_off[units] = total.  Each pass covers XSCAN_BLOCK * 8 units, eight consecutive per thread.
+__global__ __launch_bounds__(XSCAN_BLOCK) void k_extract_scan(const VehDesc* __restrict__ descs)
 {
-    __shared__ int s_w[BLOCK / WAVE];
+    __shared__ int s_w[XSCAN_BLOCK / WAVE];
     const VehDesc& d = descs[blockIdx.x];
     const int n = d.nu[SP_C];
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
     int carry = 0;
-    for (int k0 = 0; k0 < n; k0 += BLOCK)
+    for (int k0 = 0; k0 < n; k0 += XSCAN_BLOCK * XSCAN_ITEMS)
     {
-        const int k = k0 + threadIdx.x;
-        const int v = k < n ? d.x_cnt[k] : 0;
-        int x = v;
+        const int k = k0 + threadIdx.x * XSCAN_ITEMS;
+        int v[XSCAN_ITEMS] = {};
+        if (k < n)
+        {
+            const int4 a = *reinterpret_cast<const int4*>(d.x_cnt + k);
+            const int4 b = *reinterpret_cast<const int4*>(d.x_cnt + k + 4);
+            v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+        }
+        int t = 0;
+#pragma unroll
+        for (int j = 0; j < XSCAN_ITEMS; ++j)
+            t += v[j];
+        int x = t;
 #pragma unroll
         for (int o = 1; o < WAVE; o <<= 1)
         {
@@ -135,12 +201,26 @@ __global__ __launch_bounds__(BLOCK) void k_extract_scan(const VehDesc* __restric
         if (lane == WAVE - 1)
             s_w[w] = x;
         __syncthreads();
-        int pre = carry;
-        for (int j = 0; j < w; ++j)
-            pre += s_w[j];
-        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        int pre = carry, tot = 0;
+        for (int j = 0; j < XSCAN_BLOCK / WAVE; ++j)
+        {
+            const int s_j = s_w[j];
+            pre += j < w ? s_j : 0;
+            tot += s_j;
+        }
         if (k < n)
-            d.x_off[k] = pre + x - v;
+        {
+            int e[XSCAN_ITEMS];
+            int run = pre + x - t;
+#pragma unroll
+            for (int j = 0; j < XSCAN_ITEMS; ++j)
+            {
+                e[j] = run;
+                run += v[j];
+            }
+            *reinterpret_cast<int4*>(d.x_off + k) = make_int4(e[0], e[1], e[2], e[3]);
+            *reinterpret_cast<int4*>(d.x_off + k + 4) = make_int4(e[4], e[5], e[6], e[7]);
+        }
         carry += tot;
         __syncthreads();
     }
@@ -148,39 +228,53 @@ __global__ __launch_bounds__(BLOCK) void k_extract_scan(const VehDesc* __restric
         d.x_off[n] = carry;
 }
 
+// Compact: one wave per unit (two units per wave, eight per workgroup); units without passing
+// cells are skipped.  Cell 4l + j of the unit has rank
+//   sum_j' popc(mask_j' & lanes below l) + sum_{j' < j} bit l of mask_j'.
 __global__ __launch_bounds__(BLOCK) void k_extract_compact(const VehDesc* __restrict__ descs,
                                                            const uint32_t* __restrict__ tab)
 {
-    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const TileRef tr = tile_ref(tab, UNIT_ALIGN, blockIdx.x);
     const VehDesc& d = descs[tr.v];
-    const int u = tr.base / UNIT;
-    if (d.x_cnt[u] == 0)
-        return;
     const int w = threadIdx.x / WAVE;
-    const unsigned long long* masks = d.x_mask + (size_t)u * (UNIT / WAVE);
-    int pre = d.x_off[u];
-    for (int j = 0; j < w; ++j)
-        pre += __popcll(masks[j]);
-    const uint64_t mine = masks[w];
     const int lane = threadIdx.x & (WAVE - 1);
-    if ((mine >> lane) & 1ull)
-        d.x_idx[pre + __popcll(mine & lanemask_lt())] = tr.base + threadIdx.x;
+    const uint64_t below = lanemask_lt();
+    for (int k = 0; k < UNIT_ALIGN / XUNITS; ++k)
+    {
+        const int u = tr.base / UNIT + k * XUNITS + w;
+        if (d.x_cnt[u] == 0)
+            continue;
+        uint64_t m[XCELLS];
+        int r = d.x_off[u];
+#pragma unroll
+        for (int j = 0; j < XCELLS; ++j)
+        {
+            m[j] = d.x_mask[(size_t)u * XCELLS + j];
+            r += __popcll(m[j] & below);
+        }
+#pragma unroll
+        for (int j = 0; j < XCELLS; ++j)
+        {
+            if ((m[j] >> lane) & 1ull)
+                d.x_idx[r++] = u * UNIT + lane * XCELLS + j;
+        }
+    }
 }
 
 void run_extract(Engine* E, int fsel)
 {
     const ExtractArgs a{E->ext.what, E->ext.source, fsel, E->ext.min_occ, E->ext.min_vel};
     launch_begin(E, K_EXTRACT_MARK);
-    hipLaunchKernelGGL(k_extract_mark, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       a);
+    hipLaunchKernelGGL(k_extract_mark, dim3(E->tiles(SP_C, XUNITS)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                       E->tab(SP_C), a);
     launch_end(E, K_EXTRACT_MARK);
     if (!(a.what & DOGM_EXTRACT_VELOCITY))
         return;
     launch_begin(E, K_EXTRACT_SCAN);
-    hipLaunchKernelGGL(k_extract_scan, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc);
+    hipLaunchKernelGGL(k_extract_scan, dim3(E->m), dim3(XSCAN_BLOCK), 0, E->stream, E->d_desc);
     launch_end(E, K_EXTRACT_SCAN);
     launch_begin(E, K_EXTRACT_COMPACT);
-    hipLaunchKernelGGL(k_extract_compact, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc,
+    hipLaunchKernelGGL(k_extract_compact, dim3(E->tiles(SP_C, UNIT_ALIGN)), dim3(BLOCK), 0, E->stream, E->d_desc,
                        E->tab(SP_C));
     launch_end(E, K_EXTRACT_COMPACT);
 }

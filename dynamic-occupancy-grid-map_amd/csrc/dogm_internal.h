@@ -378,6 +378,12 @@ struct FrameIn
 // Measurement copies, lazy pose updates, the full stage sequence (plus ST_EXTRACT with
 // E->ext when `extract`), and a wait for the stream.
 int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool extract = false);
+// The same step split over two engines ("lanes", each with its own stream, tables and scratch):
+// every vehicle runs all its stages on one lane, and the two lanes' launches overlap on the device
+// (one lane's small, latency-bound launches and tails run beside the other's bulk work).  Results
+// equal those of engine_frame.  Errors are reported on lanes[0].
+int engine_frame_lanes(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m,
+                       bool extract = false);
 // Allocates the extraction buffers `what` needs on a vehicle.
 int vehicle_ext_alloc(Vehicle* v, int what, std::string* why);
 // Velocity records are packed and read back in chunks of at most this many.

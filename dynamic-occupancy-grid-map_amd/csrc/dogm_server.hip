@@ -28,6 +28,7 @@ using namespace dogm_hip;
 struct dogm_server
 {
     Engine eng;
+    Engine aux;  // second lane of steps over two or more vehicles (engine_frame_lanes)
     std::mutex eng_mu;  // serialises steps and read-backs on the engine's stream
     std::mutex mu;      // guards everything below
     std::condition_variable cv_work, cv_done;
@@ -108,7 +109,8 @@ int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* 
     std::lock_guard<std::mutex> ge(s->eng_mu);
     const auto a = std::chrono::steady_clock::now();
     s->eng.ext = s->ext;
-    const int st = engine_frame(&s->eng, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
+    Engine* lanes[2] = {&s->eng, &s->aux};
+    const int st = engine_frame_lanes(lanes, 2, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
     *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     if (st != DOGM_OK)
     {
@@ -174,9 +176,12 @@ int dogm_server_create(const dogm_server_config* cfg, dogm_server** out)
     if (!s)
         return DOGM_ERR_OUT_OF_MEMORY;
     std::string why;
-    const int st = engine_init(&s->eng, &why);
+    int st = engine_init(&s->eng, &why);
+    if (st == DOGM_OK)
+        st = engine_init(&s->aux, &why);
     if (st != DOGM_OK)
     {
+        engine_release(&s->aux);
         engine_release(&s->eng);
         delete s;
         return st;
@@ -197,6 +202,7 @@ void dogm_server_destroy(dogm_server* s)
         for (auto& kv : s->veh)
             vehicle_free(kv.second.get());
         s->veh.clear();
+        engine_release(&s->aux);
         engine_release(&s->eng);
     }
     delete s;

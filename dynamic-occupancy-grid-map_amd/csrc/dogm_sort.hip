@@ -314,14 +314,16 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
         hipLaunchKernelGGL(k_sort_scan, dim3(RADIX, E->m), dim3(BLOCK), 0, E->stream, E->d_desc, E->sort_thist, tiles,
                            E->sort_ghist, ps);
         launch_end(E, K_SORT_SCAN);
-        launch_begin(E, K_SORT_DOWNSWEEP);
-        if (gather && ps == passes - 1)
+        // The gathering last pass is timed as the sort's gather (the standalone gather's work).
+        const bool g = gather && ps == passes - 1;
+        launch_begin(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
+        if (g)
             hipLaunchKernelGGL(k_sort_downsweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
                                pph, ps, tiles, E->sort_thist);
         else
             hipLaunchKernelGGL(k_sort_downsweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                E->tab(SP_P), pph, ps, tiles, E->sort_thist);
-        launch_end(E, K_SORT_DOWNSWEEP);
+        launch_end(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
     }
 }
 

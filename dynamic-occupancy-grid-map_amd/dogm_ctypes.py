@@ -137,6 +137,7 @@ _SIGS = {
     "dogm_batch_get_grid_cells": (_i, [_h, _i, _vp]),
     "dogm_batch_get_particles": (_i, [_h, _i, _vp, _vp, _vp, _vp]),
     "dogm_batch_get_resampled_indices": (_i, [_h, _i, _vp]),
+    "dogm_batch_set_lanes": (_i, [_h, _i]),
     "dogm_batch_set_extraction": (_i, [_h, _i, _f, _f]),
     "dogm_batch_get_occupancy": (_i, [_h, _i, _vp]),
     "dogm_batch_get_occupancy_image": (_i, [_h, _i, _vp]),
@@ -531,6 +532,9 @@ class Batch:
 
     def set_extraction(self, what: int, min_occupancy: float = 0.7, min_velocity: float = 4.0) -> None:
         self._check(lib.dogm_batch_set_extraction(self.h, what, min_occupancy, min_velocity), "setExtraction")
+
+    def set_lanes(self, lanes: int) -> None:
+        self._check(lib.dogm_batch_set_lanes(self.h, lanes), "setLanes")
 
     def get_occupancy(self, vid: int) -> np.ndarray:
         out = np.empty(self.grid_cell_count(vid), np.float32)

@@ -24,24 +24,26 @@ def short_name(name: str) -> str:
 
 
 def _trace_rows(path: str):
-    """(name, calls, total ns, average ns) per kernel from a kernel-trace database or the
-    kernel_stats.csv of `--output-format csv`."""
+    """(name, grid, calls, total ns, average ns) per kernel and grid size from a kernel-trace database
+    (grid None for the kernel_stats.csv of `--output-format csv`)."""
     if path.endswith(".csv"):
-        return [(r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]))
+        return [(r["Name"], None, int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]))
                 for r in csv.DictReader(open(path))]
     c = sqlite3.connect(path)
-    return [(n, int(k), float(t), float(t) / int(k)) for n, k, t in
-            c.execute("select name, count(*), sum(duration) from kernels group by name")]
+    return [(n, int(g), int(k), float(t), float(t) / int(k)) for n, g, k, t in
+            c.execute("select name, grid_x, count(*), sum(duration) from kernels group by name, grid_x")]
 
 
 def trace_summary(trace: str, title: str, command: str) -> str:
     rows = _trace_rows(trace)
-    total = sum(r[2] for r in rows)
+    total = sum(r[3] for r in rows)
     out = [f"# {title}", "", "Command (from the repository root, after `export TMPDIR=/tmp`):", "", f"    {command}", "",
-           "All dispatches of the run (warm-up and timed steps alike).", "",
-           "| kernel | calls | total us | avg us | % |", "|---|---:|---:|---:|---:|"]
-    for name, calls, t, avg in sorted(rows, key=lambda r: -r[2]):
-        out.append(f"| `{short_name(name)}` | {calls} | {t / 1e3:.1f} | {avg / 1e3:.2f} | {100 * t / total:.2f} |")
+           "All dispatches of the run (warm-up and timed steps alike), per kernel and grid size (work-items; a "
+           "launch's grid covers the vehicles of its step or lane).", "",
+           "| kernel | grid | calls | total us | avg us | % |", "|---|---:|---:|---:|---:|---:|"]
+    for name, grid, calls, t, avg in sorted(rows, key=lambda r: -r[3]):
+        out.append(f"| `{short_name(name)}` | {grid if grid is not None else ''} | {calls} | {t / 1e3:.1f} | "
+                   f"{avg / 1e3:.2f} | {100 * t / total:.2f} |")
     return "\n".join(out) + "\n"
 
 

@@ -285,6 +285,11 @@ int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, in
                              uint8_t* associated);
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
 int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
+/* Lanes of a step (1 or 2, default 2).  With 2, a step over two or more vehicles splits them over
+ * two engines with their own HIP streams, balanced by per-frame traffic; both lanes' launches run
+ * concurrently on the device, which fills one lane's small launches and tails with the other's
+ * work.  Results do not depend on the lane count. */
+int dogm_batch_set_lanes(dogm_batch* b, int lanes);
 /* Result extraction of the grid source inside every dogm_batch_update step, for all vehicles of
  * the step in the same launches; what = OR of DOGM_EXTRACT_* bits (0 = off).  The getters below
  * return the results of the vehicle's last step, or extract them on demand (with these
