@@ -159,7 +159,7 @@ def run_plain(args, p, meas_bufs, poses, dt):
             "dom_avg_s": dom_ms / max(1, dom_n) / 1e3, "launches_per_step": dom_n / args.steps}
 
 
-def run_batched(args, k, p, meas_bufs, poses, dt):
+def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
     """k vehicles of the same workload (Philox instances 0..k-1, the same frame sequence) per
     dogm_batch_update step."""
     from dogm_ctypes import Batch, DOGM, lib
@@ -211,9 +211,11 @@ def run_batched(args, k, p, meas_bufs, poses, dt):
     b.set_kernel_timing(dominant, False)
     b.set_lanes(2)
     extraction = run_extraction(args, b, ids, p, step, total) if args.extract_steps > 0 else None
+    generation = run_generation(args, b, ids, sc, total) if (args.generation_steps > 0 and sc is not None) else None
     b.close()
     return {"elapsed": elapsed, "frames": args.steps * k, "lat": lat, "per_frame": per_step, "dominant": dominant,
             "dom_avg_s": dom_ms / max(1, dom_n) / 1e3, "vehicles": k, "extraction": extraction,
+            "generation": generation,
             "launches_per_step": dom_n / args.steps,
             "roofline_leg": {"dom_avg_s": roof_ms / max(1, roof_n) / 1e3, "launches_per_step": roof_n / nroof,
                              "steps": nroof, "lanes": 1},
@@ -221,6 +223,40 @@ def run_batched(args, k, p, meas_bufs, poses, dt):
 
 
 EXTRACT_KERNELS = ("extract_mark", "extract_scan", "extract_compact")
+
+
+def run_generation(args, b, ids, sc, total):
+    """Measurement grids generated on the device from the scene's lidar ranges, straight into the
+    vehicles' measurement buffers (dogm_lidar_generate_many, one call per step for all vehicles):
+    grids per second and the algorithmic bytes of the three launches -- a 16-byte clear per cell,
+    a 4-byte free-mass store per free sample, an 8-byte store per return, and the ranges."""
+    from dogm_ctypes import Lidar, lib
+    cs, sn = sc.beam_directions()
+    c = sc.cfg
+    ranges = [np.stack([sc.ranges(f)] * len(ids)) for f in range(min(total, 8))]
+    outs = [b.measurement_buffer(v) for v in ids]
+    cells = c.grid_size ** 2
+    step = 0.5 * c.resolution
+    n_steps = int(np.ceil(c.max_range / step)) + 1
+    dist = np.arange(n_steps) * step
+    free_samples = sum(int(np.count_nonzero(dist[None, :] < np.minimum(r[0], c.max_range)[:, None]))
+                       for r in ranges) / len(ranges)
+    returns = sum(int(np.count_nonzero(np.isfinite(r[0]))) for r in ranges) / len(ranges)
+    per_grid = 16 * cells + 4 * free_samples + 8 * returns + 8 * c.n_beams
+    with Lidar(**sc.lidar_params(), beam_cos=cs, beam_sin=sn) as lidar:
+        for f in range(2):
+            lidar.generate_many(ranges[f % len(ranges)], outs)
+        lib.dogm_device_synchronize()
+        reps = args.generation_steps
+        t0 = time.perf_counter()
+        for f in range(reps):
+            lidar.generate_many(ranges[f % len(ranges)], outs)
+        el = time.perf_counter() - t0
+    grids = reps * len(ids)
+    return {"grids_per_s": round(grids / el, 1), "us_per_call": round(el / reps * 1e6, 2), "grids_per_call": len(ids),
+            "algorithmic_bytes_per_grid": int(per_grid),
+            "achieved_GBs_wall": round(per_grid * grids / el / 1e9, 1),
+            "note": "wall time of blocking calls (ranges upload, three launches, stream wait)"}
 
 
 def run_extraction(args, b, ids, p, step, first_frame):
@@ -538,6 +574,8 @@ def main() -> int:
     ap.add_argument("--sweep-seconds", type=float, default=10.0, help="simulated seconds per sweep point")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device-memory admission budget")
     ap.add_argument("--max-batch", type=int, default=0, help="frames per serving step (0: no limit)")
+    ap.add_argument("--generation-steps", type=int, default=20,
+                    help="device measurement-grid generations for the batch after the timed frames (0: skip)")
     ap.add_argument("--extract-steps", type=int, default=5,
                     help="batched steps with result extraction after the timed frames (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -592,7 +630,7 @@ def main() -> int:
     head = plain
     batched = None
     if args.batch > 0:
-        batched = run_batched(args, args.batch, p, bufs, poses, dt)
+        batched = run_batched(args, args.batch, p, bufs, poses, dt, sc)
         print(f"[bench] batched x{args.batch}: {_summary(batched)}", file=sys.stderr, flush=True)
         head = batched
 
@@ -652,6 +690,7 @@ def main() -> int:
         "batched": _summary(batched) if batched else None,
         "batched_vs_plain": round(fps / (plain["frames"] / plain["elapsed"]), 3) if batched else None,
         "extraction": batched.get("extraction") if batched else None,
+        "measurement_generation": batched.get("generation") if batched else None,
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),

@@ -44,6 +44,11 @@ class dogm_params(C.Structure):
     ]
 
 
+class dogm_lidar_params(C.Structure):
+    _fields_ = [("grid_size", C.c_int), ("beam_count", C.c_int), ("resolution", C.c_double),
+                ("max_range", C.c_double), ("occ_mass", C.c_float), ("free_mass", C.c_float)]
+
+
 class dogm_server_config(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_int)]
 
@@ -110,6 +115,11 @@ _SIGS = {
     "dogm_get_occupancy": (_i, [_h, _i, _vp]),
     "dogm_get_occupancy_image": (_i, [_h, _i, _vp]),
     "dogm_get_cells_with_velocity": (_i, [_h, _f, _f, _vp, _i, C.POINTER(_i)]),
+    "dogm_lidar_create": (_i, [C.POINTER(dogm_lidar_params), _vp, _vp, C.POINTER(_h)]),
+    "dogm_lidar_destroy": (None, [_h]),
+    "dogm_lidar_last_error": (C.c_char_p, [_h]),
+    "dogm_lidar_generate": (_i, [_h, _vp, _vp]),
+    "dogm_lidar_generate_many": (_i, [_h, _i, _vp, _vp]),
     "dogm_device_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dogm_device_free": (_i, [_vp]),
     "dogm_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),
@@ -722,3 +732,51 @@ def _as_particle_arrays(n, state, idx, weight, assoc):
     w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32).reshape(n)
     a = None if assoc is None else np.ascontiguousarray(assoc, dtype=np.uint8).reshape(n)
     return s, i, w, a
+
+
+class Lidar:
+    """Measurement-grid generator on the device (dogm_lidar_*): ranges in, measurement grids out."""
+
+    def __init__(self, grid_size: int, beam_count: int, resolution: float, max_range: float, occ_mass: float = 0.7,
+                 free_mass: float = 0.4, beam_cos=None, beam_sin=None):
+        self.params = dogm_lidar_params(grid_size, beam_count, resolution, max_range, occ_mass, free_mass)
+        self.grid_size = grid_size
+        self.beam_count = beam_count
+        cs = None if beam_cos is None else np.ascontiguousarray(beam_cos, np.float64)
+        sn = None if beam_sin is None else np.ascontiguousarray(beam_sin, np.float64)
+        h = C.c_void_p()
+        st = lib.dogm_lidar_create(C.byref(self.params), _ptr(cs), _ptr(sn), C.byref(h))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_lidar_create failed")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.dogm_lidar_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != DOGM_OK:
+            raise DogmError(st, f"{what}: {lib.dogm_lidar_last_error(self.h).decode()}")
+
+    def generate(self, ranges, device_out: int) -> None:
+        """One scan into a device buffer of grid_size^2 measurement cells (its address)."""
+        r = np.ascontiguousarray(ranges, np.float64)
+        assert r.size == self.beam_count
+        self._check(lib.dogm_lidar_generate(self.h, _ptr(r), C.c_void_p(device_out)), "generate")
+
+    def generate_many(self, ranges, device_outs) -> None:
+        r = np.ascontiguousarray(ranges, np.float64).reshape(-1)
+        n = len(device_outs)
+        assert r.size == n * self.beam_count
+        outs = (C.c_void_p * n)(*device_outs)
+        self._check(lib.dogm_lidar_generate_many(self.h, n, _ptr(r), C.cast(outs, C.c_void_p)), "generateMany")

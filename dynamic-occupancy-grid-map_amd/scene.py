@@ -80,6 +80,17 @@ class Scene:
         ends = np.roll(corners, -1, axis=1).reshape(-1, 2)
         return starts, ends - starts
 
+    def beam_directions(self) -> tuple[np.ndarray, np.ndarray]:
+        """(cos, sin) of every beam angle 2 pi b / n_beams, as the generators here use them."""
+        ang = 2.0 * np.pi * np.arange(self.cfg.n_beams) / self.cfg.n_beams
+        return np.cos(ang), np.sin(ang)
+
+    def lidar_params(self) -> dict:
+        """Arguments of the device generator (dogm_lidar_params) for this scene's sensor."""
+        c = self.cfg
+        return dict(grid_size=c.grid_size, beam_count=c.n_beams, resolution=c.resolution, max_range=c.max_range,
+                    occ_mass=c.occ_mass, free_mass=c.free_mass)
+
     def ranges(self, frame: int) -> np.ndarray:
         """Distance to the first hit per beam (inf when nothing is hit within max_range)."""
         c = self.cfg

@@ -203,6 +203,45 @@ int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_
                                  int capacity, int* count);
 
 /* ----------------------------------------------------------------------------------------------
+ * Measurement-grid generation on the device (the step before updateGrid).  Replaces the reference's
+ * LaserMeasurementGrid::generateGrid (dogm/demo/simulator/mapping/laser_to_meas_grid.cu:25-70), a polar
+ * inverse-sensor texture rendered to a Cartesian grid with OpenGL, by a direct ray cast with a
+ * two-level inverse sensor model:
+ *   free_mass on every cell a beam samples before its return (samples every resolution / 2 metres
+ *   from the sensor, distance < min(range, max_range)); occ_mass and free 0 on the cell of a return
+ *   (range <= max_range); unknown (0, 0) elsewhere; likelihood = p_A = 1 (measurement_grid.cu:129-130).
+ * The sensor sits at the grid centre, 0.5 * grid_size * resolution metres along both axes; a
+ * sample at (x, y) metres lies in cell floor(x / resolution) + grid_size * floor(y / resolution).
+ * Positions are f64 with that operation order, so grids are reproducible bit for bit.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct dogm_lidar_params
+{
+    int grid_size;     /* cells per side of the generated grids */
+    int beam_count;
+    double resolution; /* [m/cell] */
+    double max_range;  /* [m] */
+    float occ_mass;    /* occupied mass at a return (0.7 in the benchmark scenes) */
+    float free_mass;   /* free mass along a beam (0.4) */
+} dogm_lidar_params;
+
+typedef struct dogm_lidar dogm_lidar;
+
+/* beam_cos / beam_sin: unit direction of each beam (both NULL: 360 degrees, beam b at
+ * 2 pi b / beam_count, directions from the host's cos / sin in double). */
+int dogm_lidar_create(const dogm_lidar_params* params, const double* beam_cos, const double* beam_sin,
+                      dogm_lidar** out);
+void dogm_lidar_destroy(dogm_lidar* l);
+const char* dogm_lidar_last_error(const dogm_lidar* l);
+/* ranges: beam_count distances [m] in host memory (inf or NaN: no return).  device_out: grid_size^2
+ * cells of device memory, e.g. a vehicle's measurement buffer (dogm_batch_measurement_buffer).
+ * Blocking. */
+int dogm_lidar_generate(dogm_lidar* l, const double* ranges, dogm_measurement_cell* device_out);
+/* count grids in the same launches: ranges holds count * beam_count distances, grid i from
+ * ranges + i * beam_count into device_out[i]. */
+int dogm_lidar_generate_many(dogm_lidar* l, int count, const double* ranges,
+                             dogm_measurement_cell* const* device_out);
+
+/* ----------------------------------------------------------------------------------------------
  * Device-memory helpers for callers that pass device measurement grids (the reference's callers use
  * cudaMalloc / cudaMemcpy for this, e.g. dogm/demo/simulator/mapping/laser_to_meas_grid.cu:15).
  * kind: 0 host->device, 1 device->host, 2 device->device.

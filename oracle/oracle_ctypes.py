@@ -66,6 +66,7 @@ def lib():
             "oracle_occupancy": (None, [vp, vp, i, vp, vp]),
             "oracle_cells_with_velocity": (i, [i] + [vp] * 7 + [f, f, vp]),
             "oracle_velocity_normalized": (f, [f, f, f, f, f]),
+            "oracle_lidar_grid": (None, [i, i, C.c_double, C.c_double, f, f, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -359,3 +360,16 @@ def cells_with_velocity(grid_size: int, cells, min_occupancy: float, min_velocit
 def velocity_normalized(mean_x: float, mean_y: float, var_x: float, var_y: float, covar: float) -> float:
     """v^T S^-1 v of one cell as the demo evaluates it (image_creation.cpp:35-50)."""
     return lib().oracle_velocity_normalized(mean_x, mean_y, var_x, var_y, covar)
+
+
+def lidar_grid(grid_size: int, resolution: float, max_range: float, occ_mass: float, free_mass: float,
+               beam_cos, beam_sin, ranges) -> np.ndarray:
+    """Measurement grid of one scan (structured array with the dogm_measurement_cell fields)."""
+    cs = np.ascontiguousarray(beam_cos, np.float64)
+    sn = np.ascontiguousarray(beam_sin, np.float64)
+    r = np.ascontiguousarray(ranges, np.float64)
+    out = np.zeros(grid_size * grid_size, dtype=[("free_mass", "<f4"), ("occ_mass", "<f4"), ("likelihood", "<f4"),
+                                                 ("p_A", "<f4")])
+    lib().oracle_lidar_grid(grid_size, cs.size, resolution, max_range, occ_mass, free_mass, cs.ctypes.data,
+                            sn.ctypes.data, r.ctypes.data, out.ctypes.data)
+    return out

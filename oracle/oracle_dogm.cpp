@@ -1068,4 +1068,50 @@ float oracle_velocity_normalized(float mx, float my, float vxx, float vyy, float
     return velocity_normalized_by_variance(mx, my, vxx, vyy, cxy);
 }
 
+// Measurement grid of one lidar scan: the two-level ray cast that dogm_lidar_generate runs on the
+// device (the step before updateGrid; the reference's LaserMeasurementGrid::generateGrid,
+// dogm/demo/simulator/mapping/laser_to_meas_grid.cu:25-70).  Sensor at the grid centre; samples every
+// resolution / 2 metres while the distance is below min(range, max_range) mark free_mass; the cell of
+// a return within max_range gets occ_mass and free 0; likelihood = p_A = 1 (measurement_grid.cu:129-130).
+void oracle_lidar_grid(int gs, int beams, double res, double max_range, float occ_mass, float free_mass,
+                       const double* cosv, const double* sinv, const double* ranges, dogm_measurement_cell* out)
+{
+    const size_t cells = (size_t)gs * (size_t)gs;
+    std::vector<unsigned char> is_free(cells, 0), is_occ(cells, 0);
+    const double centre = 0.5 * gs * res;
+    const double step = 0.5 * res;
+    const int steps = (int)std::ceil(max_range / step) + 1;
+    auto cell_of = [&](double x, double y, size_t* c) {
+        const double fx = std::floor(x / res), fy = std::floor(y / res);
+        if (!(fx >= 0.0 && fx < (double)gs && fy >= 0.0 && fy < (double)gs))
+            return false;
+        *c = (size_t)fx + (size_t)gs * (size_t)fy;
+        return true;
+    };
+    for (int b = 0; b < beams; ++b)
+    {
+        const double r = ranges[b];
+        const double lim = std::isnan(r) ? r : std::min(r, max_range);
+        for (int k = 0; k < steps; ++k)
+        {
+            const double dist = k * step;
+            if (!(dist < lim))
+                break;
+            size_t c;
+            if (cell_of(centre + cosv[b] * dist, centre + sinv[b] * dist, &c))
+                is_free[c] = 1;
+        }
+        size_t c;
+        if (r <= max_range && cell_of(centre + cosv[b] * r, centre + sinv[b] * r, &c))
+            is_occ[c] = 1;
+    }
+    for (size_t c = 0; c < cells; ++c)
+    {
+        out[c].free_mass = (is_free[c] && !is_occ[c]) ? free_mass : 0.0f;
+        out[c].occ_mass = is_occ[c] ? occ_mass : 0.0f;
+        out[c].likelihood = 1.0f;
+        out[c].p_A = 1.0f;
+    }
+}
+
 }  // extern "C"
