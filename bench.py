@@ -330,8 +330,9 @@ def run_serving(args) -> int:
     if args.serve_vehicle == "fleet":
         fleet = fl.make_fleet(args.fleet, seed=args.seed)
         t0 = time.perf_counter()
-        bank = fl.measurement_bank([v.side for v in fleet])
-        say(f"measurement bank for sides {sorted(bank)} in {time.perf_counter() - t0:.1f} s")
+        sides = [v.side for v in fleet]
+        bank = fl.ScanBank(sides) if args.serve_input == "scan" else fl.measurement_bank(sides)
+        say(f"{args.serve_input} bank for sides {sorted(set(sides))} in {time.perf_counter() - t0:.1f} s")
         label = (f"{args.fleet} vehicles, grid sides drawn from {{128, 256, 512, 768, 1024}} at 0.1 m, N = 1.5 x cells, "
                  "B = N/10")
     else:
@@ -341,13 +342,16 @@ def run_serving(args) -> int:
         for v in fleet:
             v.params = p_full
         sc = scene.make_scene("plain_full", seed=args.seed)
-        bufs = []
-        for f in range(4):
-            m = np.ascontiguousarray(sc.measurement(f))
-            d = DeviceBuffer(m.nbytes)
-            d.upload(m)
-            bufs.append(d)
-        bank = {1200: bufs}
+        if args.serve_input == "scan":
+            bank = fl.ScanBank([1200], scenes={1200: sc})
+        else:
+            bufs = []
+            for f in range(4):
+                m = np.ascontiguousarray(sc.measurement(f))
+                d = DeviceBuffer(m.nbytes)
+                d.upload(m)
+                bufs.append(d)
+            bank = {1200: bufs}
         label = f"{args.fleet} vehicles at 1200x1200 cells, 2e6 particles + 2e5 birth each"
     budget = int(args.budget_gb * (1 << 30))
     srv = Server(hbm_budget_bytes=budget, max_batch=args.max_batch)
@@ -364,7 +368,10 @@ def run_serving(args) -> int:
     # Warm-up: two frames per vehicle (the first initialises its particles), not measured.
     for k in range(2):
         for j, v in enumerate(fleet_adm):
-            srv.submit(vids_adm[j], bank[v.side][k % len(bank[v.side])], v.pose(k), 0.1, 0.0)
+            if isinstance(bank, fl.ScanBank):
+                bank.submit(srv, vids_adm[j], v.side, k, v.pose(k), 0.1, 0.0)
+            else:
+                srv.submit(vids_adm[j], bank[v.side][k % len(bank[v.side])], v.pose(k), 0.1, 0.0)
         while srv.queued():
             srv.step(0.0)
         srv.poll(1 << 16)
@@ -393,6 +400,8 @@ def run_serving(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": label, "arrivals": "10 Hz per vehicle, +-20 ms jitter, random phase",
+                   "input": ("lidar scans (720 or 1080 beams); measurement grids generated on the device inside each "
+                             "step" if args.serve_input == "scan" else "device-resident measurement grids"),
                    "hbm_budget_gb": args.budget_gb, "max_batch": args.max_batch,
                    "clock": "virtual time; every step's duration is the measured GPU step"},
         "admitted": len(admitted), "rejected": rejected,
@@ -403,9 +412,12 @@ def run_serving(args) -> int:
     }
     print(json.dumps(out))
     srv.close()
-    for bl in bank.values():
-        for d in bl:
-            d.free()
+    if isinstance(bank, fl.ScanBank):
+        bank.close()
+    else:
+        for bl in bank.values():
+            for d in bl:
+                d.free()
     return 0
 
 
@@ -573,6 +585,8 @@ def main() -> int:
     ap.add_argument("--sweep", default="1,2,4,8", help="arrival-rate multipliers (the first is the nominal run)")
     ap.add_argument("--sweep-seconds", type=float, default=10.0, help="simulated seconds per sweep point")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device-memory admission budget")
+    ap.add_argument("--serve-input", default="scan", choices=["scan", "grid"],
+                    help="serving frames as lidar scans (grids generated on the device) or as ready grids")
     ap.add_argument("--max-batch", type=int, default=0, help="frames per serving step (0: no limit)")
     ap.add_argument("--generation-steps", type=int, default=20,
                     help="device measurement-grid generations for the batch after the timed frames (0: skip)")

@@ -386,6 +386,10 @@ int engine_frame_lanes(Engine* const* lanes, int nl, Vehicle* const* veh, const 
                        bool extract = false);
 // Allocates the extraction buffers `what` needs on a vehicle.
 int vehicle_ext_alloc(Vehicle* v, int what, std::string* why);
+// Lidar generator internals used by the server (dogm_lidar.hip).
+int lidar_grid_size(const dogm_lidar* l);
+int lidar_beam_count(const dogm_lidar* l);
+
 // Velocity records are packed and read back in chunks of at most this many.
 constexpr int EXTRACT_REC_CHUNK = 1 << 16;
 // Device bytes the extraction buffers for `what` take on a vehicle of C cells (record staging included).

@@ -112,6 +112,16 @@ __global__ __launch_bounds__(BLOCK) void k_lidar_occ(dogm_measurement_cell* cons
     *reinterpret_cast<float2*>(&outs[blockIdx.y][c].free_mass) = make_float2(0.0f, a.occ_mass);
 }
 
+int lidar_grid_size(const dogm_lidar* l)
+{
+    return l->p.grid_size;
+}
+
+int lidar_beam_count(const dogm_lidar* l)
+{
+    return l->p.beam_count;
+}
+
 }  // namespace dogm_hip
 
 using namespace dogm_hip;

@@ -46,6 +46,8 @@ struct dogm_server
         int vid;
         FrameIn in;
         double t_arrival;
+        dogm_lidar* lidar;           // frames submitted as a scan: the generator of their grid
+        std::vector<double> ranges;  // the scan (beam_count distances)
     };
     std::deque<Frame> queue;
     std::vector<dogm_frame_result> done;
@@ -93,6 +95,40 @@ std::vector<dogm_server::Frame> take_frames(dogm_server* s)
     return out;
 }
 
+// Generates the measurement grids of the step's scan frames into their vehicles' measurement
+// buffers, one generate_many call per generator.
+int generate_scans(dogm_server* s, const std::vector<dogm_server::Frame>& fr, const std::vector<Vehicle*>& list)
+{
+    std::vector<dogm_lidar*> gens;
+    for (const auto& f : fr)
+        if (f.lidar && std::find(gens.begin(), gens.end(), f.lidar) == gens.end())
+            gens.push_back(f.lidar);
+    for (dogm_lidar* l : gens)
+    {
+        const int beams = lidar_beam_count(l);
+        std::vector<double> ranges;
+        std::vector<dogm_measurement_cell*> outs;
+        for (size_t i = 0; i < fr.size(); ++i)
+        {
+            if (fr[i].lidar != l)
+                continue;
+            ranges.insert(ranges.end(), fr[i].ranges.begin(), fr[i].ranges.begin() + beams);
+            outs.push_back(list[i]->meas);
+        }
+        for (size_t o = 0; o < outs.size(); o += 65535)
+        {
+            const int n = (int)std::min<size_t>(65535, outs.size() - o);
+            const int st = dogm_lidar_generate_many(l, n, ranges.data() + o * (size_t)beams, outs.data() + o);
+            if (st != DOGM_OK)
+            {
+                s->eng.last_error = std::string("generating measurement grids: ") + dogm_lidar_last_error(l);
+                return st;
+            }
+        }
+    }
+    return DOGM_OK;
+}
+
 // Runs one step over `fr`; wall duration in *secs.
 int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* secs)
 {
@@ -108,6 +144,14 @@ int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* 
     }
     std::lock_guard<std::mutex> ge(s->eng_mu);
     const auto a = std::chrono::steady_clock::now();
+    const int gst = generate_scans(s, fr, list);
+    if (gst != DOGM_OK)
+    {
+        *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        std::lock_guard<std::mutex> g(s->mu);
+        s->last_error = s->eng.last_error;
+        return gst;
+    }
     s->eng.ext = s->ext;
     Engine* lanes[2] = {&s->eng, &s->aux};
     const int st = engine_frame_lanes(lanes, 2, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
@@ -320,7 +364,40 @@ int dogm_server_submit(dogm_server* s, int vehicle_id, const dogm_measurement_ce
         return DOGM_ERR_INVALID_ARGUMENT;
     }
     const uint64_t t = s->next_ticket++;
-    s->queue.push_back(dogm_server::Frame{t, vehicle_id, FrameIn{meas, is_device, x, y, yaw, dt}, t_arrival});
+    s->queue.push_back(
+        dogm_server::Frame{t, vehicle_id, FrameIn{meas, is_device, x, y, yaw, dt}, t_arrival, nullptr, {}});
+    s->inflight[vehicle_id] += 1;
+    if (ticket)
+        *ticket = t;
+    s->cv_work.notify_one();
+    return DOGM_OK;
+}
+
+int dogm_server_submit_scan(dogm_server* s, int vehicle_id, dogm_lidar* lidar, const double* ranges, float x,
+                            float y, float yaw, float dt, double t_arrival, uint64_t* ticket)
+{
+    if (!s)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    if (!lidar || !ranges)
+        return sfail(s, DOGM_ERR_INVALID_ARGUMENT, "a scan needs a generator and its ranges");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->veh.find(vehicle_id);
+    if (it == s->veh.end())
+    {
+        s->last_error = "unknown vehicle id " + std::to_string(vehicle_id);
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    if (lidar_grid_size(lidar) != it->second->gs)
+    {
+        s->last_error = "the generator's grid size " + std::to_string(lidar_grid_size(lidar)) +
+                        " differs from the vehicle's " + std::to_string(it->second->gs);
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t t = s->next_ticket++;
+    const int beams = lidar_beam_count(lidar);
+    // The grid is generated into the vehicle's measurement buffer at the start of the step.
+    s->queue.push_back(dogm_server::Frame{t, vehicle_id, FrameIn{nullptr, 1, x, y, yaw, dt}, t_arrival, lidar,
+                                          std::vector<double>(ranges, ranges + beams)});
     s->inflight[vehicle_id] += 1;
     if (ticket)
         *ticket = t;

@@ -163,6 +163,7 @@ _SIGS = {
     "dogm_server_release": (_i, [_h, _i]),
     "dogm_server_resident_bytes": (C.c_uint64, [_h]),
     "dogm_server_submit": (_i, [_h, _i, _vp, _i, _f, _f, _f, _f, C.c_double, C.POINTER(C.c_uint64)]),
+    "dogm_server_submit_scan": (_i, [_h, _i, _h, _vp, _f, _f, _f, _f, C.c_double, C.POINTER(C.c_uint64)]),
     "dogm_server_queued": (_i, [_h]),
     "dogm_server_step": (_i, [_h, C.c_double, C.POINTER(_i), C.POINTER(C.c_double)]),
     "dogm_server_start": (_i, [_h]),
@@ -640,6 +641,16 @@ class Server:
             st = lib.dogm_server_submit(self.h, vid, _ptr(keep), 0, x, y, yaw, dt, t_arrival, C.byref(t))
         self._check(st, "submit")
         self._keep[t.value] = keep
+        return t.value
+
+    def submit_scan(self, vid: int, lidar: "Lidar", ranges, pose, dt: float, t_arrival: float) -> int:
+        """A frame given as lidar ranges; its grid is generated on the device inside the step."""
+        x, y, yaw = pose
+        r = np.ascontiguousarray(ranges, np.float64)
+        assert r.size == lidar.beam_count
+        t = C.c_uint64()
+        self._check(lib.dogm_server_submit_scan(self.h, vid, lidar.h, _ptr(r), x, y, yaw, dt, t_arrival, C.byref(t)),
+                    "submitScan")
         return t.value
 
     def queued(self) -> int:

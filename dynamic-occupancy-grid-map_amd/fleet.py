@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 import scene as scene_mod
-from dogm_ctypes import DeviceBuffer, Params
+from dogm_ctypes import DeviceBuffer, Lidar, Params
 
 SIDES = (128, 256, 512, 768, 1024)
 RESOLUTION = 0.1
@@ -60,15 +60,41 @@ def make_fleet(count: int, seed: int = 7, sides=SIDES) -> list[FleetVehicle]:
     return out
 
 
+def side_scene(side: int, seed: int = 3) -> scene_mod.Scene:
+    """The scene (15 moving boxes, 720-beam lidar reaching the grid edge) of a grid side."""
+    cfg = scene_mod.SceneConfig(grid_size=side, resolution=RESOLUTION, n_boxes=15, n_beams=720,
+                                max_range=0.5 * side * RESOLUTION, ego_speed=10.0, ego_yaw_rate=0.1, seed=seed + side)
+    return scene_mod.Scene(cfg)
+
+
+class ScanBank:
+    """A few lidar scans per grid side and one device generator per side: frames are submitted as
+    scans and their grids generated on the device inside the serving step."""
+
+    def __init__(self, sides, frames: int = 4, seed: int = 3, scenes=None):
+        self.lidar = {}
+        self.ranges = {}
+        for side in sorted(set(sides)):
+            sc = scenes[side] if scenes else side_scene(side, seed)
+            cs, sn = sc.beam_directions()
+            self.lidar[side] = Lidar(**sc.lidar_params(), beam_cos=cs, beam_sin=sn)
+            self.ranges[side] = [sc.ranges(f) for f in range(frames)]
+
+    def submit(self, server, vid, side, k, pose, dt, t_arrival):
+        r = self.ranges[side]
+        server.submit_scan(vid, self.lidar[side], r[k % len(r)], pose, dt, t_arrival)
+
+    def close(self):
+        for lid in self.lidar.values():
+            lid.close()
+
+
 def measurement_bank(sides, frames: int = 4, seed: int = 3) -> dict[int, list[DeviceBuffer]]:
     """A few device-resident measurement grids per grid side (720-beam lidar, moving boxes); the
     vehicles of one side cycle through them."""
     bank = {}
     for side in sorted(set(sides)):
-        cfg = scene_mod.SceneConfig(grid_size=side, resolution=RESOLUTION, n_boxes=15, n_beams=720,
-                                    max_range=0.5 * side * RESOLUTION, ego_speed=10.0, ego_yaw_rate=0.1,
-                                    seed=seed + side)
-        sc = scene_mod.Scene(cfg)
+        sc = side_scene(side, seed)
         bufs = []
         for f in range(frames):
             m = np.ascontiguousarray(sc.measurement(f))
@@ -111,7 +137,10 @@ def simulate(server, fleet, vids, bank, events, dt: float = 0.1, progress=None) 
         while i < n_ev and events[i][0] <= t:
             ta, v, k = events[i]
             fv = fleet[v]
-            server.submit(vids[v], bank[fv.side][k % len(bank[fv.side])], fv.pose(k), dt, ta)
+            if isinstance(bank, ScanBank):
+                bank.submit(server, vids[v], fv.side, k, fv.pose(k), dt, ta)
+            else:
+                server.submit(vids[v], bank[fv.side][k % len(bank[fv.side])], fv.pose(k), dt, ta)
             i += 1
         if not server.queued():
             t = events[i][0]

@@ -386,6 +386,13 @@ int dogm_server_release(dogm_server* s, int vehicle_id);
 uint64_t dogm_server_resident_bytes(const dogm_server* s);
 int dogm_server_submit(dogm_server* s, int vehicle_id, const dogm_measurement_cell* meas, int is_device, float x,
                        float y, float yaw, float dt, double t_arrival, uint64_t* ticket);
+/* A frame given as a lidar scan: at the start of its step the server generates the vehicle's
+ * measurement grid from `ranges` (beam_count distances, copied at submission) with `lidar`
+ * (dogm_lidar_generate_many over every scan frame of the step), then runs the update.  The
+ * generator's grid_size must equal the vehicle's; the server uses it during steps, so do not call
+ * it elsewhere while the server may step. */
+int dogm_server_submit_scan(dogm_server* s, int vehicle_id, dogm_lidar* lidar, const double* ranges, float x,
+                            float y, float yaw, float dt, double t_arrival, uint64_t* ticket);
 int dogm_server_queued(const dogm_server* s);
 /* One step on the caller's thread: start = now, done = now + the step's measured duration. */
 int dogm_server_step(dogm_server* s, double now, int* frames_run, double* step_seconds);

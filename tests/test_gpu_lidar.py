@@ -100,3 +100,34 @@ def test_batch_fed_from_device_generated_grids():
                                            host_batch.get_particles(hids[k]))
                     assert_grid_equal(f"frame {f} vehicle {k}", gen_batch.get_grid_cells(gids[k]),
                                       host_batch.get_grid_cells(hids[k]))
+
+
+def test_server_scan_frames_match_plain_handles():
+    """Frames submitted as scans (grids generated inside the serving step) give the same filter state
+    as plain handles fed the numpy generator's grids."""
+    from dogm_ctypes import DOGM, Server, vehicle_footprint
+    vs = [(small_params(gs=gs, res=0.5, n=6000, b=600), small_scene(gs=gs, res=0.5, seed=60 + k))
+          for k, gs in enumerate((40, 56, 40))]
+    lidars = {}
+    for _, sc in vs:
+        if sc.cfg.grid_size not in lidars:
+            cs, sn = sc.beam_directions()
+            lidars[sc.cfg.grid_size] = Lidar(**sc.lidar_params(), beam_cos=cs, beam_sin=sn)
+    budget = sum(vehicle_footprint(p) for p, _ in vs) + (16 << 20)
+    with Server(budget) as srv:
+        ids = [srv.admit(p, 5, 70 + k) for k, (p, _) in enumerate(vs)]
+        plains = [DOGM(p, seed=5, instance=70 + k) for k, (p, _) in enumerate(vs)]
+        for f in range(3):
+            _say(f"server scan frames, frame {f}")
+            for k, (_, sc) in enumerate(vs):
+                srv.submit_scan(ids[k], lidars[sc.cfg.grid_size], sc.ranges(f), sc.ego_pose(f), 0.1, srv.clock())
+            while srv.queued():
+                srv.step(srv.clock())
+            assert all(r["status"] == 0 for r in srv.poll())
+            for k, (_, sc) in enumerate(vs):
+                x, y, yaw = sc.ego_pose(f)
+                plains[k].update_grid(sc.measurement(f), x, y, yaw, 0.1)
+                assert_particles_equal(f"frame {f} vehicle {k}", srv.get_particles(ids[k]), plains[k].get_particles())
+                assert_grid_equal(f"frame {f} vehicle {k}", srv.get_grid_cells(ids[k]), plains[k].get_grid_cells())
+    for lid in lidars.values():
+        lid.close()
