@@ -421,6 +421,26 @@ def run_serving(args) -> int:
     return 0
 
 
+def run_accuracy(args) -> int:
+    """Tracking accuracy (accuracy.run) of the plain filter in each CDF mode and of a batch of
+    vehicles, against the scene's ground-truth boxes; one JSON line."""
+    import accuracy
+    say = lambda m: print(f"[accuracy] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    rows = []
+    for form, mode, veh in (("plain", "exact", 1), ("plain", "fp64", 1), ("plain", "fp32", 1),
+                            ("batched", "exact", args.batch)):
+        t0 = time.perf_counter()
+        r = accuracy.run(args.workload, args.accuracy_frames, form=form, cdf_mode=mode, vehicles=veh)
+        r.pop("per_frame")
+        r["vehicles"] = veh
+        r["seconds"] = round(time.perf_counter() - t0, 2)
+        say(json.dumps(r))
+        rows.append(r)
+    print(json.dumps({"metric": "tracking accuracy: MAE / RMSE of cluster position [m] and velocity [m/s] against "
+                                "ground-truth boxes", "workload": WORKLOAD_TEXT[args.workload], "rows": rows}))
+    return 0
+
+
 # Every kernel a resampling stage launches (CDF tile totals and top scan, float-mode CDF, spacings, merge).
 RESAMPLING_KERNELS = ("scan_tiles", "scan_top", "cdf_write", "exp_tiles", "rs_merge")
 
@@ -576,7 +596,8 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision-frames", type=int, default=30, help="frames of the precision form")
-    ap.add_argument("--form", default="both", choices=["both", "plain", "serving", "precision"],
+    ap.add_argument("--accuracy-frames", type=int, default=None, help="frames of the accuracy form (default: workload's)")
+    ap.add_argument("--form", default="both", choices=["both", "plain", "serving", "precision", "accuracy"],
                     help="both: plain + batched (default); serving: continuous batching of a vehicle fleet")
     ap.add_argument("--serve-vehicle", default="fleet", choices=["fleet", "full"],
                     help="fleet: mixed grid sides; full: every vehicle at 1200x1200 / 2e6 particles")
@@ -605,6 +626,8 @@ def main() -> int:
         return run_serving(args)
     if args.form == "precision":
         return run_precision(args)
+    if args.form == "accuracy":
+        return run_accuracy(args)
     if args.form == "plain":
         args.batch = 0
 

@@ -1,0 +1,80 @@
+"""Tracking accuracy of the filter against the scenes' ground truth (the reference demo's
+evaluation loop, dogm/demo/main.cpp:82-101 with utils/precision_evaluator.cpp:37-122).
+
+Per frame: updateGrid, the cells passing the velocity test (occupancy >= 0.7, v^T S^-1 v >= 4,
+main.cpp:51-52, extracted on the device), DBSCAN (eps 3 cells, 5 cells) and the cluster means in
+metres, each assigned to the closest ground-truth box within 5 m; MAE and RMSE of position and
+velocity over all assigned clusters.  Frames before `skip` (the filter's warm-up) are not scored.
+
+    python dynamic-occupancy-grid-map_amd/accuracy.py [--workload plain_small] [--frames 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+import scene as scene_mod
+from dogm_ctypes import DOGM, EXTRACT_VELOCITY, Batch, Evaluator, Params
+
+MIN_OCC, MIN_VEL = 0.7, 4.0
+
+
+def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5, seed: int = 1,
+        form: str = "plain", cdf_mode: str = "exact", vehicles: int = 1) -> dict:
+    """form "plain": one handle; "batched": `vehicles` vehicles of the same scene (instances 0..)
+    stepped together, all scored."""
+    w = scene_mod.WORKLOADS[workload]
+    p = Params(**w["params"])
+    sc = scene_mod.make_scene(workload, seed=seed)
+    frames = frames or w["frames"]
+    dt = float(w["dt"])
+    ev = Evaluator(p.resolution)
+    per_frame = []
+    if form == "plain":
+        g = DOGM(p, seed=123456)
+        g.set_cdf_mode(cdf_mode)
+        for f in range(frames):
+            x, y, yaw = sc.ego_pose(f)
+            g.update_grid(sc.measurement(f), x, y, yaw, dt)
+            if f >= skip:
+                cells, _ = g.get_cells_with_velocity(MIN_OCC, MIN_VEL)
+                cl = ev.add_step(cells, sc.objects(f))
+                per_frame.append({"frame": f, "cells": int(cells.size), "clusters": int(cl.size)})
+        g.close()
+    else:
+        b = Batch()
+        ids = [b.add_vehicle(p, 123456, i) for i in range(vehicles)]
+        b.set_extraction(EXTRACT_VELOCITY, MIN_OCC, MIN_VEL)  # inside every step, for all vehicles
+        for vid in ids:
+            b.set_cdf_mode(vid, cdf_mode)
+        for f in range(frames):
+            m = sc.measurement(f)
+            b.update(ids, [m] * vehicles, [sc.ego_pose(f)] * vehicles, dt)
+            if f >= skip:
+                for vid in ids:
+                    cells, _ = b.get_cells_with_velocity(vid)
+                    cl = ev.add_step(cells, sc.objects(f))
+                    per_frame.append({"frame": f, "vehicle": vid, "cells": int(cells.size), "clusters": int(cl.size)})
+        b.close()
+    s = ev.summary()
+    ev.close()
+    s.update({"workload": workload, "form": form, "cdf_mode": cdf_mode, "frames": frames, "scored_from": skip,
+              "per_frame": per_frame})
+    return s
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="plain_small", choices=sorted(scene_mod.WORKLOADS))
+    ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--skip", type=int, default=5)
+    ap.add_argument("--form", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--cdf-mode", default="exact", choices=["exact", "fp64", "fp32"])
+    ap.add_argument("--vehicles", type=int, default=1)
+    a = ap.parse_args()
+    r = run(a.workload, a.frames, a.skip, form=a.form, cdf_mode=a.cdf_mode, vehicles=a.vehicles)
+    print(json.dumps(r))
+    sys.exit(0)

@@ -68,6 +68,8 @@ MEAS_CELL_DTYPE = np.dtype([("free_mass", "<f4"), ("occ_mass", "<f4"), ("likelih
 VELOCITY_CELL_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("data", GRID_CELL_DTYPE)])
 assert GRID_CELL_DTYPE.itemsize == 64 and MEAS_CELL_DTYPE.itemsize == 16 and VELOCITY_CELL_DTYPE.itemsize == 72
 
+CLUSTER_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("cells", "<i4")])
+OBJECT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("vx", "<f4"), ("vy", "<f4")])
 SOURCES = {"grid": 0, "measurement": 1}
 EXTRACT_OCCUPANCY, EXTRACT_IMAGE, EXTRACT_VELOCITY = 1, 2, 4
 
@@ -115,6 +117,12 @@ _SIGS = {
     "dogm_get_occupancy": (_i, [_h, _i, _vp]),
     "dogm_get_occupancy_image": (_i, [_h, _i, _vp]),
     "dogm_get_cells_with_velocity": (_i, [_h, _f, _f, _vp, _i, C.POINTER(_i)]),
+    "dogm_dbscan": (_i, [_vp, _i, _f, _i, _vp, C.POINTER(_i)]),
+    "dogm_cluster_means": (_i, [_vp, _i, _vp, _i, _vp]),
+    "dogm_evaluator_create": (_i, [_f, _f, _f, _i, C.POINTER(_h)]),
+    "dogm_evaluator_destroy": (None, [_h]),
+    "dogm_evaluator_add_step": (_i, [_h, _vp, _i, _vp, _i, _vp, _i, C.POINTER(_i)]),
+    "dogm_evaluator_summary": (_i, [_h, _vp, _vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "dogm_lidar_create": (_i, [C.POINTER(dogm_lidar_params), _vp, _vp, C.POINTER(_h)]),
     "dogm_lidar_destroy": (None, [_h]),
     "dogm_lidar_last_error": (C.c_char_p, [_h]),
@@ -791,3 +799,64 @@ class Lidar:
         assert r.size == n * self.beam_count
         outs = (C.c_void_p * n)(*device_outs)
         self._check(lib.dogm_lidar_generate_many(self.h, n, _ptr(r), C.cast(outs, C.c_void_p)), "generateMany")
+
+
+def dbscan(cells: np.ndarray, eps: float = 3.0, min_cells: int = 5) -> tuple[np.ndarray, int]:
+    """(cluster id per cell, -1 = noise; number of clusters) of velocity-cell records."""
+    c = np.ascontiguousarray(cells, dtype=VELOCITY_CELL_DTYPE)
+    ids = np.empty(max(c.size, 1), np.int32)
+    k = C.c_int()
+    st = lib.dogm_dbscan(_ptr(c), c.size, eps, min_cells, _ptr(ids), C.byref(k))
+    if st != DOGM_OK:
+        raise DogmError(st, "dogm_dbscan")
+    return ids[:c.size].copy(), k.value
+
+
+def cluster_means(cells: np.ndarray, ids: np.ndarray, clusters: int) -> np.ndarray:
+    c = np.ascontiguousarray(cells, dtype=VELOCITY_CELL_DTYPE)
+    i = np.ascontiguousarray(ids, dtype=np.int32)
+    out = np.zeros(max(clusters, 1), dtype=CLUSTER_DTYPE)
+    st = lib.dogm_cluster_means(_ptr(c), c.size, _ptr(i), clusters, _ptr(out))
+    if st != DOGM_OK:
+        raise DogmError(st, "dogm_cluster_means")
+    return out[:clusters].copy()
+
+
+class Evaluator:
+    """DBSCAN + assignment + MAE / RMSE over frames (dogm_evaluator_*)."""
+
+    def __init__(self, resolution: float, max_assignment_distance: float = 5.0, eps: float = 3.0,
+                 min_cells: int = 5):
+        h = C.c_void_p()
+        st = lib.dogm_evaluator_create(resolution, max_assignment_distance, eps, min_cells, C.byref(h))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_evaluator_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.dogm_evaluator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_step(self, cells: np.ndarray, truth: np.ndarray) -> np.ndarray:
+        """Clusters of the frame (metres) after accumulating their errors against `truth`."""
+        c = np.ascontiguousarray(cells, dtype=VELOCITY_CELL_DTYPE)
+        t = np.ascontiguousarray(truth, dtype=OBJECT_DTYPE)
+        cap = max(c.size, 1)
+        out = np.zeros(cap, dtype=CLUSTER_DTYPE)
+        k = C.c_int()
+        st = lib.dogm_evaluator_add_step(self.h, _ptr(c), c.size, _ptr(t), t.size, _ptr(out), cap, C.byref(k))
+        if st != DOGM_OK:
+            raise DogmError(st, "dogm_evaluator_add_step")
+        return out[:k.value].copy()
+
+    def summary(self) -> dict:
+        mae = np.zeros(1, OBJECT_DTYPE)
+        rmse = np.zeros(1, OBJECT_DTYPE)
+        d, u = C.c_longlong(), C.c_longlong()
+        lib.dogm_evaluator_summary(self.h, _ptr(mae), _ptr(rmse), C.byref(d), C.byref(u))
+        as_dict = lambda a: {k: float(a[k][0]) for k in OBJECT_DTYPE.names}  # noqa: E731
+        return {"mae": as_dict(mae), "rmse": as_dict(rmse), "detections": d.value, "unassigned": u.value}

@@ -80,6 +80,19 @@ class Scene:
         ends = np.roll(corners, -1, axis=1).reshape(-1, 2)
         return starts, ends - starts
 
+    def objects(self, frame: int) -> np.ndarray:
+        """Ground truth of the boxes inside the grid at `frame`: centre in grid metres (the sensor
+        at 0.5 * grid_size * resolution) and world velocity [m/s], as (x, y, vx, vy) float32 rows."""
+        c = self.cfg
+        ex, ey, _ = self.ego_pose(frame)
+        half = 0.5 * c.grid_size * c.resolution
+        centre = self.box_pos0 + self.box_vel * (frame * c.dt) - np.array([ex, ey])[None, :] + half
+        inside = np.all((centre >= 0.0) & (centre < 2.0 * half), axis=1)
+        out = np.zeros(int(inside.sum()), dtype=[("x", "<f4"), ("y", "<f4"), ("vx", "<f4"), ("vy", "<f4")])
+        out["x"], out["y"] = centre[inside, 0], centre[inside, 1]
+        out["vx"], out["vy"] = self.box_vel[inside, 0], self.box_vel[inside, 1]
+        return out
+
     def beam_directions(self) -> tuple[np.ndarray, np.ndarray]:
         """(cos, sin) of every beam angle 2 pi b / n_beams, as the generators here use them."""
         ang = 2.0 * np.pi * np.arange(self.cfg.n_beams) / self.cfg.n_beams

@@ -203,6 +203,54 @@ int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_
                                  int capacity, int* count);
 
 /* ----------------------------------------------------------------------------------------------
+ * Accuracy harness (host code): DBSCAN over the cells that pass the velocity test, cluster means and
+ * MAE / RMSE against ground-truth objects -- the reference demo's evaluation
+ * (dogm/demo/utils/dbscan.cpp:24-114, precision_evaluator.cpp:37-122, metrics.cpp:7-53).
+ * DBSCAN is the textbook algorithm with the reference's conventions (a neighbourhood includes its
+ * point; core points have >= min_cells neighbours within eps, distance sqrtf of the cell-coordinate
+ * differences; clusters numbered by first core point in input order; breadth-first expansion in
+ * input order; a border point keeps the first cluster that reaches it).  The reference's
+ * expandCluster labels the first |seeds| entries of its point array instead of the seeds
+ * (dbscan.cpp:62-65); that defect is not reproduced.
+ * ---------------------------------------------------------------------------------------------- */
+/* Cluster mean: position in cell units (dogm_cluster_means) or metres (evaluator), velocity in
+ * cells/s or m/s. */
+typedef struct dogm_cluster
+{
+    float x, y, vx, vy;
+    int cells;
+} dogm_cluster;
+
+/* A ground-truth object (or an error vector): position [m] in the grid frame, velocity [m/s]. */
+typedef struct dogm_object
+{
+    float x, y, vx, vy;
+} dogm_object;
+
+/* cluster_id[i]: cluster of cell i, -1 for noise; *clusters = number of clusters. */
+int dogm_dbscan(const dogm_velocity_cell* cells, int n, float eps, int min_cells, int* cluster_id, int* clusters);
+/* Mean cell coordinates and mean velocities per cluster, f32 sums in input order
+ * (precision_evaluator.cpp:98-110, before its metre conversion). */
+int dogm_cluster_means(const dogm_velocity_cell* cells, int n, const int* cluster_id, int clusters,
+                       dogm_cluster* out);
+
+typedef struct dogm_evaluator dogm_evaluator;
+/* The demo uses resolution = the grid's, max_assignment_distance 5 m, eps 3 cells, min_cells 5
+ * (precision_evaluator.cpp:15-17). */
+int dogm_evaluator_create(float resolution, float max_assignment_distance, float eps, int min_cells,
+                          dogm_evaluator** out);
+void dogm_evaluator_destroy(dogm_evaluator* e);
+/* One frame: clusters the cells, converts the means to metres (x resolution), assigns each cluster
+ * to the closest object closer than max_assignment_distance and accumulates its error; a cluster
+ * without one counts as unassigned.  Cluster means (metres) are written to clusters_out (up to
+ * clusters_cap) and their number to *clusters.  Frames without cells or objects add nothing. */
+int dogm_evaluator_add_step(dogm_evaluator* e, const dogm_velocity_cell* cells, int n, const dogm_object* truth,
+                            int objects, dogm_cluster* clusters_out, int clusters_cap, int* clusters);
+/* MAE and RMSE of (x, y, vx, vy) over every assigned cluster so far (sums in f64). */
+int dogm_evaluator_summary(const dogm_evaluator* e, dogm_object* mae, dogm_object* rmse, long long* detections,
+                           long long* unassigned);
+
+/* ----------------------------------------------------------------------------------------------
  * Measurement-grid generation on the device (the step before updateGrid).  Replaces the reference's
  * LaserMeasurementGrid::generateGrid (dogm/demo/simulator/mapping/laser_to_meas_grid.cu:25-70), a polar
  * inverse-sensor texture rendered to a Cartesian grid with OpenGL, by a direct ray cast with a
