@@ -8,6 +8,12 @@
 // Box-Muller, exact fixed-point sums) with the product; those primitives are pinned by their
 // own known-answer tests.
 //
+// Beside the update it restates the steps before and after it: the two-level lidar measurement grid
+// (oracle_lidar_grid) and the demo's result extraction (oracle_occupancy, oracle_cells_with_velocity).
+// The velocity test restates OpenCV's f32 evaluation of v^T S^-1 v; closed forms pin it, but the last-bit
+// agreement with OpenCV itself on arbitrary inputs is parity unpinned (OpenCV is not installed and the
+// reference's tests hold no vectors for it).
+//
 // Two modes:
 //   T = float   bit-exact twin of the GPU path (same f32 operations, same order, no contraction);
 //   T = double  the same algorithm with every per-element formula in double, used to report the
