@@ -427,10 +427,16 @@ def run_accuracy(args) -> int:
     import accuracy
     say = lambda m: print(f"[accuracy] {m}", file=sys.stderr, flush=True)  # noqa: E731
     rows = []
-    for form, mode, veh in (("plain", "exact", 1), ("plain", "fp64", 1), ("plain", "fp32", 1),
-                            ("batched", "exact", args.batch)):
+    for form, mode, veh, ego, frame in (("plain", "exact", 1, "reference", "world"),
+                                        ("plain", "fp64", 1, "reference", "world"),
+                                        ("plain", "fp32", 1, "reference", "world"),
+                                        ("batched", "exact", args.batch, "reference", "world"),
+                                        ("plain", "exact", 1, "metric", "world"),
+                                        ("plain", "exact", 1, "reference", "vehicle"),
+                                        ("plain", "exact", 1, "metric", "vehicle")):
         t0 = time.perf_counter()
-        r = accuracy.run(args.workload, args.accuracy_frames, form=form, cdf_mode=mode, vehicles=veh)
+        r = accuracy.run(args.workload, args.accuracy_frames, form=form, cdf_mode=mode, vehicles=veh, ego_mode=ego,
+                         sensor_frame=frame)
         r.pop("per_frame")
         r["vehicles"] = veh
         r["seconds"] = round(time.perf_counter() - t0, 2)

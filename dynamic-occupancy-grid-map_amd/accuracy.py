@@ -23,12 +23,20 @@ MIN_OCC, MIN_VEL = 0.7, 4.0
 
 
 def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5, seed: int = 1,
-        form: str = "plain", cdf_mode: str = "exact", vehicles: int = 1) -> dict:
+        form: str = "plain", cdf_mode: str = "exact", vehicles: int = 1, ego_mode: str = "reference",
+        sensor_frame: str = "world") -> dict:
     """form "plain": one handle; "batched": `vehicles` vehicles of the same scene (instances 0..)
-    stepped together, all scored."""
+    stepped together, all scored.  sensor_frame "vehicle" turns the lidar and its grid with the ego
+    heading (the yaw is then passed to the filter; with a world-aligned sensor the filter gets yaw 0,
+    as the reference demo passes, main.cpp:91)."""
     w = scene_mod.WORKLOADS[workload]
     p = Params(**w["params"])
-    sc = scene_mod.make_scene(workload, seed=seed)
+    sc = scene_mod.make_scene(workload, seed=seed, sensor_frame=sensor_frame)
+
+    def pose(f):
+        x, y, yaw = sc.ego_pose(f)
+        return x, y, (yaw if sensor_frame == "vehicle" else 0.0)
+
     frames = frames or w["frames"]
     dt = float(w["dt"])
     ev = Evaluator(p.resolution)
@@ -36,8 +44,9 @@ def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5,
     if form == "plain":
         g = DOGM(p, seed=123456)
         g.set_cdf_mode(cdf_mode)
+        g.set_ego_mode(ego_mode)
         for f in range(frames):
-            x, y, yaw = sc.ego_pose(f)
+            x, y, yaw = pose(f)
             g.update_grid(sc.measurement(f), x, y, yaw, dt)
             if f >= skip:
                 cells, _ = g.get_cells_with_velocity(MIN_OCC, MIN_VEL)
@@ -50,9 +59,10 @@ def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5,
         b.set_extraction(EXTRACT_VELOCITY, MIN_OCC, MIN_VEL)  # inside every step, for all vehicles
         for vid in ids:
             b.set_cdf_mode(vid, cdf_mode)
+            b.set_ego_mode(vid, ego_mode)
         for f in range(frames):
             m = sc.measurement(f)
-            b.update(ids, [m] * vehicles, [sc.ego_pose(f)] * vehicles, dt)
+            b.update(ids, [m] * vehicles, [pose(f)] * vehicles, dt)
             if f >= skip:
                 for vid in ids:
                     cells, _ = b.get_cells_with_velocity(vid)
@@ -61,8 +71,8 @@ def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5,
         b.close()
     s = ev.summary()
     ev.close()
-    s.update({"workload": workload, "form": form, "cdf_mode": cdf_mode, "frames": frames, "scored_from": skip,
-              "per_frame": per_frame})
+    s.update({"workload": workload, "form": form, "cdf_mode": cdf_mode, "ego_mode": ego_mode,
+              "sensor_frame": sensor_frame, "frames": frames, "scored_from": skip, "per_frame": per_frame})
     return s
 
 
@@ -74,7 +84,10 @@ if __name__ == "__main__":
     ap.add_argument("--form", default="plain", choices=["plain", "batched"])
     ap.add_argument("--cdf-mode", default="exact", choices=["exact", "fp64", "fp32"])
     ap.add_argument("--vehicles", type=int, default=1)
+    ap.add_argument("--ego-mode", default="reference", choices=["reference", "metric"])
+    ap.add_argument("--sensor-frame", default="world", choices=["world", "vehicle"])
     a = ap.parse_args()
-    r = run(a.workload, a.frames, a.skip, form=a.form, cdf_mode=a.cdf_mode, vehicles=a.vehicles)
+    r = run(a.workload, a.frames, a.skip, form=a.form, cdf_mode=a.cdf_mode, vehicles=a.vehicles,
+            ego_mode=a.ego_mode, sensor_frame=a.sensor_frame)
     print(json.dumps(r))
     sys.exit(0)

@@ -191,6 +191,21 @@ void reset_timing(Engine* E)
     }
 }
 
+// Switching modes keeps the grid where it is: the metric frame starts at the current pose.
+int set_ego_mode(Engine* E, Vehicle* v, int mode)
+{
+    if (mode != DOGM_EGO_REFERENCE && mode != DOGM_EGO_METRIC)
+        return fail(E, DOGM_ERR_INVALID_ARGUMENT, "unknown ego mode " + std::to_string(mode));
+    if (mode == DOGM_EGO_METRIC && v->ego_mode != DOGM_EGO_METRIC && v->first_pose)
+    {
+        v->ref_x = v->pos_x;
+        v->ref_y = v->pos_y;
+        v->ref_yaw = v->yaw;
+    }
+    v->ego_mode = mode;
+    return DOGM_OK;
+}
+
 Vehicle* find_vehicle(dogm_batch* b, int id)
 {
     auto it = b->veh.find(id);
@@ -489,6 +504,17 @@ int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_
     int st = extract_ensure(&h->eng, &h->veh,
                             ExtractCfg{DOGM_EXTRACT_VELOCITY, DOGM_SOURCE_GRID, min_occupancy, min_velocity}, false);
     return st ? st : extract_read_velocity(&h->eng, &h->veh, out, capacity, count);
+}
+
+int dogm_set_ego_mode(dogm_handle* h, int mode)
+{
+    NEED_H(h);
+    return set_ego_mode(&h->eng, &h->veh, mode);
+}
+
+int dogm_get_ego_mode(const dogm_handle* h)
+{
+    return h ? h->veh.ego_mode : -1;
 }
 
 int dogm_get_weight_array(dogm_handle* h, float* out)
@@ -824,6 +850,16 @@ int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     v->cdf_mode = mode;
     return DOGM_OK;
+}
+
+int dogm_batch_set_ego_mode(dogm_batch* b, int vehicle_id, int mode)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = find_vehicle(b, vehicle_id);
+    if (!v)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
+    return set_ego_mode(&b->eng, v, mode);
 }
 
 int dogm_batch_set_lanes(dogm_batch* b, int lanes)

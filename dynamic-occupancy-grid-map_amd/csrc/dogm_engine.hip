@@ -18,7 +18,7 @@ const char* const kKernelNames[K_COUNT] = {
     "birth_particles", "init_cells",     "init_particles",  "copy_masses",     "seg_moments",
     "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
     "unpack_grid",    "build_tables",    "extract_mark",    "extract_scan",    "extract_compact",
-    "extract_gather",
+    "extract_gather", "ego_particles",   "ego_field",
 };
 
 // ---- timing ------------------------------------------------------------------------------------
@@ -614,10 +614,69 @@ static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
         shift_ptr(reinterpret_cast<void**>(&v->grid.f[f]));
 }
 
+// Metric ego compensation (no counterpart in the reference, which ignores yaw and drops the sub-cell
+// remainder of every shift, dogm.cu:172-200).  The grid frame is the sensor frame: x, y along the
+// vehicle axes at yaw, y pointing left of x, the sensor at the grid centre.  A world point w lies at
+// cell c + R(-yaw) (w - ref) / resolution, so a pose change moves it to
+//   c + R(-dpsi) (g - c) - R(-yaw') (pos' - ref) / resolution.
+// Only whole cells of the translation are applied; `ref` advances by exactly the applied distance,
+// so the remainder carries into later frames.  The rotation is applied at every yaw change:
+// particle positions and velocities continuously, grid fields by nearest-cell resampling.
+static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
+{
+    if (!v->first_pose)
+    {
+        v->ref_x = new_x;
+        v->ref_y = new_y;
+        v->ref_yaw = new_yaw;
+        v->pos_x = new_x;
+        v->pos_y = new_y;
+        v->yaw = new_yaw;
+        v->first_pose = true;
+        return;
+    }
+    const double res = v->params.resolution;
+    const double dpsi = std::remainder((double)new_yaw - v->ref_yaw, 2.0 * M_PI);
+    const double c = std::cos((double)new_yaw), s = std::sin((double)new_yaw);
+    const double wx = (double)new_x - v->ref_x, wy = (double)new_y - v->ref_y;
+    const double nx = std::trunc((c * wx + s * wy) / res), ny = std::trunc((-s * wx + c * wy) / res);
+    v->ref_yaw = new_yaw;
+    v->ref_x += (c * nx - s * ny) * res;
+    v->ref_y += (s * nx + c * ny) * res;
+    v->pos_x = (float)v->ref_x;
+    v->pos_y = (float)v->ref_y;
+    v->yaw = new_yaw;
+    if (dpsi == 0.0 && nx == 0.0 && ny == 0.0)
+        return;
+    const EgoXform t{0.5f * (float)v->gs, (float)std::cos(dpsi), (float)std::sin(dpsi), (float)(-nx), (float)(-ny)};
+    run_ego_particles(E, v, t);
+    auto xform = [&](void** field) {
+        run_ego_field(E, v, *field, v->field_tmp, t);
+        void* old = *field;
+        *field = v->field_tmp;
+        v->field_tmp = static_cast<float*>(old);
+    };
+    if (lazy)
+    {
+        // Inside a frame only the free mass is read before being rewritten.
+        xform(reinterpret_cast<void**>(&v->grid.f[GF_FREE]));
+        return;
+    }
+    xform(reinterpret_cast<void**>(&v->grid.start));
+    xform(reinterpret_cast<void**>(&v->grid.end));
+    for (int f = 0; f < GF_COUNT; ++f)
+        xform(reinterpret_cast<void**>(&v->grid.f[f]));
+}
+
 // dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
 // occupancy update (free mass, the only grid field read before being rewritten in a frame).
 void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
 {
+    if (v->ego_mode == DOGM_EGO_METRIC)
+    {
+        ego_metric_update(E, v, new_x, new_y, new_yaw, lazy);
+        return;
+    }
     if (!v->first_pose)
     {
         v->pos_x = new_x;

@@ -192,6 +192,8 @@ enum KernelId
     K_EXTRACT_SCAN,
     K_EXTRACT_COMPACT,
     K_EXTRACT_GATHER,
+    K_EGO_PARTICLES,
+    K_EGO_FIELD,
     K_COUNT
 };
 
@@ -253,6 +255,8 @@ struct Vehicle
     float ext_min_occ = 0.0f, ext_min_vel = 0.0f;
 
     // pose (dogm/src/dogm.cu:161-203)
+    int ego_mode = 0;  // DOGM_EGO_REFERENCE / DOGM_EGO_METRIC
+    double ref_x = 0.0, ref_y = 0.0, ref_yaw = 0.0;  // metric mode: pose the grid frame sits at
     bool first_pose = false;
     bool first_meas = false;
     float pos_x = 0.0f, pos_y = 0.0f, yaw = 0.0f;
@@ -425,6 +429,15 @@ void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 
 void run_sort(Engine* E, int pph, bool hist_done, bool gather);  // gather: last pass writes the particles
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
+// Metric ego compensation: rotation by -dpsi about the grid centre (c) plus a whole-cell translation.
+struct EgoXform
+{
+    float c;       // grid centre in cells (grid_size / 2)
+    float cs, sn;  // cos / sin of the yaw change
+    float tx, ty;  // translation in cells
+};
+void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t);
+void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, const EgoXform& t);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);
 void run_pack_grid(Engine* E, const Vehicle* v);
 void run_unpack_grid(Engine* E, const Vehicle* v);

@@ -109,6 +109,64 @@ void run_move_particles(Engine* E, const Vehicle* v, float mx, float my)
     launch_end(E, K_MOVE_PARTICLES);
 }
 
+// Metric ego compensation of the particles: positions rotated by -dpsi about the grid centre and
+// translated by whole cells, velocities rotated (a pure translation keeps positions exact).
+__global__ __launch_bounds__(BLOCK) void k_ego_particles(DevParticles P, int n, EgoXform t)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    float4 s = P.state[i];
+    if (t.sn == 0.0f && t.cs == 1.0f)
+    {
+        s.x = s.x + t.tx;
+        s.y = s.y + t.ty;
+    }
+    else
+    {
+        const float dx = s.x - t.c, dy = s.y - t.c;
+        const float rx = t.cs * dx + t.sn * dy;
+        const float ry = t.cs * dy - t.sn * dx;
+        const float vx = t.cs * s.z + t.sn * s.w;
+        const float vy = t.cs * s.w - t.sn * s.z;
+        s = make_float4((t.c + rx) + t.tx, (t.c + ry) + t.ty, vx, vy);
+    }
+    P.state[i] = s;
+}
+
+void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t)
+{
+    if (v->N <= 0)
+        return;
+    launch_begin(E, K_EGO_PARTICLES);
+    hipLaunchKernelGGL(k_ego_particles, dim3(div_up(v->N, BLOCK)), dim3(BLOCK), 0, E->stream, v->part[0], v->N, t);
+    launch_end(E, K_EGO_PARTICLES);
+}
+
+// The same transform on a grid field, by nearest cell: destination cell (x, y) takes the source cell
+// containing c + R(dpsi) ((x + 0.5, y + 0.5) - c - t); sources outside the grid give 0.
+__global__ __launch_bounds__(BLOCK) void k_ego_field(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                     int gs, EgoXform t)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= gs * gs)
+        return;
+    const float qx = ((float)(c % gs) + 0.5f - t.c) - t.tx;
+    const float qy = ((float)(c / gs) + 0.5f - t.c) - t.ty;
+    const float sx = floorf(t.c + (t.cs * qx - t.sn * qy));
+    const float sy = floorf(t.c + (t.sn * qx + t.cs * qy));
+    const bool in = sx >= 0.0f && sx < (float)gs && sy >= 0.0f && sy < (float)gs;
+    dst[c] = in ? src[(int)sx + gs * (int)sy] : 0u;
+}
+
+void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, const EgoXform& t)
+{
+    launch_begin(E, K_EGO_FIELD);
+    hipLaunchKernelGGL(k_ego_field, dim3(div_up(v->C, BLOCK)), dim3(BLOCK), 0, E->stream,
+                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), v->gs, t);
+    launch_end(E, K_EGO_FIELD);
+}
+
 // new[x, y] = old[x + ix, y + iy] when 0 < x + ix < gs and 0 < y + iy < gs (strict lower bound,
 // ego_motion_compensation.cu:38), else 0 (the reference memsets the grid first, dogm.cu:185).
 __global__ __launch_bounds__(BLOCK) void k_shift_field(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,

@@ -26,6 +26,7 @@ lib = C.CDLL(str(LIB_PATH))
 DOGM_OK = 0
 DOGM_DEFAULT_SEED = 123456
 CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2}
+EGO_MODES = {"reference": 0, "metric": 1}
 
 
 class dogm_params(C.Structure):
@@ -139,6 +140,9 @@ _SIGS = {
     "dogm_get_kernel_timing": (_i, [_h, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "dogm_reset_kernel_timing": (_i, [_h]),
     "dogm_set_cdf_mode": (_i, [_h, _i]),
+    "dogm_set_ego_mode": (_i, [_h, _i]),
+    "dogm_get_ego_mode": (_i, [_h]),
+    "dogm_batch_set_ego_mode": (_i, [_h, _i, _i]),
     "dogm_get_cdf_mode": (_i, [_h]),
     "dogm_get_epoch": (C.c_uint32, [_h]),
     "dogm_batch_set_cdf_mode": (_i, [_h, _i, _i]),
@@ -411,6 +415,9 @@ class DOGM:
         self._check(lib.dogm_get_resampled_indices(self.h, _ptr(out)), "getResampledIndices")
         return out
 
+    def set_ego_mode(self, mode: str) -> None:
+        self._check(lib.dogm_set_ego_mode(self.h, EGO_MODES[mode]), "setEgoMode")
+
     def set_cdf_mode(self, mode: str) -> None:
         """Resampling CDF arithmetic: 'exact' (default), 'fp64' or 'fp32'."""
         self._check(lib.dogm_set_cdf_mode(self.h, CDF_MODES[mode]), "setCdfMode")
@@ -548,6 +555,9 @@ class Batch:
 
     def set_cdf_mode(self, vid: int, mode: str) -> None:
         self._check(lib.dogm_batch_set_cdf_mode(self.h, vid, CDF_MODES[mode]), "setCdfMode")
+
+    def set_ego_mode(self, vid: int, mode: str) -> None:
+        self._check(lib.dogm_batch_set_ego_mode(self.h, vid, EGO_MODES[mode]), "setEgoMode")
 
     def set_extraction(self, what: int, min_occupancy: float = 0.7, min_velocity: float = 4.0) -> None:
         self._check(lib.dogm_batch_set_extraction(self.h, what, min_occupancy, min_velocity), "setExtraction")

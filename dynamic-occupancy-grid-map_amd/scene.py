@@ -34,6 +34,9 @@ class SceneConfig:
     ego_yaw_rate: float = 0.0
     dt: float = 0.1
     seed: int = 1
+    # "world": grid axes along the world axes (beams at fixed world angles); "vehicle": the sensor and
+    # its grid turn with the ego heading (beam b at 2 pi b / n_beams from the heading)
+    sensor_frame: str = "world"
 
 
 class Scene:
@@ -84,13 +87,20 @@ class Scene:
         """Ground truth of the boxes inside the grid at `frame`: centre in grid metres (the sensor
         at 0.5 * grid_size * resolution) and world velocity [m/s], as (x, y, vx, vy) float32 rows."""
         c = self.cfg
-        ex, ey, _ = self.ego_pose(frame)
+        ex, ey, yaw = self.ego_pose(frame)
         half = 0.5 * c.grid_size * c.resolution
-        centre = self.box_pos0 + self.box_vel * (frame * c.dt) - np.array([ex, ey])[None, :] + half
+        rel = self.box_pos0 + self.box_vel * (frame * c.dt) - np.array([ex, ey])[None, :]
+        vel = self.box_vel
+        if c.sensor_frame == "vehicle":
+            cy, sy = np.cos(yaw), np.sin(yaw)
+            rot = np.array([[cy, sy], [-sy, cy]])  # world -> vehicle axes
+            rel = rel @ rot.T
+            vel = vel @ rot.T
+        centre = rel + half
         inside = np.all((centre >= 0.0) & (centre < 2.0 * half), axis=1)
         out = np.zeros(int(inside.sum()), dtype=[("x", "<f4"), ("y", "<f4"), ("vx", "<f4"), ("vy", "<f4")])
         out["x"], out["y"] = centre[inside, 0], centre[inside, 1]
-        out["vx"], out["vy"] = self.box_vel[inside, 0], self.box_vel[inside, 1]
+        out["vx"], out["vy"] = vel[inside, 0], vel[inside, 1]
         return out
 
     def beam_directions(self) -> tuple[np.ndarray, np.ndarray]:
@@ -107,9 +117,11 @@ class Scene:
     def ranges(self, frame: int) -> np.ndarray:
         """Distance to the first hit per beam (inf when nothing is hit within max_range)."""
         c = self.cfg
-        ex, ey, _ = self.ego_pose(frame)
+        ex, ey, yaw = self.ego_pose(frame)
         p, e = self._box_edges(frame, np.array([ex, ey]) + self.ego0)
         ang = 2.0 * np.pi * np.arange(c.n_beams) / c.n_beams
+        if c.sensor_frame == "vehicle":
+            ang = ang + yaw  # world directions of the beams
         d = np.stack([np.cos(ang), np.sin(ang)], 1)  # (b, 2)
         # Solve t d = p + s e for every beam/edge pair.
         den = d[:, None, 0] * e[None, :, 1] - d[:, None, 1] * e[None, :, 0]
@@ -173,6 +185,6 @@ WORKLOADS = {
 }
 
 
-def make_scene(workload: str, seed: int = 1) -> Scene:
+def make_scene(workload: str, seed: int = 1, sensor_frame: str = "world") -> Scene:
     w = WORKLOADS[workload]
-    return Scene(SceneConfig(seed=seed, dt=w["dt"], **w["scene"]))
+    return Scene(SceneConfig(seed=seed, dt=w["dt"], sensor_frame=sensor_frame, **w["scene"]))

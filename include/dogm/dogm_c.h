@@ -331,6 +331,28 @@ enum
 };
 int dogm_set_cdf_mode(dogm_handle* h, int mode);
 int dogm_get_cdf_mode(const dogm_handle* h);
+/* ----------------------------------------------------------------------------------------------
+ * Ego-motion compensation mode.  DOGM_EGO_REFERENCE (default) is the reference's updatePose
+ * (dogm.cu:161-203): a move of more than one resolution shifts particles and grid by the truncated
+ * cell offset with the sign convention of ego_motion_compensation.cu:16-43, the sub-cell remainder is
+ * dropped and yaw is stored but never applied.
+ * DOGM_EGO_METRIC (opt-in, no counterpart in the reference): the grid frame is the sensor frame (axes
+ * along the vehicle at its yaw, the sensor at the grid centre); a world point w sits at cell
+ * c + R(-yaw) (w - ref) / resolution.  Every pose change applies the whole-cell part of the translation
+ * (the remainder carries into later frames: `ref` advances by exactly the applied distance) and the
+ * yaw change as a rotation about the grid centre -- particle positions and velocities exactly, grid
+ * fields by nearest-cell resampling; cells whose source lies outside the grid become 0.  For a sensor
+ * whose grid does not turn with the vehicle, pass yaw 0.  Switching to it starts the metric frame at
+ * the current pose.
+ * ---------------------------------------------------------------------------------------------- */
+enum
+{
+    DOGM_EGO_REFERENCE = 0,
+    DOGM_EGO_METRIC = 1
+};
+int dogm_set_ego_mode(dogm_handle* h, int mode);
+int dogm_get_ego_mode(const dogm_handle* h);
+
 /* Philox epoch of the handle's next random stage (the draws of a resampling called now use it);
  * lets a host reference rebuild the same uniforms. */
 uint32_t dogm_get_epoch(const dogm_handle* h);
@@ -372,6 +394,7 @@ int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, in
                              uint8_t* associated);
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
 int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
+int dogm_batch_set_ego_mode(dogm_batch* b, int vehicle_id, int mode);
 /* Lanes of a step (1 or 2, default 2).  With 2, a step over two or more vehicles splits them over
  * two engines with their own HIP streams, balanced by per-frame traffic; both lanes' launches run
  * concurrently on the device, which fills one lane's small launches and tails with the other's

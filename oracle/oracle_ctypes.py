@@ -67,6 +67,7 @@ def lib():
             "oracle_cells_with_velocity": (i, [i] + [vp] * 7 + [f, f, vp]),
             "oracle_velocity_normalized": (f, [f, f, f, f, f]),
             "oracle_lidar_grid": (None, [i, i, C.c_double, C.c_double, f, f, vp, vp, vp, vp]),
+            "oracle_set_ego_mode": (None, [vp, i]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -247,6 +248,9 @@ class OracleDOGM:
 
     def set_flags(self, first_pose: bool, first_meas: bool) -> None:
         lib().oracle_set_flags(self.h, int(first_pose), int(first_meas))
+
+    def set_ego_mode(self, mode: str) -> None:
+        lib().oracle_set_ego_mode(self.h, {"reference": 0, "metric": 1}[mode])
 
     def set_cdf_mode(self, mode: str) -> None:
         """Resampling CDF arithmetic of the f32 checker: 'exact', 'fp64' or 'fp32'."""

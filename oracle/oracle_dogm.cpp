@@ -89,6 +89,8 @@ struct OracleBase
     virtual int set_i32(int field, const int* in) = 0;
     float pos_x = 0.0f, pos_y = 0.0f, yaw = 0.0f;
     bool first_pose = false, first_meas = false;
+    int ego_mode = 0;                                // 0 reference, 1 metric (opt-in extension)
+    double ref_x = 0.0, ref_y = 0.0, ref_yaw = 0.0;  // metric mode: pose of the grid frame
     uint32_t epoch = 0;
     int cdf_mode = 0;  // CdfMode of the resampling scan (f32 checker only; the f64 twin is exact)
 };
@@ -296,8 +298,90 @@ struct Oracle final : OracleBase
     // the integer cell offset (x -= x_move) and the grid by new[x, y] = old[x + x_move, y + y_move]
     // when 0 < source < grid_size (strict lower bound, ego_motion_compensation.cu:38), else zero.
     // Yaw is stored but never applied; the sub-cell remainder is dropped.
+    // Metric ego compensation (the library's opt-in DOGM_EGO_METRIC; no counterpart in the
+    // reference).  Grid frame = sensor frame at `ref`: a world point w at cell
+    // c + R(-yaw)(w - ref) / resolution.  A pose change applies the whole cells of the translation,
+    // expressed in the new frame, and rotates by -dpsi about c = grid_size / 2: particles exactly,
+    // every grid field by the nearest source cell of each destination cell centre.
+    void update_pose_metric(float new_x, float new_y, float new_yaw)
+    {
+        if (!first_pose)
+        {
+            ref_x = new_x;
+            ref_y = new_y;
+            ref_yaw = new_yaw;
+            pos_x = new_x;
+            pos_y = new_y;
+            yaw = new_yaw;
+            first_pose = true;
+            return;
+        }
+        const double res = p.resolution;
+        const double two_pi = 2.0 * 3.14159265358979323846;
+        const double dpsi = std::remainder((double)new_yaw - ref_yaw, two_pi);
+        const double cy = std::cos((double)new_yaw), sy = std::sin((double)new_yaw);
+        const double wx = (double)new_x - ref_x, wy = (double)new_y - ref_y;
+        // displacement in cells along the new frame's axes, whole cells only
+        const double fx = (cy * wx + sy * wy) / res, fy = (-sy * wx + cy * wy) / res;
+        const double nx = std::trunc(fx), ny = std::trunc(fy);
+        ref_yaw = new_yaw;
+        ref_x += (cy * nx - sy * ny) * res;
+        ref_y += (sy * nx + cy * ny) * res;
+        pos_x = (float)ref_x;
+        pos_y = (float)ref_y;
+        yaw = new_yaw;
+        if (dpsi == 0.0 && nx == 0.0 && ny == 0.0)
+            return;
+        const float c = 0.5f * (float)gs;
+        const float cs = (float)std::cos(dpsi), sn = (float)std::sin(dpsi);
+        const float tx = (float)(-nx), ty = (float)(-ny);
+        const bool translate_only = sn == 0.0f && cs == 1.0f;
+        for (int i = 0; i < N; ++i)
+        {
+            if (translate_only)
+            {
+                part.x[i] = part.x[i] + (T)tx;
+                part.y[i] = part.y[i] + (T)ty;
+                continue;
+            }
+            const T dx = part.x[i] - (T)c, dy = part.y[i] - (T)c;
+            const T rx = (T)cs * dx + (T)sn * dy;
+            const T ry = (T)cs * dy - (T)sn * dx;
+            const T vx = (T)cs * part.vx[i] + (T)sn * part.vy[i];
+            const T vy = (T)cs * part.vy[i] - (T)sn * part.vx[i];
+            part.x[i] = ((T)c + rx) + (T)tx;
+            part.y[i] = ((T)c + ry) + (T)ty;
+            part.vx[i] = vx;
+            part.vy[i] = vy;
+        }
+        std::vector<int> os = start, oe = end;
+        std::vector<T> og[GF_COUNT];
+        for (int f = 0; f < GF_COUNT; ++f)
+            og[f] = g[f];
+        for (int yy = 0; yy < gs; ++yy)
+            for (int xx = 0; xx < gs; ++xx)
+            {
+                const float qx = ((float)xx + 0.5f - c) - tx;
+                const float qy = ((float)yy + 0.5f - c) - ty;
+                const float sxf = std::floor(c + (cs * qx - sn * qy));
+                const float syf = std::floor(c + (sn * qx + cs * qy));
+                const bool in = sxf >= 0.0f && sxf < (float)gs && syf >= 0.0f && syf < (float)gs;
+                const int i = xx + gs * yy;
+                const int j = in ? (int)sxf + gs * (int)syf : 0;
+                start[i] = in ? os[j] : 0;
+                end[i] = in ? oe[j] : 0;
+                for (int f = 0; f < GF_COUNT; ++f)
+                    g[f][i] = in ? og[f][j] : T(0);
+            }
+    }
+
     void update_pose(float new_x, float new_y, float new_yaw) override
     {
+        if (ego_mode == 1)
+        {
+            update_pose_metric(new_x, new_y, new_yaw);
+            return;
+        }
         if (!first_pose)
         {
             pos_x = new_x;
@@ -960,6 +1044,16 @@ float oracle_position_x(void* h) { return O->pos_x; }
 float oracle_position_y(void* h) { return O->pos_y; }
 float oracle_yaw(void* h) { return O->yaw; }
 void oracle_set_cdf_mode(void* h, int mode) { O->cdf_mode = mode; }
+void oracle_set_ego_mode(void* h, int mode)
+{
+    if (mode == 1 && O->ego_mode != 1 && O->first_pose)
+    {
+        O->ref_x = O->pos_x;
+        O->ref_y = O->pos_y;
+        O->ref_yaw = O->yaw;
+    }
+    O->ego_mode = mode;
+}
 void oracle_set_flags(void* h, int first_pose, int first_meas)
 {
     O->first_pose = first_pose != 0;
