@@ -12,7 +12,7 @@ consumes the next one.
    with its own Philox instance, all fed the same frame sequence; frames/s counts K frames per step.
 Prints one JSON line (stdout); per-kernel breakdowns go to stderr.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 8] [--cpu-frames 8]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 16] [--cpu-frames 8]
 """
 from __future__ import annotations
 
@@ -223,6 +223,63 @@ def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
 
 
 EXTRACT_KERNELS = ("extract_mark", "extract_scan", "extract_compact")
+
+
+def run_uneven(args) -> dict:
+    """Config 4: `args.uneven` grids of uneven size (sides from {128, 256, 512, 768, 1024} at 0.1 m,
+    N = 1.5 x cells, B = N / 10, each with its own scene, 720-beam lidar and ego motion) per
+    dogm_batch_update step.  Every step generates the grids on the device from the scans
+    (dogm_lidar_generate_many per side, into the vehicles' measurement buffers) and then updates all
+    vehicles; both parts are timed, the update also alone."""
+    import fleet as fl
+    from dogm_ctypes import Batch, lib
+    vehicles = fl.make_fleet(args.uneven, seed=args.seed + 100)
+    sides = [v.side for v in vehicles]
+    bank = fl.ScanBank(sides, frames=4)
+    b = Batch()
+    ids = [b.add_vehicle(v.params, 123456, i) for i, v in enumerate(vehicles)]
+    by_side = {}
+    for j, v in enumerate(vehicles):
+        by_side.setdefault(v.side, []).append(j)
+    outs = {sd: [b.measurement_buffer(ids[j]) for j in js] for sd, js in by_side.items()}
+    byte_sum = 0
+    for v in vehicles:
+        c = v.side * v.side
+        n, nb = v.params.particle_count, v.params.new_born_particle_count
+        byte_sum += frame_bytes(n, nb, c, True)
+
+    def generate(k):
+        for sd, js in by_side.items():
+            r = bank.ranges[sd][k % len(bank.ranges[sd])]
+            bank.lidar[sd].generate_many(np.stack([r] * len(js)), outs[sd])
+
+    def update(k):
+        b.update(ids, [None] * len(ids), [v.pose(k) for v in vehicles], 0.1)
+
+    for k in range(2):
+        generate(k)
+        update(k)
+    lib.dogm_device_synchronize()
+    steps = args.uneven_steps
+    t_gen = t_upd = 0.0
+    for k in range(2, 2 + steps):
+        t0 = time.perf_counter()
+        generate(k)
+        t1 = time.perf_counter()
+        update(k)
+        t2 = time.perf_counter()
+        t_gen += t1 - t0
+        t_upd += t2 - t1
+    frames = steps * len(ids)
+    b.close()
+    bank.close()
+    return {"vehicles": len(ids), "sides": {str(sd): len(js) for sd, js in sorted(by_side.items())}, "steps": steps,
+            "frames_per_s": round(frames / (t_gen + t_upd), 1), "update_frames_per_s": round(frames / t_upd, 1),
+            "ms_per_step": round((t_gen + t_upd) / steps * 1e3, 3), "update_ms_per_step": round(t_upd / steps * 1e3, 3),
+            "generation_ms_per_step": round(t_gen / steps * 1e3, 3),
+            "algorithmic_bytes_per_step": byte_sum,
+            "update_hbm_fraction": round(byte_sum / (t_upd / steps) / (HBM_PEAK_GBS * 1e9), 4),
+            "note": "grids generated on the device from lidar scans every step; ego shift every frame"}
 
 
 def run_generation(args, b, ids, sc, total):
@@ -597,7 +654,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="plain_full", choices=sorted(WORKLOAD_TEXT))
-    ap.add_argument("--batch", type=int, default=8,
+    ap.add_argument("--batch", type=int, default=16,
                     help="vehicles per batched step (0: plain form only); the batched form is the headline")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
@@ -615,6 +672,8 @@ def main() -> int:
     ap.add_argument("--serve-input", default="scan", choices=["scan", "grid"],
                     help="serving frames as lidar scans (grids generated on the device) or as ready grids")
     ap.add_argument("--max-batch", type=int, default=0, help="frames per serving step (0: no limit)")
+    ap.add_argument("--uneven", type=int, default=64, help="grids of uneven size in the config-4 leg (0: skip)")
+    ap.add_argument("--uneven-steps", type=int, default=10, help="timed steps of the config-4 leg")
     ap.add_argument("--generation-steps", type=int, default=20,
                     help="device measurement-grid generations for the batch after the timed frames (0: skip)")
     ap.add_argument("--extract-steps", type=int, default=5,
@@ -677,6 +736,11 @@ def main() -> int:
         print(f"[bench] batched x{args.batch}: {_summary(batched)}", file=sys.stderr, flush=True)
         head = batched
 
+    uneven = None
+    if args.batch > 0 and args.uneven > 0:
+        uneven = run_uneven(args)
+        print(f"[bench] uneven x{args.uneven}: {json.dumps(uneven)}", file=sys.stderr, flush=True)
+
     for tag, r in (("plain", plain), ("batched", batched)):
         if r is None:
             continue
@@ -734,6 +798,7 @@ def main() -> int:
         "batched_vs_plain": round(fps / (plain["frames"] / plain["elapsed"]), 3) if batched else None,
         "extraction": batched.get("extraction") if batched else None,
         "measurement_generation": batched.get("generation") if batched else None,
+        "batched_uneven": uneven,
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),
