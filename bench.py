@@ -532,7 +532,7 @@ def run_precision(args) -> int:
                             ego_speed=10.0, ego_yaw_rate=0.1, seed=args.seed)
     sc = scene.Scene(cfg)
     frames = args.precision_frames
-    modes = ["exact", "fp64", "fp32"]
+    modes = ["exact", "fp64", "fp32c", "fp32"]
     handles = {}
     for mo in modes:
         g = DOGM(p, seed=123456, instance=0)

@@ -550,7 +550,7 @@ int dogm_get_resampled_indices(dogm_handle* h, int* out)
 int dogm_set_cdf_mode(dogm_handle* h, int mode)
 {
     NEED_H(h);
-    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32)
+    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32C)
         return fail(&h->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     h->veh.cdf_mode = mode;
     return DOGM_OK;
@@ -846,7 +846,7 @@ int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
     Vehicle* v = find_vehicle(b, vehicle_id);
     if (!v)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
-    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32)
+    if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32C)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     v->cdf_mode = mode;
     return DOGM_OK;

@@ -64,6 +64,32 @@ __device__ __forceinline__ double dpp_val(double x)
     return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ f32c dpp_val(f32c x)
+{
+    f32c r;
+    r.hi = dpp_val<CTRL, ROW_MASK>(x.hi);
+    r.lo = dpp_val<CTRL, ROW_MASK>(x.lo);
+    return r;
+}
+
+// Value of lane - 1 (f32 / f64 / compensated f32); lane 0 gets its own value.
+__device__ __forceinline__ float shfl_up1(float x)
+{
+    return __shfl_up(x, 1, WAVE);
+}
+__device__ __forceinline__ double shfl_up1(double x)
+{
+    return __shfl_up(x, 1, WAVE);
+}
+__device__ __forceinline__ f32c shfl_up1(f32c x)
+{
+    f32c r;
+    r.hi = __shfl_up(x.hi, 1, WAVE);
+    r.lo = __shfl_up(x.lo, 1, WAVE);
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan_val(T x)
 {

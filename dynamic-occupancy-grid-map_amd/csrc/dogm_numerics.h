@@ -29,8 +29,10 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define DOGM_HD __host__ __device__ __forceinline__
+#define DOGM_HDM __host__ __device__ __forceinline__
 #else
 #define DOGM_HD static inline
+#define DOGM_HDM inline
 #endif
 
 namespace dogm_num
@@ -501,13 +503,53 @@ enum CdfMode
     CDF_EXACT = 0,
     CDF_FP64 = 1,
     CDF_FP32 = 2,
+    CDF_FP32C = 3,  // compensated f32: double-float pairs
 };
 
 DOGM_HD double cdf_float_draw(double total, double u, int mode)
 {
     if (mode == CDF_FP32)
         return (double)((float)total * (float)u);
-    return total * u;
+    return total * u;  // FP64, and compensated FP32 whose totals carry ~48 bits
+}
+
+// Error-free sum of two floats (Knuth's TwoSum): s = fl(a + b), e = (a + b) - s exactly.
+DOGM_HD void two_sum_f32(float a, float b, float* s, float* e)
+{
+    const float x = a + b;
+    const float bv = x - a;
+    const float av = x - bv;
+    *e = (a - av) + (b - bv);
+    *s = x;
+}
+
+// Compensated f32 value ("double-float"): hi + lo with |lo| <= ulp(hi) / 2 after every addition.
+// Addition (Dekker / Knuth): TwoSum of the high parts, the low parts and that error added in f32,
+// then renormalised with a second TwoSum.  Conversions to and from double are exact for normalised
+// pairs.
+struct f32c
+{
+    float hi, lo;
+    f32c() = default;
+    DOGM_HDM explicit f32c(int i) : hi((float)i), lo(0.0f) {}
+    DOGM_HDM explicit f32c(float f) : hi(f), lo(0.0f) {}
+    DOGM_HDM explicit f32c(double d) : hi((float)d), lo((float)(d - (double)(float)d)) {}
+    DOGM_HDM explicit operator double() const { return (double)hi + (double)lo; }
+};
+
+DOGM_HD f32c operator+(f32c a, f32c b)
+{
+    float s, e;
+    two_sum_f32(a.hi, b.hi, &s, &e);
+    const float t = (a.lo + b.lo) + e;
+    f32c r;
+    two_sum_f32(s, t, &r.hi, &r.lo);
+    return r;
+}
+
+DOGM_HD bool operator>(f32c a, f32c b)
+{
+    return (double)a > (double)b;
 }
 
 // CDF contribution of a joint weight: negative values (from rounding in rho_p = occ - rho_b) and

@@ -94,6 +94,11 @@ struct CdfFloatMode<float>
 {
     static constexpr int mode = CDF_FP32;
 };
+template <>
+struct CdfFloatMode<f32c>
+{
+    static constexpr int mode = CDF_FP32C;
+};
 
 // Thread fold, wave scan and wave carries of one tile: the thread's local prefixes, its wave
 // carry (sum of the earlier waves' totals), its exclusive prefix inside the wave, and the tile total
@@ -116,7 +121,7 @@ __device__ __forceinline__ void tile_float_scan(const VehDesc& d, int p0, int cn
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const T incl = wave_inclusive_scan_val(run);
-    T excl = __shfl_up(incl, 1, WAVE);
+    T excl = shfl_up1(incl);
     if (lane == 0)
         excl = (T)0;
     if (lane == WAVE - 1)
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_cdff_apply(const VehDesc* __restrict_
     }
     // Exclusive running maximum over the earlier threads of the tile.
     const T wmax = wave_inclusive_max_val(mx);
-    T before = __shfl_up(wmax, 1, WAVE);
+    T before = shfl_up1(wmax);
     if ((threadIdx.x & (WAVE - 1)) == 0)
         before = (T)0;
     if ((threadIdx.x & (WAVE - 1)) == WAVE - 1)
@@ -551,6 +556,8 @@ void run_resampling(Engine* E, int pph, int modes)
         float_mode(0.0);
     if (modes & (1 << CDF_FP32))
         float_mode(0.0f);
+    if (modes & (1 << CDF_FP32C))
+        float_mode(f32c(0));
     if (modes & ~(1 << CDF_EXACT))
         hipLaunchKernelGGL(k_cdff_env, dim3(vb), dim3(64), 0, E->stream, E->d_desc, E->m, env, E->scalars);
 

@@ -25,7 +25,7 @@ lib = C.CDLL(str(LIB_PATH))
 
 DOGM_OK = 0
 DOGM_DEFAULT_SEED = 123456
-CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2}
+CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2, "fp32c": 3}
 EGO_MODES = {"reference": 0, "metric": 1}
 
 

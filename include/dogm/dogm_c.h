@@ -318,7 +318,9 @@ int dogm_reset_kernel_timing(dogm_handle* h);
  * new weight total / N, dogm.cu:386-423).  Particle state and weights stay f32 in every mode.
  *  DOGM_CDF_EXACT (default): exact 128-bit fixed-point prefix sums; the draws and the ancestors
  *    do not depend on any summation order.
- *  DOGM_CDF_FP64 / DOGM_CDF_FP32: an f64 or f32 scan with one fixed association (4-element
+ *  DOGM_CDF_FP64 / DOGM_CDF_FP32 / DOGM_CDF_FP32C: an f64, f32 or compensated-f32 (double-float
+ *    pairs: TwoSum of the high parts, low parts added with the error, renormalised) scan with one
+ *    fixed association (4-element
  *    sequential folds, 64-lane wave scans, sequential carries across waves and 1024-entry tiles),
  *    ancestor = first entry of its running maximum >= the draw.  FP32 is the reference's class of
  *    scan (thrust::inclusive_scan in f32, dogm.cu:405-407).
@@ -327,7 +329,8 @@ enum
 {
     DOGM_CDF_EXACT = 0,
     DOGM_CDF_FP64 = 1,
-    DOGM_CDF_FP32 = 2
+    DOGM_CDF_FP32 = 2,
+    DOGM_CDF_FP32C = 3
 };
 int dogm_set_cdf_mode(dogm_handle* h, int mode);
 int dogm_get_cdf_mode(const dogm_handle* h);

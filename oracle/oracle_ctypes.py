@@ -68,6 +68,8 @@ def lib():
             "oracle_velocity_normalized": (f, [f, f, f, f, f]),
             "oracle_lidar_grid": (None, [i, i, C.c_double, C.c_double, f, f, vp, vp, vp, vp]),
             "oracle_set_ego_mode": (None, [vp, i]),
+            "oracle_two_sum_f32": (None, [f, f, C.POINTER(f), C.POINTER(f)]),
+            "oracle_f32c_add": (None, [f, f, f, f, C.POINTER(f), C.POINTER(f)]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -254,7 +256,7 @@ class OracleDOGM:
 
     def set_cdf_mode(self, mode: str) -> None:
         """Resampling CDF arithmetic of the f32 checker: 'exact', 'fp64' or 'fp32'."""
-        lib().oracle_set_cdf_mode(self.h, {"exact": 0, "fp64": 1, "fp32": 2}[mode])
+        lib().oracle_set_cdf_mode(self.h, {"exact": 0, "fp64": 1, "fp32": 2, "fp32c": 3}[mode])
 
     def position(self):
         return (lib().oracle_position_x(self.h), lib().oracle_position_y(self.h), lib().oracle_yaw(self.h))
@@ -377,3 +379,20 @@ def lidar_grid(grid_size: int, resolution: float, max_range: float, occ_mass: fl
     lib().oracle_lidar_grid(grid_size, cs.size, resolution, max_range, occ_mass, free_mass, cs.ctypes.data,
                             sn.ctypes.data, r.ctypes.data, out.ctypes.data)
     return out
+
+
+def two_sum_f32(a: float, b: float):
+    s, e = C.c_float(), C.c_float()
+    lib().oracle_two_sum_f32(a, b, C.byref(s), C.byref(e))
+    return np.float32(s.value), np.float32(e.value)
+
+
+def f32c_from_double(x: float):
+    hi = np.float32(x)
+    return hi, np.float32(x - float(hi))
+
+
+def f32c_add(a, b):
+    hi, lo = C.c_float(), C.c_float()
+    lib().oracle_f32c_add(a[0], a[1], b[0], b[1], C.byref(hi), C.byref(lo))
+    return np.float32(hi.value), np.float32(lo.value)

@@ -789,6 +789,8 @@ struct Oracle final : OracleBase
         std::vector<double> E;
         if (cdf_mode == CDF_FP64)
             float_cdf<double>(wj, E);
+        else if (cdf_mode == CDF_FP32C)
+            float_cdf<f32c>(wj, E);
         else
             float_cdf<float>(wj, E);
         const double total = J > 0 ? E[J - 1] : 0.0;
@@ -1212,6 +1214,20 @@ void oracle_lidar_grid(int gs, int beams, double res, double max_range, float oc
         out[c].likelihood = 1.0f;
         out[c].p_A = 1.0f;
     }
+}
+
+// Compensated f32 primitives (dogm_numerics.h) for their known-answer tests.
+void oracle_two_sum_f32(float a, float b, float* s, float* e) { two_sum_f32(a, b, s, e); }
+void oracle_f32c_add(float ahi, float alo, float bhi, float blo, float* hi, float* lo)
+{
+    f32c a, b;
+    a.hi = ahi;
+    a.lo = alo;
+    b.hi = bhi;
+    b.lo = blo;
+    const f32c r = a + b;
+    *hi = r.hi;
+    *lo = r.lo;
 }
 
 }  // extern "C"

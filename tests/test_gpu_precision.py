@@ -1,4 +1,5 @@
-"""Resampling CDF modes on the GPU against the checker, bit for bit (exact / FP64 / FP32 scans)."""
+"""Resampling CDF modes on the GPU against the checker, bit for bit (exact / FP64 / FP32 /
+compensated FP32 scans)."""
 import sys
 
 import numpy as np
@@ -19,7 +20,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "fp32c"])
 def test_float_modes_full_frames(mode):
     p = small_params(gs=48, res=0.5, n=30000, b=3000)
     sc = small_scene(gs=48, res=0.5)
@@ -36,7 +37,7 @@ def test_float_modes_full_frames(mode):
         assert_state_equal(f"{mode} frame {f}", gpu, orc, ancestors=True)
 
 
-@pytest.mark.parametrize("mode", ["exact", "fp64", "fp32"])
+@pytest.mark.parametrize("mode", ["exact", "fp64", "fp32", "fp32c"])
 def test_log_uniform_weights_many_tiles(mode):
     """Weights 10^(-9 U) injected before resampling (the precision workload's weights)."""
     n, b = 150_000, 15_000
@@ -71,8 +72,8 @@ def test_log_uniform_weights_many_tiles(mode):
 
 
 def test_batch_mixes_modes():
-    vs = [(small_params(gs=40, res=0.5, n=8000, b=800), small_scene(gs=40, res=0.5, seed=50 + k)) for k in range(3)]
-    modes = ["exact", "fp64", "fp32"]
+    vs = [(small_params(gs=40, res=0.5, n=8000, b=800), small_scene(gs=40, res=0.5, seed=50 + k)) for k in range(4)]
+    modes = ["exact", "fp64", "fp32", "fp32c"]
     with Batch() as batch:
         ids = [batch.add_vehicle(p, SEED, 60 + k) for k, (p, _) in enumerate(vs)]
         for k, vid in enumerate(ids):

@@ -42,7 +42,7 @@ def test_integer_weights_all_modes_equal_exact():
     w = rng.integers(0, 4, n).astype(np.float32)
     bw = rng.integers(0, 3, b).astype(np.float32)
     ref, ref_w = _resample_with("exact", w, bw, n, b)
-    for mode in ("fp64", "fp32"):
+    for mode in ("fp64", "fp32", "fp32c"):
         anc, ww = _resample_with(mode, w, bw, n, b)
         assert np.array_equal(anc, ref), mode
         assert np.array_equal(ww.view(np.uint32), ref_w.view(np.uint32)), mode
@@ -59,19 +59,20 @@ def test_exact_mode_equals_sequential_fp64_on_dyadic_weights():
 
 
 def test_log_uniform_weights_fp32_loses_ancestors_fp64_does_not():
-    """Weights 10^(-9 U): the CDF spans 9 decades.  FP64 and exact agree with the sequential FP64 CDF
-    almost everywhere; the FP32 scan absorbs the small weights and moves many ancestors."""
+    """Weights 10^(-9 U): the CDF spans 9 decades.  FP64, compensated FP32 and exact agree with the
+    sequential FP64 CDF almost everywhere; the FP32 scan absorbs the small weights and moves many
+    ancestors."""
     rng = np.random.default_rng(3)
     n, b = 200_000, 20_000
     w = (10.0 ** (-9.0 * rng.random(n))).astype(np.float32)
     bw = (10.0 ** (-9.0 * rng.random(b))).astype(np.float32)
     seq = _sequential_fp64_ancestors(w, bw, n)
     agree = {}
-    for mode in ("exact", "fp64", "fp32"):
+    for mode in ("exact", "fp64", "fp32c", "fp32"):
         anc, _ = _resample_with(mode, w, bw, n, b)
         assert np.all(np.diff(anc) >= 0), "ancestors of sorted draws are non-decreasing"
         agree[mode] = float(np.mean(anc == seq))
-    assert agree["exact"] >= 0.9999 and agree["fp64"] >= 0.9999, agree
+    assert agree["exact"] >= 0.9999 and agree["fp64"] >= 0.9999 and agree["fp32c"] >= 0.9999, agree
     assert agree["fp32"] < 0.999, agree
 
 
@@ -80,7 +81,28 @@ def test_weights_follow_mode_total():
     n, b = 3000, 0
     w = np.full(n, 0.25, np.float32)
     w[::7] = -1.0  # negative weights count as zero in the CDF
-    for mode in ("exact", "fp64", "fp32"):
+    for mode in ("exact", "fp64", "fp32", "fp32c"):
         _, ww = _resample_with(mode, w, np.zeros(0, np.float32), n, b)
         expect = np.float32(np.float32(0.25 * np.sum(w > 0)) / np.float32(n))
         assert np.all(ww == expect), mode
+
+
+def test_two_sum_is_error_free_and_compensated_add_is_accurate():
+    """TwoSum returns s = fl(a + b) and the exact error; the compensated add of (hi, lo) pairs keeps
+    the sum to ~2^-44 relative (rational arithmetic)."""
+    from fractions import Fraction
+    import oracle_ctypes as o
+    rng = np.random.default_rng(5)
+    for _ in range(3000):
+        a = np.float32(rng.normal() * 10.0 ** rng.integers(-8, 8))
+        b = np.float32(rng.normal() * 10.0 ** rng.integers(-8, 8))
+        s, e = o.two_sum_f32(a, b)
+        assert s == np.float32(a + b)
+        assert Fraction(float(s)) + Fraction(float(e)) == Fraction(float(a)) + Fraction(float(b))
+    for _ in range(3000):
+        x = [float(v) for v in rng.uniform(0, 1, 2) * 10.0 ** rng.integers(-9, 1, 2)]
+        pa, pb = o.f32c_from_double(x[0]), o.f32c_from_double(x[1])
+        hi, lo = o.f32c_add(pa, pb)
+        exact = sum(Fraction(float(np.float32(v))) for v in pa + pb)
+        assert abs(Fraction(float(hi)) + Fraction(float(lo)) - exact) <= exact * Fraction(1, 2 ** 44)
+        assert abs(lo) <= abs(hi) * 2.0 ** -24
