@@ -506,6 +506,20 @@ int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_
     return st ? st : extract_read_velocity(&h->eng, &h->veh, out, capacity, count);
 }
 
+int dogm_set_resampling_mode(dogm_handle* h, int mode)
+{
+    NEED_H(h);
+    if (mode != DOGM_RESAMPLE_MULTINOMIAL && mode != DOGM_RESAMPLE_SYSTEMATIC)
+        return fail(&h->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown resampling mode " + std::to_string(mode));
+    h->veh.draw_mode = mode;
+    return DOGM_OK;
+}
+
+int dogm_get_resampling_mode(const dogm_handle* h)
+{
+    return h ? h->veh.draw_mode : -1;
+}
+
 int dogm_set_ego_mode(dogm_handle* h, int mode)
 {
     NEED_H(h);
@@ -849,6 +863,19 @@ int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode)
     if (mode < DOGM_CDF_EXACT || mode > DOGM_CDF_FP32C)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown CDF mode " + std::to_string(mode));
     v->cdf_mode = mode;
+    return DOGM_OK;
+}
+
+int dogm_batch_set_resampling_mode(dogm_batch* b, int vehicle_id, int mode)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    Vehicle* v = find_vehicle(b, vehicle_id);
+    if (!v)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
+    if (mode != DOGM_RESAMPLE_MULTINOMIAL && mode != DOGM_RESAMPLE_SYSTEMATIC)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown resampling mode " + std::to_string(mode));
+    v->draw_mode = mode;
     return DOGM_OK;
 }
 

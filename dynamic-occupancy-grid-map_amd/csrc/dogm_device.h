@@ -708,8 +708,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
     {
         if (mode == SCAN_EXP)
         {
-            sc->exp_total = carry;
-            sc->spacing = spacing_setup(carry, sc->total);
+            // Systematic draws do not use the exponentials: S_N = N * 2^32 (dogm_numerics.h).
+            const fx_t s_n = d.draw_mode == DRAW_SYSTEMATIC ? systematic_s(d.N, 0u) : carry;
+            sc->exp_total = s_n;
+            sc->spacing = spacing_setup(s_n, sc->total);
             return;
         }
         sc->total = carry;

@@ -364,6 +364,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.gx = v->pend_gx;
         d.gy = v->pend_gy;
         d.cdf_mode = v->cdf_mode;
+        d.draw_mode = v->draw_mode;
         d.part[0] = v->part[0];
         d.part[1] = v->part[1];
         d.birth = v->birth;

@@ -102,6 +102,7 @@ struct VehDesc
     int shifted;             // pending grid shift (fused into the occupancy update)
     int gx, gy;
     int cdf_mode;            // resampling CDF arithmetic (dogm_num::CdfMode)
+    int draw_mode;           // resampling draws (dogm_num::DrawMode)
     int u0[SP_COUNT];        // first table unit of this vehicle per space
     int nu[SP_COUNT];        // its units per space (multiple of UNIT_ALIGN)
     DevParticles part[2];
@@ -220,6 +221,7 @@ struct Vehicle
     uint64_t uid = 0;    // unique per allocation (work tables are reused while the list is unchanged)
     bool init_pending = false;  // initialise particles in the next step that includes ST_INIT
     int cdf_mode = 0;           // resampling CDF arithmetic (dogm_num::CdfMode)
+    int draw_mode = 0;          // resampling draws (dogm_num::DrawMode)
     float dt = 0.0f;            // time step of the next step
 
     DevParticles part[2]{};  // part[0] is current

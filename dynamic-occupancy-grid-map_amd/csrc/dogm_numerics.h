@@ -490,6 +490,27 @@ DOGM_HD fx_t spacing_draw(fx_t s_k, const Spacing& sp)
     return fx_from_f64(sp.total * spacing_fraction(s_k, sp));
 }
 
+// Systematic resampling (an opt-in draw mode; the reference draws multinomially,
+// resampling.cu:17-47): one offset u in [0, 1) per resampling and draw k at (k + u) / N of the
+// total.  Written in the spacing form: S_k = k * 2^32 + u32, S_N = N * 2^32, so the fraction, draw
+// and ancestor search are those of the multinomial path.  u32 is the .x word of the Philox block
+// with counter 0xffffffff of the resampling stream (the exponentials use counters below 2^30).
+enum DrawMode
+{
+    DRAW_MULTINOMIAL = 0,
+    DRAW_SYSTEMATIC = 1,
+};
+
+DOGM_HD uint32_t systematic_u32(uint64_t seed, uint32_t instance, uint32_t epoch)
+{
+    return draw(seed, instance, TAG_RESAMPLE, epoch, 0xffffffffu).x;
+}
+
+DOGM_HD fx_t systematic_s(int k, uint32_t u32)
+{
+    return (fx_t)(((ufx_t)(uint32_t)k << 32) | (ufx_t)u32);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Floating-point CDF modes of the resampling scan (precision study).  The exact mode above is the
 // default.  In the float modes the CDF is a scan of the non-negative joint weights in f64 or f32

@@ -63,10 +63,10 @@ __device__ __forceinline__ void exp4(const VehDesc& d, int p0, int n, fx_t* e)
         e[i] = p0 + i < n ? fx_from_f32(f[i]) : (fx_t)0;
 }
 
-// The N + 1 exponentials as a scan source (tile totals only).
+// The N + 1 exponentials as a scan source (tile totals only); systematic draws need none.
 struct SrcExp
 {
-    __device__ bool active(const VehDesc&) const { return true; }
+    __device__ bool active(const VehDesc& d) const { return d.draw_mode == DRAW_MULTINOMIAL; }
     __device__ int len(const VehDesc& d) const { return d.N + 1; }
     __device__ void load4(const VehDesc& d, int p0, int, float (&v)[4]) const
     {
@@ -347,18 +347,28 @@ __global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ 
     const DevScalars* sc = scs + tr.v;
     const int J = n + d.B;
     const int p0 = base + threadIdx.x * RS_ITEMS;
-    fx_t e[4];
-    exp4(d, p0, n, e);
-    fx_t tot;
-    fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) +
-               exp_tile_pre[d.u0[SP_E] / SCAN_Q + base / RS_TILE];
     const Spacing sp = sc->spacing;
     V r[RS_ITEMS];
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i)
+    if (d.draw_mode == DRAW_SYSTEMATIC)
     {
-        run += e[i];
-        r[i] = Tr::draw(run, sp, sc, d.cdf_mode);
+        const uint32_t u = systematic_u32(d.seed, d.inst, d.ep_rs);
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+            r[i] = Tr::draw(systematic_s(p0 + i, u), sp, sc, d.cdf_mode);
+    }
+    else
+    {
+        fx_t e[4];
+        exp4(d, p0, n, e);
+        fx_t tot;
+        fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) +
+                   exp_tile_pre[d.u0[SP_E] / SCAN_Q + base / RS_TILE];
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+        {
+            run += e[i];
+            r[i] = Tr::draw(run, sp, sc, d.cdf_mode);
+        }
     }
     const int last = min(base + RS_TILE, n) - 1;
     if (p0 == base)

@@ -27,6 +27,7 @@ DOGM_OK = 0
 DOGM_DEFAULT_SEED = 123456
 CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2, "fp32c": 3}
 EGO_MODES = {"reference": 0, "metric": 1}
+RESAMPLING_MODES = {"multinomial": 0, "systematic": 1}
 
 
 class dogm_params(C.Structure):
@@ -141,6 +142,9 @@ _SIGS = {
     "dogm_reset_kernel_timing": (_i, [_h]),
     "dogm_set_cdf_mode": (_i, [_h, _i]),
     "dogm_set_ego_mode": (_i, [_h, _i]),
+    "dogm_set_resampling_mode": (_i, [_h, _i]),
+    "dogm_get_resampling_mode": (_i, [_h]),
+    "dogm_batch_set_resampling_mode": (_i, [_h, _i, _i]),
     "dogm_get_ego_mode": (_i, [_h]),
     "dogm_batch_set_ego_mode": (_i, [_h, _i, _i]),
     "dogm_get_cdf_mode": (_i, [_h]),
@@ -418,6 +422,9 @@ class DOGM:
     def set_ego_mode(self, mode: str) -> None:
         self._check(lib.dogm_set_ego_mode(self.h, EGO_MODES[mode]), "setEgoMode")
 
+    def set_resampling_mode(self, mode: str) -> None:
+        self._check(lib.dogm_set_resampling_mode(self.h, RESAMPLING_MODES[mode]), "setResamplingMode")
+
     def set_cdf_mode(self, mode: str) -> None:
         """Resampling CDF arithmetic: 'exact' (default), 'fp64' or 'fp32'."""
         self._check(lib.dogm_set_cdf_mode(self.h, CDF_MODES[mode]), "setCdfMode")
@@ -558,6 +565,9 @@ class Batch:
 
     def set_ego_mode(self, vid: int, mode: str) -> None:
         self._check(lib.dogm_batch_set_ego_mode(self.h, vid, EGO_MODES[mode]), "setEgoMode")
+
+    def set_resampling_mode(self, vid: int, mode: str) -> None:
+        self._check(lib.dogm_batch_set_resampling_mode(self.h, vid, RESAMPLING_MODES[mode]), "setResamplingMode")
 
     def set_extraction(self, what: int, min_occupancy: float = 0.7, min_velocity: float = 4.0) -> None:
         self._check(lib.dogm_batch_set_extraction(self.h, what, min_occupancy, min_velocity), "setExtraction")

@@ -335,6 +335,21 @@ enum
 int dogm_set_cdf_mode(dogm_handle* h, int mode);
 int dogm_get_cdf_mode(const dogm_handle* h);
 /* ----------------------------------------------------------------------------------------------
+ * Resampling draws.  DOGM_RESAMPLE_MULTINOMIAL (default) is the reference's: N sorted i.i.d. uniform
+ * draws over the CDF (resampling.cu:17-47), generated here as exponential spacings.
+ * DOGM_RESAMPLE_SYSTEMATIC (opt-in): draw k at (k + u) / N of the total with one uniform u per
+ * resampling, so a particle of weight w gets floor(N w / W) or ceil(N w / W) copies; no
+ * exponentials are generated.  Works with every CDF mode.
+ * ---------------------------------------------------------------------------------------------- */
+enum
+{
+    DOGM_RESAMPLE_MULTINOMIAL = 0,
+    DOGM_RESAMPLE_SYSTEMATIC = 1
+};
+int dogm_set_resampling_mode(dogm_handle* h, int mode);
+int dogm_get_resampling_mode(const dogm_handle* h);
+
+/* ----------------------------------------------------------------------------------------------
  * Ego-motion compensation mode.  DOGM_EGO_REFERENCE (default) is the reference's updatePose
  * (dogm.cu:161-203): a move of more than one resolution shifts particles and grid by the truncated
  * cell offset with the sign convention of ego_motion_compensation.cu:16-43, the sub-cell remainder is
@@ -398,6 +413,7 @@ int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, in
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
 int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
 int dogm_batch_set_ego_mode(dogm_batch* b, int vehicle_id, int mode);
+int dogm_batch_set_resampling_mode(dogm_batch* b, int vehicle_id, int mode);
 /* Lanes of a step (1 or 2, default 2).  With 2, a step over two or more vehicles splits them over
  * two engines with their own HIP streams, balanced by per-frame traffic; both lanes' launches run
  * concurrently on the device, which fills one lane's small launches and tails with the other's

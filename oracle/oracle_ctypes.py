@@ -68,6 +68,7 @@ def lib():
             "oracle_velocity_normalized": (f, [f, f, f, f, f]),
             "oracle_lidar_grid": (None, [i, i, C.c_double, C.c_double, f, f, vp, vp, vp, vp]),
             "oracle_set_ego_mode": (None, [vp, i]),
+            "oracle_set_draw_mode": (None, [vp, i]),
             "oracle_two_sum_f32": (None, [f, f, C.POINTER(f), C.POINTER(f)]),
             "oracle_f32c_add": (None, [f, f, f, f, C.POINTER(f), C.POINTER(f)]),
         }
@@ -250,6 +251,9 @@ class OracleDOGM:
 
     def set_flags(self, first_pose: bool, first_meas: bool) -> None:
         lib().oracle_set_flags(self.h, int(first_pose), int(first_meas))
+
+    def set_draw_mode(self, mode: str) -> None:
+        lib().oracle_set_draw_mode(self.h, {"multinomial": 0, "systematic": 1}[mode])
 
     def set_ego_mode(self, mode: str) -> None:
         lib().oracle_set_ego_mode(self.h, {"reference": 0, "metric": 1}[mode])

@@ -93,6 +93,31 @@ struct OracleBase
     double ref_x = 0.0, ref_y = 0.0, ref_yaw = 0.0;  // metric mode: pose of the grid frame
     uint32_t epoch = 0;
     int cdf_mode = 0;  // CdfMode of the resampling scan (f32 checker only; the f64 twin is exact)
+    int draw_mode = 0;  // DrawMode of the resampling draws
+
+    // Draw prefixes S_0 .. S_N of a resampling: exponential spacings (multinomial, the reference's
+    // draw) or k * 2^32 + u32 (systematic; dogm_numerics.h, systematic_s).
+    void draw_prefixes(int N, uint64_t seed, uint32_t instance, std::vector<fx_t>& spac) const
+    {
+        spac.assign(N + 1, 0);
+        if (draw_mode == DRAW_SYSTEMATIC)
+        {
+            const uint32_t u = systematic_u32(seed, instance, epoch);
+            for (int k = 0; k < N; ++k)
+                spac[k] = systematic_s(k, u);
+            spac[N] = systematic_s(N, 0u);
+            return;
+        }
+        fx_t e = 0;
+        float e4[4];
+        for (int k = 0; k <= N; ++k)
+        {
+            if ((k & 3) == 0)
+                resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
+            e += fx_from_f32(e4[k & 3]);
+            spac[k] = e;
+        }
+    }
 };
 
 // Float CDF modes: restatement of the device association (dogm_resample.hip, k_cdff_*): per tile of
@@ -736,16 +761,8 @@ struct Oracle final : OracleBase
             cdf[N + k] = s;
         }
         const fx_t total = s;
-        std::vector<fx_t> spac(N + 1);
-        fx_t e = 0;
-        float e4[4];
-        for (int k = 0; k <= N; ++k)
-        {
-            if ((k & 3) == 0)
-                resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
-            e += fx_from_f32(e4[k & 3]);
-            spac[k] = e;
-        }
+        std::vector<fx_t> spac;
+        draw_prefixes(N, seed, instance, spac);
         const Spacing sp = spacing_setup(spac[N], total);
         for (int k = 0; k < N; ++k)
         {
@@ -794,16 +811,8 @@ struct Oracle final : OracleBase
         else
             float_cdf<float>(wj, E);
         const double total = J > 0 ? E[J - 1] : 0.0;
-        std::vector<fx_t> spac(N + 1);
-        fx_t e = 0;
-        float e4[4];
-        for (int k = 0; k <= N; ++k)
-        {
-            if ((k & 3) == 0)
-                resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
-            e += fx_from_f32(e4[k & 3]);
-            spac[k] = e;
-        }
+        std::vector<fx_t> spac;
+        draw_prefixes(N, seed, instance, spac);
         const Spacing sp = spacing_setup(spac[N], 0);
         for (int k = 0; k < N; ++k)
         {
@@ -1046,6 +1055,7 @@ float oracle_position_x(void* h) { return O->pos_x; }
 float oracle_position_y(void* h) { return O->pos_y; }
 float oracle_yaw(void* h) { return O->yaw; }
 void oracle_set_cdf_mode(void* h, int mode) { O->cdf_mode = mode; }
+void oracle_set_draw_mode(void* h, int mode) { O->draw_mode = mode; }
 void oracle_set_ego_mode(void* h, int mode)
 {
     if (mode == 1 && O->ego_mode != 1 && O->first_pose)

@@ -95,3 +95,45 @@ def test_batch_mixes_modes():
                 assert_grid_equal(f"frame {f} vehicle {k}", batch.get_grid_cells(ids[k]), g.get_grid_cells())
                 assert_int_equal(f"frame {f} vehicle {k} ancestors", batch.get_resampled_indices(ids[k]),
                                  g.get_resampled_indices())
+
+
+@pytest.mark.parametrize("mode", ["exact", "fp64", "fp32"])
+def test_systematic_full_frames(mode):
+    p = small_params(gs=48, res=0.5, n=30000, b=3000)
+    sc = small_scene(gs=48, res=0.5)
+    gpu = DOGM(p, seed=SEED, instance=5)
+    orc = OracleDOGM(p, seed=SEED, instance=5)
+    for o in (gpu, orc):
+        o.set_cdf_mode(mode)
+    gpu.set_resampling_mode("systematic")
+    orc.set_draw_mode("systematic")
+    for f in range(3):
+        _say(f"systematic {mode} frame {f}")
+        x, y, yaw = sc.ego_pose(f)
+        m = sc.measurement(f)
+        gpu.update_grid(m, x, y, yaw, 0.1)
+        orc.update_grid(m, x, y, yaw, 0.1)
+        assert_state_equal(f"systematic {mode} frame {f}", gpu, orc, ancestors=True)
+
+
+def test_batch_mixes_draw_modes():
+    vs = [(small_params(gs=40, res=0.5, n=8000, b=800), small_scene(gs=40, res=0.5, seed=80 + k)) for k in range(2)]
+    draws = ["systematic", "multinomial"]
+    with Batch() as batch:
+        ids = [batch.add_vehicle(p, SEED, 90 + k) for k, (p, _) in enumerate(vs)]
+        plains = []
+        for k, (p, _) in enumerate(vs):
+            batch.set_resampling_mode(ids[k], draws[k])
+            g = DOGM(p, seed=SEED, instance=90 + k)
+            g.set_resampling_mode(draws[k])
+            plains.append(g)
+        for f in range(3):
+            _say(f"mixed draw modes frame {f}")
+            meas = [sc.measurement(f) for _, sc in vs]
+            poses = [sc.ego_pose(f) for _, sc in vs]
+            batch.update(ids, meas, poses, 0.1)
+            for k, g in enumerate(plains):
+                g.update_grid(meas[k], *poses[k], 0.1)
+                assert_particles_equal(f"frame {f} vehicle {k}", batch.get_particles(ids[k]), g.get_particles())
+                assert_int_equal(f"frame {f} vehicle {k} ancestors", batch.get_resampled_indices(ids[k]),
+                                 g.get_resampled_indices())

@@ -106,3 +106,34 @@ def test_two_sum_is_error_free_and_compensated_add_is_accurate():
         exact = sum(Fraction(float(np.float32(v))) for v in pa + pb)
         assert abs(Fraction(float(hi)) + Fraction(float(lo)) - exact) <= exact * Fraction(1, 2 ** 44)
         assert abs(lo) <= abs(hi) * 2.0 ** -24
+
+
+@pytest.mark.parametrize("mode", ["exact", "fp64", "fp32c"])
+def test_systematic_draws_give_floor_or_ceil_copies(mode):
+    """Systematic resampling: particle i gets floor(N w_i / W) or ceil(N w_i / W) copies, ancestors
+    are sorted, and the offset changes with the epoch."""
+    rng = np.random.default_rng(7)
+    n, b = 6000, 600
+    w = rng.gamma(0.5, size=n).astype(np.float32)
+    bw = rng.gamma(0.5, size=b).astype(np.float32)
+    w[::11] = 0.0
+    p = small_params(gs=16, res=0.5, n=n, b=b)
+    outs = []
+    for epoch in (0, 1):
+        o = OracleDOGM(p, seed=SEED, instance=3)
+        o.set_cdf_mode(mode)
+        o.set_draw_mode("systematic")
+        o.set_weight_array(w)
+        o.set_birth_particles(weight=bw)
+        o.epoch = epoch
+        o.resampling()
+        anc = o.get_resampled_indices()
+        o.close()
+        assert np.all(np.diff(anc) >= 0)
+        wj = np.concatenate([w, bw]).astype(np.float64)
+        x = n * wj / wj.sum()
+        copies = np.bincount(anc, minlength=n + b)
+        assert np.all(copies >= np.floor(x - 1e-9)) and np.all(copies <= np.ceil(x + 1e-9))
+        assert copies[::11][: n // 11].sum() == 0  # zero weights are never drawn
+        outs.append(anc)
+    assert not np.array_equal(outs[0], outs[1])
