@@ -506,6 +506,13 @@ int dogm_get_cells_with_velocity(dogm_handle* h, float min_occupancy, float min_
     return st ? st : extract_read_velocity(&h->eng, &h->veh, out, capacity, count);
 }
 
+int dogm_set_graph(dogm_handle* h, int enable)
+{
+    NEED_H(h);
+    h->eng.use_graph = enable != 0;
+    return DOGM_OK;
+}
+
 int dogm_set_resampling_mode(dogm_handle* h, int mode)
 {
     NEED_H(h);
@@ -815,6 +822,7 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
         }
         for (int k = 0; k < K_COUNT; ++k)
             b->aux.timing_on[k] = E->timing_on[k];
+        b->aux.use_graph = E->use_graph;
         b->aux_ready = true;
     }
     Engine* lanes[2] = {E, &b->aux};
@@ -887,6 +895,15 @@ int dogm_batch_set_ego_mode(dogm_batch* b, int vehicle_id, int mode)
     if (!v)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
     return set_ego_mode(&b->eng, v, mode);
+}
+
+int dogm_batch_set_graph(dogm_batch* b, int enable)
+{
+    if (!b)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    b->eng.use_graph = enable != 0;
+    b->aux.use_graph = enable != 0;
+    return DOGM_OK;
 }
 
 int dogm_batch_set_lanes(dogm_batch* b, int lanes)

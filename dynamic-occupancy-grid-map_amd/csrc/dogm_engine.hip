@@ -192,6 +192,9 @@ void engine_release(Engine* E)
 {
     if (E->stream)
         (void)hipStreamSynchronize(E->stream);
+    if (E->graph_exec)
+        (void)hipGraphExecDestroy(E->graph_exec);
+    E->graph_exec = nullptr;
     if (E->h_desc)
         (void)hipHostFree(E->h_desc);
     if (E->h_upre)
@@ -317,6 +320,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
             return set_err(&E->last_error, st, why);
         E->passes = passes;
         E->m = m;
+        ++E->plan_epoch;
         E->plan_veh.assign(m, 0);
         for (int i = 0; i < m; ++i)
             E->plan_veh[i] = veh[i]->uid;
@@ -434,39 +438,93 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     bool any_init = false;
     for (int i = 0; i < m; ++i)
         any_init = any_init || ((stages & ST_INIT) && veh[i]->init_pending);
+    int modes = 0;
+    for (int i = 0; i < m; ++i)
+        modes |= 1 << veh[i]->cdf_mode;
+    bool timing = false;
+    for (int k = 0; k < K_COUNT; ++k)
+        timing = timing || E->timing_on[k];
+    // Launch signature of the step: everything besides the plan that selects kernels or their
+    // by-value arguments (descriptors live in device memory and are uploaded before the launches).
+    uint32_t occ_bits = 0, vel_bits = 0;
+    std::memcpy(&occ_bits, &E->ext.min_occ, 4);
+    std::memcpy(&vel_bits, &E->ext.min_vel, 4);
+    const uint64_t sig[3] = {(uint64_t)stages | ((uint64_t)any_init << 16) | ((uint64_t)modes << 17) |
+                                 ((uint64_t)(stages & ST_EXTRACT ? E->ext.what : 0) << 24) |
+                                 ((uint64_t)(stages & ST_EXTRACT ? E->ext.source : 0) << 32),
+                             (stages & ST_EXTRACT) ? ((uint64_t)occ_bits << 32 | vel_bits) : 0, 0};
+    const bool graph = E->use_graph && !timing;
+    bool replay = graph && E->graph_exec && E->graph_plan == E->plan_epoch && sig[0] == E->graph_sig[0] &&
+                  sig[1] == E->graph_sig[1];
+    if (graph && !replay)
+    {
+        if (E->graph_exec)
+            (void)hipGraphExecDestroy(E->graph_exec);
+        E->graph_exec = nullptr;
+        if (hipStreamBeginCapture(E->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "hipStreamBeginCapture failed");
+    }
     int pph = 0;
     const bool hist = (stages & ST_PREDICT) && (stages & ST_ASSIGN);
     const bool occ_pers = (stages & ST_OCC) && (stages & ST_PERS);
     const bool assign_occ = (stages & ST_ASSIGN) && (stages & ST_OCC);
     const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
     const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
-    if (any_init)
-        run_init_particles(E);
-    if (stages & ST_PREDICT)
-        run_predict(E, pph, hist);
+    if (!replay)
+    {
+        // pph below follows the same sequence in capture and replay
+        if (any_init)
+            run_init_particles(E);
+        if (stages & ST_PREDICT)
+            run_predict(E, pph, hist);
+    }
     if (stages & ST_ASSIGN)
     {
-        run_assignment(E, pph, hist, assign_occ);
+        if (!replay)
+            run_assignment(E, pph, hist, assign_occ);
         pph ^= 1;
     }
-    if (stages & ST_OCC)
-        run_occupancy(E, pph, zero_empty, occ_pers, assign_occ);
-    if (stages & ST_PERS)
-        run_persistent(E, pph, occ_pers, (zero_empty & 1) != 0, pers_mom);
-    if (stages & ST_BIRTH)
-        run_births(E);
-    if (stages & ST_MOMENTS)
-        run_moments(E, pph, pers_mom, (zero_empty & 2) != 0);
+    if (!replay)
+    {
+        if (stages & ST_OCC)
+            run_occupancy(E, pph, zero_empty, occ_pers, assign_occ);
+        if (stages & ST_PERS)
+            run_persistent(E, pph, occ_pers, (zero_empty & 1) != 0, pers_mom);
+        if (stages & ST_BIRTH)
+            run_births(E);
+        if (stages & ST_MOMENTS)
+            run_moments(E, pph, pers_mom, (zero_empty & 2) != 0);
+    }
     if (stages & ST_RESAMPLE)
     {
-        int modes = 0;
-        for (int i = 0; i < m; ++i)
-            modes |= 1 << veh[i]->cdf_mode;
-        run_resampling(E, pph, modes);
+        if (!replay)
+            run_resampling(E, pph, modes);
         pph ^= 1;
     }
-    if (stages & ST_EXTRACT)
+    if ((stages & ST_EXTRACT) && !replay)
         run_extract(E, (stages & ST_OCC) ? 1 : 0);
+    if (graph)
+    {
+        if (!replay)
+        {
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(E->stream, &g) != hipSuccess || !g)
+                return set_err(&E->last_error, DOGM_ERR_HIP, "capturing the step graph failed");
+            const hipError_t ie = hipGraphInstantiateWithFlags(&E->graph_exec, g, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess)
+            {
+                E->graph_exec = nullptr;
+                return set_err(&E->last_error, DOGM_ERR_HIP, std::string("instantiating the step graph: ") +
+                                                                 hipGetErrorString(ie));
+            }
+            E->graph_plan = E->plan_epoch;
+            E->graph_sig[0] = sig[0];
+            E->graph_sig[1] = sig[1];
+        }
+        if (hipGraphLaunch(E->graph_exec, E->stream) != hipSuccess)
+            return set_err(&E->last_error, DOGM_ERR_HIP, "launching the step graph failed");
+    }
     // Host state: the descriptors already hold this step's buffers, so swaps take effect for the
     // next step.
     for (int i = 0; i < m; ++i)

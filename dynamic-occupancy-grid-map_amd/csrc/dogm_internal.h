@@ -337,6 +337,14 @@ struct Engine
 
     ExtractCfg ext;  // read by steps with ST_EXTRACT
 
+    // Step graphs: with use_graph, a step's kernel sequence is captured once into a HIP graph and
+    // replayed while the plan (vehicle list, tables, scratch) and the launch signature are unchanged.
+    bool use_graph = false;
+    long long plan_epoch = 0;  // incremented whenever the tables / scratch are rebuilt
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_plan = -1;
+    uint64_t graph_sig[3] = {0, 0, 0};
+
     std::string last_error;
 
     const uint32_t* tab(int s) const { return d_tab + tab_off[s]; }

@@ -143,6 +143,8 @@ _SIGS = {
     "dogm_set_cdf_mode": (_i, [_h, _i]),
     "dogm_set_ego_mode": (_i, [_h, _i]),
     "dogm_set_resampling_mode": (_i, [_h, _i]),
+    "dogm_set_graph": (_i, [_h, _i]),
+    "dogm_batch_set_graph": (_i, [_h, _i]),
     "dogm_get_resampling_mode": (_i, [_h]),
     "dogm_batch_set_resampling_mode": (_i, [_h, _i, _i]),
     "dogm_get_ego_mode": (_i, [_h]),
@@ -425,6 +427,9 @@ class DOGM:
     def set_resampling_mode(self, mode: str) -> None:
         self._check(lib.dogm_set_resampling_mode(self.h, RESAMPLING_MODES[mode]), "setResamplingMode")
 
+    def set_graph(self, enable: bool = True) -> None:
+        self._check(lib.dogm_set_graph(self.h, int(enable)), "setGraph")
+
     def set_cdf_mode(self, mode: str) -> None:
         """Resampling CDF arithmetic: 'exact' (default), 'fp64' or 'fp32'."""
         self._check(lib.dogm_set_cdf_mode(self.h, CDF_MODES[mode]), "setCdfMode")
@@ -574,6 +579,9 @@ class Batch:
 
     def set_lanes(self, lanes: int) -> None:
         self._check(lib.dogm_batch_set_lanes(self.h, lanes), "setLanes")
+
+    def set_graph(self, enable: bool = True) -> None:
+        self._check(lib.dogm_batch_set_graph(self.h, int(enable)), "setGraph")
 
     def get_occupancy(self, vid: int) -> np.ndarray:
         out = np.empty(self.grid_cell_count(vid), np.float32)

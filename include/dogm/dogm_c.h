@@ -334,6 +334,12 @@ enum
 };
 int dogm_set_cdf_mode(dogm_handle* h, int mode);
 int dogm_get_cdf_mode(const dogm_handle* h);
+/* Step graphs (opt-in): the kernel sequence of an update step is captured once into a HIP graph
+ * and replayed while the vehicle list, its tables and the step's launch signature (stage set,
+ * first-frame initialisation, CDF modes, extraction) are unchanged; results are identical to eager
+ * launches.  Disabled while any kernel timing is on. */
+int dogm_set_graph(dogm_handle* h, int enable);
+
 /* ----------------------------------------------------------------------------------------------
  * Resampling draws.  DOGM_RESAMPLE_MULTINOMIAL (default) is the reference's: N sorted i.i.d. uniform
  * draws over the CDF (resampling.cu:17-47), generated here as exponential spacings.
@@ -419,6 +425,8 @@ int dogm_batch_set_resampling_mode(dogm_batch* b, int vehicle_id, int mode);
  * concurrently on the device, which fills one lane's small launches and tails with the other's
  * work.  Results do not depend on the lane count. */
 int dogm_batch_set_lanes(dogm_batch* b, int lanes);
+/* Step graphs for the batch (both lanes), as dogm_set_graph. */
+int dogm_batch_set_graph(dogm_batch* b, int enable);
 /* Result extraction of the grid source inside every dogm_batch_update step, for all vehicles of
  * the step in the same launches; what = OR of DOGM_EXTRACT_* bits (0 = off).  The getters below
  * return the results of the vehicle's last step, or extract them on demand (with these
