@@ -591,7 +591,7 @@ def run_precision(args) -> int:
         "value": round(min(agree["exact"]), 6),
         "unit": "fraction of ancestors equal (exact mode, worst frame)",
         "n_gpus": 1,
-        "dtype": "f32 state and weights; CDF exact / f64 / f32",
+        "dtype": "f32 state and weights; CDF exact / f64 / compensated f32 / f32",
         "data": "synthetic",
         "config": {"workload": "2048x2048 cells at 0.1 m, 8e6 particles + 8e5 birth, 60 boxes, 1440-beam 100 m lidar; "
                                "joint weights replaced by 10^(-9 U) before every resampling",
@@ -684,7 +684,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): every box in this setup has a single "
-              "MI355X; see DESIGN.md for how the update shards by vehicle across GPUs.", file=sys.stderr)
+              "MI355X. Filter instances are independent, so more GPUs would each run this engine on their own "
+              "share of vehicles.", file=sys.stderr)
         return 2
     assert args.warmup >= 2, "--warmup must be at least 2"
     if args.form == "serving":
