@@ -304,6 +304,13 @@ __device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid
     }
 }
 
+// Radix-sort tiles: SORT_ITEMS keys per thread, SORT_Q units per tile (the engine sizes the
+// per-tile digit counts for tiles of 8 units).
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
+constexpr int SORT_Q = SORT_TILE / UNIT;
+static_assert(SORT_Q == 8, "sort tiles are 8 units");
+
 // ---------------------------------------------------------------------------------------------
 // Work-table lookup: the vehicle and local element base of this workgroup's tile of q units.
 // ---------------------------------------------------------------------------------------------

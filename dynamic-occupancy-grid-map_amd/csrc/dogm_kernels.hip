@@ -205,16 +205,25 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
 constexpr int PRED_ITEMS = 8;  // particles per thread (strided by BLOCK)
 
 // With passes > 0 the workgroup also counts the radix digits of the new cell indices for every
-// sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram).
+// sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram), and
+// stores the first pass's digit counts of its tile as thist[digit * tiles + tile]: a workgroup
+// covers exactly one sort tile (PRED_ITEMS * BLOCK particles), so the sort skips that upsweep.
 __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
-                                                   int pph, int passes, unsigned* __restrict__ ghist)
+                                                   int pph, int passes, unsigned* __restrict__ ghist,
+                                                   unsigned* __restrict__ thist, int tiles)
 {
+    static_assert(PRED_ITEMS * BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
     __shared__ unsigned s_h[4][RADIX];
     const TileRef tr = tile_ref(tab, PRED_ITEMS, blockIdx.x);
     const VehDesc& d = descs[tr.v];
     const int n = d.N;
     if (tr.base >= n)
+    {
+        if (passes > 0)
+            for (int i = threadIdx.x; i < RADIX; i += BLOCK)
+                thist[(size_t)i * tiles + blockIdx.x] = 0u;
         return;
+    }
     if (passes > 0)
     {
         for (int i = threadIdx.x; i < 4 * RADIX; i += BLOCK)
@@ -277,6 +286,8 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ d
             if (v)
                 atomicAdd(&gh[i], v);
         }
+        for (int i = threadIdx.x; i < RADIX; i += BLOCK)
+            thist[(size_t)i * tiles + blockIdx.x] = s_h[0][i];
     }
 }
 
@@ -289,7 +300,7 @@ void run_predict(Engine* E, int pph, bool hist)
         run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
     launch_begin(E, K_PREDICT);
     hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                       hist ? E->passes : 0, E->sort_ghist);
+                       hist ? E->passes : 0, E->sort_ghist, E->sort_thist, blocks);
     launch_end(E, K_PREDICT);
 }
 

@@ -14,10 +14,7 @@
 namespace dogm_hip
 {
 
-constexpr int SORT_ITEMS = 8;
-constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int SORT_WAVES = BLOCK / WAVE;
-constexpr int SORT_Q = SORT_TILE / UNIT;  // units per sort tile
 
 // Lanes of the wave holding the same digit (among `active` lanes).  Nearly sorted keys usually
 // give one digit per wave: that case costs one ballot.
@@ -306,10 +303,14 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
 
     for (int ps = 0; ps < passes; ++ps)
     {
-        launch_begin(E, K_SORT_UPSWEEP);
-        hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, ps,
-                           tiles, E->sort_thist);
-        launch_end(E, K_SORT_UPSWEEP);
+        // With hist_done, predict has also stored the first pass's tile counts.
+        if (ps > 0 || !hist_done)
+        {
+            launch_begin(E, K_SORT_UPSWEEP);
+            hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                               ps, tiles, E->sort_thist);
+            launch_end(E, K_SORT_UPSWEEP);
+        }
         launch_begin(E, K_SORT_SCAN);
         hipLaunchKernelGGL(k_sort_scan, dim3(RADIX, E->m), dim3(BLOCK), 0, E->stream, E->d_desc, E->sort_thist, tiles,
                            E->sort_ghist, ps);
