@@ -12,7 +12,7 @@ consumes the next one.
    with its own Philox instance, all fed the same frame sequence; frames/s counts K frames per step.
 Prints one JSON line (stdout); per-kernel breakdowns go to stderr.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 16] [--cpu-frames 8]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 16] [--cpu-frames 24]
 """
 from __future__ import annotations
 
@@ -93,8 +93,8 @@ def frame_bytes(N: int, B: int, C: int, shifted: bool) -> int:
 
 
 def cpu_baseline(workload: str, frames: int, meas_list, poses, dt):
-    """The checker's f32 restatement timed on this host, one thread, on the first `frames`
-    frames of the same workload."""
+    """The checker's f32 restatement timed on this host's cores (its OpenMP loops over particles and
+    cells, OMP_NUM_THREADS threads) on the first `frames` frames of the same workload."""
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle_ctypes  # noqa: E402  (checker: host-core baseline only)
     import scene
@@ -112,7 +112,7 @@ def cpu_baseline(workload: str, frames: int, meas_list, poses, dt):
     return {
         "value": round(1.0 / float(np.mean(steady)), 4),
         "unit": "frames/s",
-        "cores": 1,
+        "cores": oracle_ctypes.threads(),
         "kind": "port",
         "sample": f"{frames} frames of the same workload (first frame, which also initialises the particles, "
                   f"excluded from the rate); median {1e3 * float(np.median(steady)):.1f} ms/frame",
@@ -656,7 +656,7 @@ def main() -> int:
     ap.add_argument("--workload", default="plain_full", choices=sorted(WORKLOAD_TEXT))
     ap.add_argument("--batch", type=int, default=16,
                     help="vehicles per batched step (0: plain form only); the batched form is the headline")
-    ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the host-core baseline (0: skip)")
+    ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision-frames", type=int, default=30, help="frames of the precision form")
     ap.add_argument("--accuracy-frames", type=int, default=None, help="frames of the accuracy form (default: workload's)")

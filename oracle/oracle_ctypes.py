@@ -43,6 +43,7 @@ def lib():
             "oracle_get_i32": (i, [vp, i, vp]),
             "oracle_set_i32": (i, [vp, i, vp]),
             "oracle_get_epoch": (C.c_uint32, [vp]),
+            "oracle_threads": (i, []),
             "oracle_set_epoch": (None, [vp, C.c_uint32]),
             "oracle_position_x": (f, [vp]),
             "oracle_position_y": (f, [vp]),
@@ -267,6 +268,11 @@ class OracleDOGM:
 
 
 # ---- numerics helpers for known-answer tests -------------------------------------------------------
+def threads() -> int:
+    """OpenMP threads the checker's parallel loops use (OMP_NUM_THREADS, else the host's cores)."""
+    return int(lib().oracle_threads())
+
+
 def philox(ctr, key):
     out = (C.c_uint32 * 4)()
     lib().oracle_philox(*ctr, *key, out)

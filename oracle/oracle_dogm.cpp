@@ -18,8 +18,15 @@
 //   T = float   bit-exact twin of the GPU path (same f32 operations, same order, no contraction);
 //   T = double  the same algorithm with every per-element formula in double, used to report the
 //               error of the f32 results against an FP64 evaluation.
+//
+// Loops over independent particles or cells run on OpenMP threads.  Results do not depend on the
+// thread count: every parallel loop writes disjoint elements, prefixes and sums are exact integer
+// (fixed-point) arithmetic, and the sort is a stable LSD radix sort (the permutation of a stable
+// sort is unique).
 #include "../include/dogm/dogm_c.h"
 #include "../dynamic-occupancy-grid-map_amd/csrc/dogm_numerics.h"
+
+#include <omp.h>
 
 #include <algorithm>
 #include <cmath>
@@ -31,6 +38,88 @@ using namespace dogm_num;
 
 namespace
 {
+
+// Contiguous chunk [b, e) of n elements for thread t of nt.
+inline void chunk(long long n, int t, int nt, long long* b, long long* e)
+{
+    *b = n * t / nt;
+    *e = n * (t + 1) / nt;
+}
+
+// Inclusive exact prefix out[i] = value(0) + ... + value(i).  Integer addition, so the chunked
+// parallel evaluation equals the sequential one.
+template <typename F>
+void fx_prefix(int n, F value, fx_t* out)
+{
+    std::vector<fx_t> carry((size_t)omp_get_max_threads() + 1, 0);
+#pragma omp parallel
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        long long b, e;
+        chunk(n, t, nt, &b, &e);
+        fx_t s = 0;
+        for (long long i = b; i < e; ++i)
+        {
+            s += value((int)i);
+            out[i] = s;
+        }
+        carry[t + 1] = s;
+#pragma omp barrier
+#pragma omp single
+        for (int k = 1; k <= nt; ++k)
+            carry[k] += carry[k - 1];
+        const fx_t off = carry[t];
+        for (long long i = b; i < e; ++i)
+            out[i] += off;
+    }
+}
+
+// Stable sort of 0..n-1 by key[] in [0, max_key]: LSD radix with 11-bit digits; per pass each thread
+// counts its contiguous chunk, offsets run digit-major then thread order, and each thread scatters
+// its chunk in order, so equal keys keep their input order.
+void stable_sort_by_key(const int* key, int n, int max_key, std::vector<int>& perm)
+{
+    constexpr int BITS = 11, R = 1 << BITS;
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    int bits = 0;
+    while (bits < 31 && (max_key >> bits) != 0)
+        ++bits;
+    std::vector<int> tmp(n);
+    const int nt_max = omp_get_max_threads();
+    std::vector<long long> cnt((size_t)nt_max * R);
+    for (int shift = 0; shift < bits; shift += BITS)
+    {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        int nt_used = 1;
+#pragma omp parallel
+        {
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#pragma omp single
+            nt_used = nt;
+            long long b, e;
+            chunk(n, t, nt, &b, &e);
+            long long* c = &cnt[(size_t)t * R];
+            for (long long i = b; i < e; ++i)
+                ++c[(key[perm[i]] >> shift) & (R - 1)];
+#pragma omp barrier
+#pragma omp single
+            {
+                long long run = 0;
+                for (int d = 0; d < R; ++d)
+                    for (int u = 0; u < nt_used; ++u)
+                    {
+                        const long long v = cnt[(size_t)u * R + d];
+                        cnt[(size_t)u * R + d] = run;
+                        run += v;
+                    }
+            }
+            for (long long i = b; i < e; ++i)
+                tmp[c[(key[perm[i]] >> shift) & (R - 1)]++] = perm[i];
+        }
+        perm.swap(tmp);
+    }
+}
 
 template <typename T>
 struct ParticleSet
@@ -108,15 +197,14 @@ struct OracleBase
             spac[N] = systematic_s(N, 0u);
             return;
         }
-        fx_t e = 0;
-        float e4[4];
-        for (int k = 0; k <= N; ++k)
-        {
-            if ((k & 3) == 0)
+        fx_prefix(
+            N + 1,
+            [&](int k) {
+                float e4[4];
                 resample_exponentials4(seed, instance, epoch, (uint32_t)(k >> 2), e4);
-            e += fx_from_f32(e4[k & 3]);
-            spac[k] = e;
-        }
+                return fx_from_f32(e4[k & 3]);
+            },
+            spac.data());
     }
 };
 
@@ -328,6 +416,23 @@ struct Oracle final : OracleBase
     // c + R(-yaw)(w - ref) / resolution.  A pose change applies the whole cells of the translation,
     // expressed in the new frame, and rotates by -dpsi about c = grid_size / 2: particles exactly,
     // every grid field by the nearest source cell of each destination cell centre.
+    // Copy of the cell bounds and grid fields (the source of a shift).
+    void snapshot_grid(std::vector<int>& os, std::vector<int>& oe, std::vector<T> (&og)[GF_COUNT]) const
+    {
+        os.resize(C);
+        oe.resize(C);
+        for (int f = 0; f < GF_COUNT; ++f)
+            og[f].resize(C);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < C; ++i)
+        {
+            os[i] = start[i];
+            oe[i] = end[i];
+            for (int f = 0; f < GF_COUNT; ++f)
+                og[f][i] = g[f][i];
+        }
+    }
+
     void update_pose_metric(float new_x, float new_y, float new_yaw)
     {
         if (!first_pose)
@@ -361,6 +466,7 @@ struct Oracle final : OracleBase
         const float cs = (float)std::cos(dpsi), sn = (float)std::sin(dpsi);
         const float tx = (float)(-nx), ty = (float)(-ny);
         const bool translate_only = sn == 0.0f && cs == 1.0f;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
         {
             if (translate_only)
@@ -379,10 +485,10 @@ struct Oracle final : OracleBase
             part.vx[i] = vx;
             part.vy[i] = vy;
         }
-        std::vector<int> os = start, oe = end;
+        std::vector<int> os, oe;
         std::vector<T> og[GF_COUNT];
-        for (int f = 0; f < GF_COUNT; ++f)
-            og[f] = g[f];
+        snapshot_grid(os, oe, og);
+#pragma omp parallel for schedule(static)
         for (int yy = 0; yy < gs; ++yy)
             for (int xx = 0; xx < gs; ++xx)
             {
@@ -421,15 +527,16 @@ struct Oracle final : OracleBase
         {
             const int x_move = -static_cast<int>(x_diff / p.resolution);
             const int y_move = -static_cast<int>(y_diff / p.resolution);
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < N; ++i)
             {
                 part.x[i] -= (T)(float)x_move;
                 part.y[i] -= (T)(float)y_move;
             }
-            std::vector<int> os = start, oe = end;
+            std::vector<int> os, oe;
             std::vector<T> og[GF_COUNT];
-            for (int f = 0; f < GF_COUNT; ++f)
-                og[f] = g[f];
+            snapshot_grid(os, oe, og);
+#pragma omp parallel for schedule(static)
             for (int y = 0; y < gs; ++y)
                 for (int x = 0; x < gs; ++x)
                 {
@@ -454,13 +561,9 @@ struct Oracle final : OracleBase
     void particle_orders(const std::vector<T>& masses, int v, std::vector<int>& pend)
     {
         pend.resize(C);
-        fx_t s = 0;
         std::vector<fx_t> pref(C);
-        for (int j = 0; j < C; ++j)
-        {
-            s += fx_of(masses[j]);
-            pref[j] = s;
-        }
+        fx_prefix(C, [&](int j) { return fx_of(masses[j]); }, pref.data());
+        const fx_t s = C > 0 ? pref[C - 1] : (fx_t)0;
         if (s == 0)
         {
             for (int j = 0; j < C; ++j)
@@ -468,6 +571,7 @@ struct Oracle final : OracleBase
             return;
         }
         const double scale = order_scale(s, v);
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < C; ++j)
             pend[j] = order_end(pref[j], scale, v);
         pend[C - 1] = v;
@@ -487,6 +591,7 @@ struct Oracle final : OracleBase
                 part.idx[k] = j;  // initParticlesKernel1
         const T new_weight = (T)1 / (T)N;  // 1.0f / particle_count
         const float v = p.init_max_velocity;
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < N; ++k)
         {  // initParticlesKernel2
             const int cell = part.idx[k];
@@ -508,6 +613,7 @@ struct Oracle final : OracleBase
         const T one = 1, zero = 0;
         const T sp = (T)p.stddev_process_noise_position, sv = (T)p.stddev_process_noise_velocity;
         const T ps = (T)p.persistence_prob;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
         {
             const u32x4 r = draw(seed, instance, TAG_PREDICT, epoch, (uint32_t)i);
@@ -544,11 +650,11 @@ struct Oracle final : OracleBase
     {
         std::fill(start.begin(), start.end(), -1);
         std::fill(end.begin(), end.end(), -1);
-        std::vector<int> perm(N);
-        std::iota(perm.begin(), perm.end(), 0);
-        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return part.idx[a] < part.idx[b]; });
+        std::vector<int> perm;
+        stable_sort_by_key(part.idx.data(), N, C - 1, perm);
         ParticleSet<T> s;
         s.resize(N);
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < N; ++k)
         {
             const int o = perm[k];
@@ -561,6 +667,7 @@ struct Oracle final : OracleBase
             s.assoc[k] = part.assoc[o];
         }
         part = std::move(s);
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
         {
             const int c = part.idx[i];
@@ -588,6 +695,7 @@ struct Oracle final : OracleBase
         const T alpha = freespace_factor(dt);
         const T one = 1, zero = 0;
         const T pB = (T)p.birth_prob;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < C; ++i)
         {
             const int s = start[i], e = end[i];
@@ -629,8 +737,10 @@ struct Oracle final : OracleBase
     void update_persistent_particles() override
     {
         const T zero = 0, one = 1;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
             weight_array[i] = (T)meas[part.idx[i]].likelihood * part.w[i];
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < C; ++c)
         {
             T mu_A = zero, mu_UA = zero;
@@ -645,6 +755,7 @@ struct Oracle final : OracleBase
             g[GF_MU_A][c] = mu_A;
             g[GF_MU_UA][c] = mu_UA;
         }
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
         {
             const int c = part.idx[i];
@@ -663,6 +774,7 @@ struct Oracle final : OracleBase
         const T zero = 0, one = 1;
         std::vector<int> pend;
         particle_orders(born, B, pend);
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < C; ++j)
         {
             const int s = j == 0 ? 0 : pend[j - 1];
@@ -680,6 +792,7 @@ struct Oracle final : OracleBase
             }
         }
         const T sd = (T)p.stddev_velocity;
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < B; ++k)
         {
             const int cell = birth.idx[k];
@@ -702,6 +815,7 @@ struct Oracle final : OracleBase
     {
         const T zero = 0, one = 1;
         std::vector<T> m1(N), m2(N), m3(N), m4(N), m5(N);
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i)
         {
             const T w = weight_array[i], vx = part.vx[i], vy = part.vy[i];
@@ -711,6 +825,7 @@ struct Oracle final : OracleBase
             m4[i] = w * vy * vy;
             m5[i] = w * vx * vy;
         }
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < C; ++c)
         {
             T mx = zero, my = zero, vx = zero, vy = zero, cv = zero;
@@ -749,33 +864,37 @@ struct Oracle final : OracleBase
         }
         const int J = N + B;
         std::vector<fx_t> cdf(J);
-        fx_t s = 0;
-        for (int k = 0; k < N; ++k)
-        {
-            s += fx_of(pos_weight(weight_array[k]));
-            cdf[k] = s;
-        }
-        for (int k = 0; k < B; ++k)
-        {
-            s += fx_of(pos_weight(birth.w[k]));
-            cdf[N + k] = s;
-        }
-        const fx_t total = s;
+        fx_prefix(
+            J, [&](int k) { return fx_of(pos_weight(k < N ? weight_array[k] : birth.w[k - N])); }, cdf.data());
+        const fx_t total = J > 0 ? cdf[J - 1] : (fx_t)0;
         std::vector<fx_t> spac;
         draw_prefixes(N, seed, instance, spac);
         const Spacing sp = spacing_setup(spac[N], total);
-        for (int k = 0; k < N; ++k)
+        // Lower bound of each draw in the CDF.  Draws are non-decreasing in k, so each thread takes
+        // the lower bound of its first draw and walks forward (the same indices as a binary search
+        // per draw).  Draws are < total (or 0 with total 0), so the index is always < J.
+#pragma omp parallel
         {
-            const fx_t rv = spacing_draw(spac[k], sp);
-            // The CDF is non-decreasing, so any lower bound gives the same index; draws are < total
-            // (or 0 with total 0), so the index is always < J.
-            const int lo = (int)(std::lower_bound(cdf.begin(), cdf.end(), rv) - cdf.begin());
-            ancestors[k] = lo < J ? lo : J - 1;
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+            long long b, e;
+            chunk(N, t, nt, &b, &e);
+            int lo = 0;
+            for (long long k = b; k < e; ++k)
+            {
+                const fx_t rv = spacing_draw(spac[k], sp);
+                if (k == b)
+                    lo = (int)(std::lower_bound(cdf.begin(), cdf.end(), rv) - cdf.begin());
+                else
+                    while (lo < J && cdf[lo] < rv)
+                        ++lo;
+                ancestors[k] = lo < J ? lo : J - 1;
+            }
         }
         const T joint_max = to_T_from_fx<T>(total);
         const T new_weight = joint_max / (T)N;
         ParticleSet<T> next;
         next.resize(N);
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < N; ++k)
         {
             const int a = ancestors[k];
@@ -824,6 +943,7 @@ struct Oracle final : OracleBase
         const float new_weight = joint_max / (float)N;
         ParticleSet<T> next;
         next.resize(N);
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < N; ++k)
         {
             const int a = ancestors[k];
@@ -1050,6 +1170,7 @@ int oracle_set(void* h, int field, const double* in) { return O->set(field, in);
 int oracle_get_i32(void* h, int field, int* out) { return O->get_i32(field, out); }
 int oracle_set_i32(void* h, int field, const int* in) { return O->set_i32(field, in); }
 uint32_t oracle_get_epoch(void* h) { return O->epoch; }
+int oracle_threads(void) { return omp_get_max_threads(); }
 void oracle_set_epoch(void* h, uint32_t e) { O->epoch = e; }
 float oracle_position_x(void* h) { return O->pos_x; }
 float oracle_position_y(void* h) { return O->pos_y; }
