@@ -225,6 +225,50 @@ def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
 EXTRACT_KERNELS = ("extract_mark", "extract_scan", "extract_compact")
 
 
+def _percentiles(lat) -> dict:
+    a = 1e3 * np.asarray(lat)
+    return {"samples": int(a.size), "p50_ms": round(float(np.percentile(a, 50)), 4),
+            "p99_ms": round(float(np.percentile(a, 99)), 4), "max_ms": round(float(a.max()), 4)}
+
+
+def run_latency(args, p, sc, dt, meas_bufs) -> dict:
+    """Latency percentiles over `args.latency_frames` frames of the same workload: the plain path (one
+    blocking updateGrid per frame) and the batched path (one blocking step of `args.batch` vehicles
+    per frame).  The ego pose advances every frame (a grid shift every frame); the measurement grids
+    cycle through the timed region's resident grids (the scene's boxes leave the sensor range after
+    about a minute of driving, so later frames of the scene would be empty).  The first
+    `args.warmup` frames of each path are not counted."""
+    from dogm_ctypes import Batch, DOGM
+    nf = args.latency_frames
+    bufs = [meas_bufs[f % len(meas_bufs)] for f in range(nf)]
+    poses = [sc.ego_pose(f) for f in range(nf)]
+    out = {"frames": nf, "warmup_frames_excluded": args.warmup,
+           "note": (f"one latency sample per blocking call; grids cycle through {len(meas_bufs)} resident "
+                    "grids while the ego pose advances every frame")}
+    g = DOGM(p, seed=123456)
+    lat = []
+    for f in range(nf):
+        x, y, yaw = poses[f]
+        t1 = time.perf_counter()
+        g.update_grid(bufs[f], x, y, yaw, dt)
+        lat.append(time.perf_counter() - t1)
+    g.close()
+    out["plain"] = _percentiles(lat[args.warmup:])
+    if args.batch > 0:
+        k = args.batch
+        b = Batch()
+        ids = [b.add_vehicle(p, 123456, i) for i in range(k)]
+        lat = []
+        for f in range(nf):
+            x, y, yaw = poses[f]
+            t1 = time.perf_counter()
+            b.update(ids, [bufs[f]] * k, [(x, y, yaw)] * k, dt)
+            lat.append(time.perf_counter() - t1)
+        b.close()
+        out["batched"] = dict(_percentiles(lat[args.warmup:]), vehicles=k)
+    return out
+
+
 def run_uneven(args) -> dict:
     """Config 4: `args.uneven` grids of uneven size (sides from {128, 256, 512, 768, 1024} at 0.1 m,
     N = 1.5 x cells, B = N / 10, each with its own scene, 720-beam lidar and ego motion) per
@@ -271,9 +315,23 @@ def run_uneven(args) -> dict:
         t_gen += t1 - t0
         t_upd += t2 - t1
     frames = steps * len(ids)
+    # The same steps with the update's kernel sequence replayed from a HIP graph (opt-in).
+    b.set_graph(True)
+    for k in range(2 + steps, 4 + steps):
+        generate(k)
+        update(k)
+    lib.dogm_device_synchronize()
+    t_upd_graph = 0.0
+    for k in range(4 + steps, 4 + 2 * steps):
+        generate(k)
+        lib.dogm_device_synchronize()
+        t1 = time.perf_counter()
+        update(k)
+        t_upd_graph += time.perf_counter() - t1
     b.close()
     bank.close()
     return {"vehicles": len(ids), "sides": {str(sd): len(js) for sd, js in sorted(by_side.items())}, "steps": steps,
+            "update_frames_per_s_graph": round(frames / t_upd_graph, 1),
             "frames_per_s": round(frames / (t_gen + t_upd), 1), "update_frames_per_s": round(frames / t_upd, 1),
             "ms_per_step": round((t_gen + t_upd) / steps * 1e3, 3), "update_ms_per_step": round(t_upd / steps * 1e3, 3),
             "generation_ms_per_step": round(t_gen / steps * 1e3, 3),
@@ -673,11 +731,13 @@ def main() -> int:
                     help="serving frames as lidar scans (grids generated on the device) or as ready grids")
     ap.add_argument("--max-batch", type=int, default=0, help="frames per serving step (0: no limit)")
     ap.add_argument("--uneven", type=int, default=64, help="grids of uneven size in the config-4 leg (0: skip)")
-    ap.add_argument("--uneven-steps", type=int, default=10, help="timed steps of the config-4 leg")
+    ap.add_argument("--uneven-steps", type=int, default=100, help="timed steps of the config-4 leg")
     ap.add_argument("--generation-steps", type=int, default=20,
                     help="device measurement-grid generations for the batch after the timed frames (0: skip)")
     ap.add_argument("--extract-steps", type=int, default=5,
                     help="batched steps with result extraction after the timed frames (0: skip)")
+    ap.add_argument("--latency-frames", type=int, default=1000,
+                    help="distinct frames of the latency-percentile leg after the timed region (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the frames (scales HBM counters)")
     args = ap.parse_args()
@@ -742,6 +802,11 @@ def main() -> int:
         uneven = run_uneven(args)
         print(f"[bench] uneven x{args.uneven}: {json.dumps(uneven)}", file=sys.stderr, flush=True)
 
+    latency = None
+    if args.latency_frames > args.warmup:
+        latency = run_latency(args, p, sc, dt, bufs)
+        print(f"[bench] latency: {json.dumps(latency)}", file=sys.stderr, flush=True)
+
     for tag, r in (("plain", plain), ("batched", batched)):
         if r is None:
             continue
@@ -792,8 +857,11 @@ def main() -> int:
             "measurement_in_hbm": True,
             "baseline_source": "reference README: 50 ms/frame at 2e6/2e5/1.44e6 (GTX 980, Nuss et al.)",
         },
-        "p50_ms": hs["p50_ms"],
-        "p99_ms": hs["p99_ms"],
+        # Step latency percentiles from the latency leg (>= 1000 distinct frames) when it ran, else from
+        # the timed steps.
+        "p50_ms": (latency["batched" if batched else "plain"]["p50_ms"] if latency else hs["p50_ms"]),
+        "p99_ms": (latency["batched" if batched else "plain"]["p99_ms"] if latency else hs["p99_ms"]),
+        "latency": latency,
         "plain": _summary(plain),
         "batched": _summary(batched) if batched else None,
         "batched_vs_plain": round(fps / (plain["frames"] / plain["elapsed"]), 3) if batched else None,
