@@ -480,6 +480,7 @@ int dogm_set_grid_cells(dogm_handle* h, const dogm_grid_cell* cells)
     if (st)
         return st;
     run_unpack_grid(&h->eng, &h->veh);
+    h->veh.nz_valid = false;
     return finish(&h->eng, "setGridCells");
 }
 

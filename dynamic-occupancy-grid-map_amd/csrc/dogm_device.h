@@ -311,6 +311,16 @@ constexpr int SORT_TILE = BLOCK * SORT_ITEMS;
 constexpr int SORT_Q = SORT_TILE / UNIT;
 static_assert(SORT_Q == 8, "sort tiles are 8 units");
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on XCD b % 8), each with its
+// own L2.  xcd_tile(b, n) gives XCD x a contiguous range of the n logical tiles, taken in order, so
+// tiles whose scattered writes or gathered reads share cache lines run behind the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Work-table lookup: the vehicle and local element base of this workgroup's tile of q units.
 // ---------------------------------------------------------------------------------------------

@@ -395,6 +395,8 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.x_cnt = v->x_cnt;
         d.x_off = v->x_off;
         d.x_idx = v->x_idx;
+        d.nz = v->nz;
+        d.nz_valid = v->nz_valid ? 1 : 0;
     }
     // Table offsets per vehicle (units); the prefixes are those the tables were built from.
     {
@@ -545,6 +547,8 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
             v->pend_part_shift = false;
         if (stages & ST_OCC)
             v->pend_grid_shift = false;
+        if (stages & ST_ALL)
+            v->nz_valid = zero_empty == 3;  // full updates rewrite the bits (k_mass_update)
         if (stages & ST_EXTRACT)
         {
             v->ext_what = E->ext.what;
@@ -661,6 +665,7 @@ int vehicle_set_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* 
 
 static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
 {
+    v->nz_valid = false;
     auto shift_ptr = [&](void** field) {
         run_shift_field(E, v, *field, v->field_tmp, ix, iy);
         void* old = *field;
@@ -721,6 +726,7 @@ static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, f
         xform(reinterpret_cast<void**>(&v->grid.f[GF_FREE]));
         return;
     }
+    v->nz_valid = false;
     xform(reinterpret_cast<void**>(&v->grid.start));
     xform(reinterpret_cast<void**>(&v->grid.end));
     for (int f = 0; f < GF_COUNT; ++f)
@@ -796,9 +802,9 @@ size_t vehicle_footprint(const dogm_params* p)
     const long long C = gs * gs, N = p->particle_count, B = p->new_born_particle_count;
     const long long part = 16 + 4 + 4 + 1;
     // particles x2 + births, 16 grid fields + free_alt + field_tmp + meas(16) + born + m_raw + pend,
-    // weights + ancestors + sort (16 B), CDF (16 B per joint entry).
+    // weights + ancestors + sort (16 B), CDF (16 B per joint entry), occupancy bits.
     return (size_t)(2 * N * part + B * part + C * 4 * (GF_COUNT + 2 + 2) + C * 16 + C * 12 + N * 8 + N * 16 +
-                    (N + B) * 16);
+                    (N + B) * 16 + (C + 63) / 64 * 8);
 }
 
 int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance, std::string* why)
@@ -832,7 +838,8 @@ int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t
         (st = dalloc(&v->ancestors, N, "ancestors", why, by)) ||
         (st = dalloc(&v->sort_keys[0], N, "sort", why, by)) || (st = dalloc(&v->sort_keys[1], N, "sort", why, by)) ||
         (st = dalloc(&v->sort_vals[0], N, "sort", why, by)) || (st = dalloc(&v->sort_vals[1], N, "sort", why, by)) ||
-        (st = dalloc(&v->cdf, (size_t)N + B, "cdf", why, by)))
+        (st = dalloc(&v->cdf, (size_t)N + B, "cdf", why, by)) ||
+        (st = dalloc(&v->nz, ((size_t)C + 63) / 64, "occupancy bits", why, by)))
         return st;
 
     // initGridCellsKernel (init.cu:69-84): start = end = -1, masses 0, measurement (0, 0, 1, 1);
@@ -873,7 +880,7 @@ void vehicle_free(Vehicle* v)
                     v->m_raw,    v->pend,         v->ancestors,    v->sort_keys[0], v->sort_keys[1],
                     v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp,
                     v->x_occ,    v->x_img,        v->x_mask,       v->x_cnt,        v->x_off,
-                    v->x_idx,    v->x_rec};
+                    v->x_idx,    v->x_rec,        v->nz};
     for (void* b : bufs)
         (void)hipFree(b);
     *v = Vehicle{};

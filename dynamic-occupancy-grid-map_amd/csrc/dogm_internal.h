@@ -127,6 +127,10 @@ struct VehDesc
     int* x_cnt;                   // [units] passing cells per unit
     int* x_off;                   // [units + 1] their exclusive prefix; x_off[units] = total
     int* x_idx;                   // [C] passing cells, row-major
+    // Occupancy bits of the last full update (bit c % 64 of nz[c / 64]: cell c had particles) and
+    // whether they describe the grid in memory (see Vehicle::nz_valid).
+    unsigned long long* nz;
+    int nz_valid;
 };
 
 // Scalars produced and consumed on the device within one step, one per vehicle.
@@ -239,6 +243,13 @@ struct Vehicle
     int* sort_vals[2] = {nullptr, nullptr};
     fx_t* cdf = nullptr;
     dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes (lazy)
+    // Occupancy bits written by the occupancy update of every full update step.  While nz_valid
+    // holds, a cell whose bit is clear has start = end = -1 and zero mu_A, mu_UA and velocity
+    // moments in memory, so the next full update skips rewriting them.  Anything else that writes
+    // those fields (stage-by-stage steps, eager shifts and rotations, grid-cell injection) clears
+    // nz_valid, and the next full update rewrites every cell.
+    unsigned long long* nz = nullptr;
+    bool nz_valid = false;
     size_t bytes = 0;                   // device bytes held
 
     // Result extraction buffers (allocated on first use, not part of the footprint) and what the

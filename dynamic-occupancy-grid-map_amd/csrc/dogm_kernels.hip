@@ -313,7 +313,8 @@ __global__ __launch_bounds__(BLOCK) void k_reset_bounds(const VehDesc* __restric
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
     const int c = tr.base + threadIdx.x;
-    if (c < d.C)
+    // With valid occupancy bits, cells that had no particles still hold -1.
+    if (c < d.C && (!d.nz_valid || ((d.nz[c >> 6] >> (c & 63)) & 1ull)))
     {
         d.start[c] = -1;
         d.end[c] = -1;
@@ -394,7 +395,9 @@ struct OccOp
 
 // zero_empty: bit 0 writes mu_A = mu_UA = 0, bit 1 the five velocity moments = 0 for cells without
 // particles (what the persistent-update and moments stages write there), when those stages run
-// in the same step.
+// in the same step.  With both (a full update) the kernel also writes the occupancy bits, and with
+// valid bits from the previous full update it skips the zeros of cells that were empty then (their
+// fields are still zero in memory).
 __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
                                                        const uint32_t* __restrict__ tab, int zero_empty)
 {
@@ -408,9 +411,19 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
     const float one = 1.0f;
     float m_occ_pred = 0.0f;
     const bool occupied = d.start[c] != -1;
+    bool zero_fields = !occupied && zero_empty;
+    if (zero_empty == 3)
+    {
+        // Work-table units are 256 cells, so each wave covers one aligned 64-cell word.
+        const unsigned long long was = d.nz_valid ? d.nz[c >> 6] : ~0ull;
+        zero_fields = zero_fields && ((was >> (c & 63)) & 1ull);
+        const unsigned long long now = __ballot(occupied);
+        if ((c & 63) == 0)
+            d.nz[c >> 6] = now;
+    }
     if (occupied)
         m_occ_pred = d.m_raw[c];
-    else if (zero_empty)
+    else if (zero_fields)
     {
         if (zero_empty & 1)
         {

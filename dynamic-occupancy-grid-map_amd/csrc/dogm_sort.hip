@@ -170,7 +170,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
     __shared__ int s_vals[SORT_TILE];
     __shared__ unsigned s_scan[SORT_WAVES];
 
-    const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileRef tr = tile_ref(tab, SORT_Q, tile);
     const VehDesc& dv = descs[tr.v];
     const int n = dv.N;
     const int tbase = tr.base;
@@ -184,7 +185,6 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
-    const int tile = blockIdx.x;
     for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += BLOCK)
         (&s_wcnt[0][0])[i] = 0u;
     __syncthreads();

@@ -203,3 +203,68 @@ def test_errors_are_reported():
         gpu.set_particles(grid_cell_idx=bad)
     # The handle stays usable after a rejected call.
     gpu.particle_prediction(0.1)
+
+
+def test_full_updates_mixed_with_grid_writers():
+    """Full updates skip re-zeroing cells that were empty in the previous full update; every other
+    writer of the grid must make the next full update rewrite all cells.  Mixes full updates with
+    grid-cell injection (stale non-zero moments, mu and bounds in empty cells), a stage-by-stage
+    frame, an eager pose update and the metric ego mode, comparing every field after each call."""
+    p = small_params(gs=64, res=0.25, n=20000, b=2000)
+    sc = small_scene(gs=64, res=0.25, n_boxes=6, n_beams=240, ego_speed=5.0, yaw_rate=0.2, seed=9)
+    gpu = DOGM(p, seed=21)
+    orc = OracleDOGM(p, seed=21)
+    rng = np.random.default_rng(4)
+
+    def full(f):
+        m = sc.measurement(f)
+        x, y, yaw = sc.ego_pose(f)
+        gpu.update_grid(m, x, y, yaw, 0.1)
+        orc.update_grid(m, x, y, yaw, 0.1)
+        assert_state_equal(f"full frame {f}", gpu, orc, ancestors=True)
+
+    f = 0
+    for _ in range(3):
+        full(f)
+        f += 1
+    _say("inject stale fields")
+    cells = gpu.get_grid_cells()
+    for name in ("mu_A", "mu_UA", "mean_x_vel", "mean_y_vel", "var_x_vel", "var_y_vel", "covar_xy_vel"):
+        cells[name] = rng.uniform(0.5, 2.0, cells.shape).astype(np.float32)
+    cells["start_idx"] = rng.integers(0, p.particle_count, cells.shape)
+    cells["end_idx"] = cells["start_idx"]
+    gpu.set_grid_cells(cells)
+    orc.set_grid_cells({k: cells[k] for k in cells.dtype.names})
+    for _ in range(2):
+        full(f)
+        f += 1
+    _say("stage-by-stage frame")
+    m = sc.measurement(f)
+    x, y, yaw = sc.ego_pose(f)
+    for o in (gpu, orc):
+        o.update_measurement_grid(m)
+        o.update_pose(x, y, yaw)
+        for name, args in STAGES[:2]:
+            getattr(o, name)(*args)
+    assert_state_equal(f"stages frame {f}", gpu, orc)
+    for o in (gpu, orc):
+        for name, args in STAGES[2:]:
+            getattr(o, name)(*args)
+    assert_state_equal(f"stages frame {f} end", gpu, orc, ancestors=True)
+    f += 1
+    full(f)
+    f += 1
+    _say("eager pose update")
+    x, y, yaw = sc.ego_pose(f)
+    gpu.update_pose(x + 1.0, y - 0.75, yaw)
+    orc.update_pose(x + 1.0, y - 0.75, yaw)
+    assert_state_equal("eager pose", gpu, orc)
+    for _ in range(2):
+        full(f)
+        f += 1
+    _say("metric ego mode")
+    gpu.set_ego_mode("metric")
+    orc.set_ego_mode("metric")
+    for _ in range(3):
+        full(f)
+        f += 1
