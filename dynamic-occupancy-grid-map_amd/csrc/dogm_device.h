@@ -304,6 +304,20 @@ __device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid
     }
 }
 
+// Sort values: the particle index (< 2^31) with its association flag in bit 31.
+__device__ __forceinline__ int sort_value(int p, uint8_t assoc)
+{
+    return (int)((unsigned)p | (assoc ? 0x80000000u : 0u));
+}
+__device__ __forceinline__ int sort_value_index(int v)
+{
+    return (int)((unsigned)v & 0x7fffffffu);
+}
+__device__ __forceinline__ uint8_t sort_value_assoc(int v)
+{
+    return (uint8_t)((unsigned)v >> 31);
+}
+
 // Radix-sort tiles: SORT_ITEMS keys per thread, SORT_Q units per tile (the engine sizes the
 // per-tile digit counts for tiles of 8 units).
 constexpr int SORT_ITEMS = 8;

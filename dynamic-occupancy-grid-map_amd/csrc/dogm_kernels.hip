@@ -334,12 +334,13 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
     const DevParticles src = d.part[pph];
     const DevParticles dst = d.part[pph ^ 1];
     const int* skeys = d.sort_keys[fin];
-    const int o = d.sort_vals[fin][k];
+    const int sv = d.sort_vals[fin][k];
+    const int o = sort_value_index(sv);
     const int key = skeys[k];
     const float w = src.weight[o];
     dst.state[k] = src.state[o];
     dst.weight[k] = w;
-    dst.assoc[k] = src.assoc[o];
+    dst.assoc[k] = sort_value_assoc(sv);
     dst.idx[k] = key;
     d.weight_array[k] = w;
     if (k == 0 || skeys[k - 1] != key)

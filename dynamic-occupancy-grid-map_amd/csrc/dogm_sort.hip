@@ -155,7 +155,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
 }
 
 // Stable ranking and scatter of one tile.  Pass 0 ranks the particles' cell indices with identity
-// values; later passes read the previous pass's output.
+// values carrying the particle's association flag in bit 31 (read here in particle order, so the
+// gather needs no random byte load); later passes read the previous pass's output.
 // GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
 // sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
 // weight_array; the cell bounds are then recorded by the next segmented reduction.
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         const int p = tbase + (wid * SORT_ITEMS + j) * WAVE + lane;
         const bool valid = p < n;
         key[j] = valid ? keys_in[p] : 0;
-        val[j] = valid ? (vals_in ? vals_in[p] : p) : 0;
+        val[j] = valid ? (vals_in ? vals_in[p] : sort_value(p, dv.part[pph].assoc[p])) : 0;
         const unsigned d = ((unsigned)key[j] >> shift) & (RADIX - 1);
         const uint64_t peers = match_digit(d, __ballot(valid));
         unsigned r = 0;
@@ -268,11 +269,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         {
             const DevParticles src = dv.part[pph];
             const DevParticles dst = dv.part[pph ^ 1];
-            const int o = s_vals[s];
+            const int o = sort_value_index(s_vals[s]);
             const float w = src.weight[o];
             dst.state[g] = src.state[o];
             dst.weight[g] = w;
-            dst.assoc[g] = src.assoc[o];
+            dst.assoc[g] = sort_value_assoc(s_vals[s]);
             dst.idx[g] = k;
             dv.weight_array[g] = w;
         }
