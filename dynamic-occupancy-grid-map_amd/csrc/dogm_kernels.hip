@@ -654,7 +654,8 @@ struct SinkBirth
             {
                 const int j = p0 + i;
                 e[i] = order_end_at(j, pre[i], sc, C, v);
-                const int s = j == 0 ? 0 : order_end_at(j - 1, pre[i] - own[i], sc, C, v);
+                // P_{j-1}: the previous item's end, or recomputed from the exclusive prefix.
+                const int s = j == 0 ? 0 : (i > 0 ? e[i - 1] : order_end_at(j - 1, pre[i] - own[i], sc, C, v));
                 const int n = e[i] - s;
                 if (n > 0)  // cells without birth slots keep w_A = w_UA = 0 and skip the measurement read
                 {
