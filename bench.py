@@ -12,7 +12,7 @@ consumes the next one.
    with its own Philox instance, all fed the same frame sequence; frames/s counts K frames per step.
 Prints one JSON line (stdout); per-kernel breakdowns go to stderr.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 16] [--cpu-frames 24]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload plain_full] [--batch 32] [--cpu-frames 24]
 """
 from __future__ import annotations
 
@@ -712,7 +712,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="plain_full", choices=sorted(WORKLOAD_TEXT))
-    ap.add_argument("--batch", type=int, default=16,
+    ap.add_argument("--batch", type=int, default=32,
                     help="vehicles per batched step (0: plain form only); the batched form is the headline")
     ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the host-core baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
