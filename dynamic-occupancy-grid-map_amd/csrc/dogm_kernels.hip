@@ -676,16 +676,66 @@ struct SinkBirth
 // ---------------------------------------------------------------------------------------------
 // Birth particles (dogm/src/dogm.cu:323-348, init.cu:46-67, init_new_particles.cu:157-195)
 // ---------------------------------------------------------------------------------------------
+
+// Cell of birth slot k, j = first index with pend[j] > k (pend non-decreasing, pend[C - 1] = B), for
+// the 64 consecutive slots k_first .. k_first + 63 of a wave (all lanes call it; k_last is the last
+// valid slot).  Lanes 0-31 and 32-63 run 32-ary searches for the cells of k_first and k_last (one
+// wave-wide load per round instead of one dependent load per binary-search level); when those cells
+// are fewer than 65 apart, each lane holds one pend entry of the range and finds its own cell with a
+// six-step search over the lanes, otherwise it binary-searches the range.
+__device__ int wave_slot_cells(const int* __restrict__ pend, int C, int k_first, int k_last, int k, int lane)
+{
+    const int half = lane >> 5, hl = lane & 31;
+    const int target = half ? k_last : k_first;
+    int lo = 0, hi = C - 1;  // the answer lies in [lo, hi]
+    while (__ballot(lo < hi))
+    {
+        const int step = (hi - lo + 31) >> 5;
+        const int pos = lo + (hl + 1) * step - 1;
+        const bool le = lo < hi && pos < hi && pend[pos] <= target;
+        const unsigned long long bal = __ballot(le);
+        const int cnt = __popcll((bal >> (32 * half)) & 0xffffffffull);
+        if (lo < hi)
+        {
+            const int nlo = lo + cnt * step;
+            hi = min(hi, nlo + step - 1);
+            lo = nlo;
+        }
+    }
+    const int jlo = __shfl(lo, 0, WAVE);
+    const int jhi = __shfl(lo, 32, WAVE);
+    const int r = jhi - jlo + 1;
+    if (r <= WAVE)
+    {
+        const int pv = jlo + lane <= jhi ? pend[jlo + lane] : 0x7fffffff;
+        int cnt = 0;  // entries of the range <= k
+#pragma unroll
+        for (int bit = WAVE / 2; bit > 0; bit >>= 1)
+        {
+            const int cand = cnt + bit;
+            const int v = __shfl(pv, cand - 1, WAVE);
+            if (v <= k)
+                cnt = cand;
+        }
+        return jlo + cnt;
+    }
+    return jlo + upper_bound_i32(pend + jlo, r, k);
+}
 __global__ __launch_bounds__(BLOCK) void k_birth_particles(const VehDesc* __restrict__ descs,
                                                            const uint32_t* __restrict__ tab)
 {
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
-    const int k = tr.base + threadIdx.x;
-    if (k >= d.B)
-        return;
+    const int B = d.B;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int k0 = tr.base + (threadIdx.x - lane);  // the wave's first slot
+    if (k0 >= B)
+        return;  // wave-uniform
+    const int k = k0 + lane;
     const int* pend = d.pend;
-    const int j = upper_bound_i32(pend, d.C, k);
+    const int j = wave_slot_cells(pend, d.C, k0, min(k0 + WAVE - 1, B - 1), k, lane);
+    if (k >= B)
+        return;
     const int s = j > 0 ? pend[j - 1] : 0;
     const int n = pend[j] - s;
     const int nu_A = (int)roundf((float)n * d.meas[j].p_A);
