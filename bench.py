@@ -537,15 +537,19 @@ def run_serving(args) -> int:
 
 
 def run_accuracy(args) -> int:
-    """Tracking accuracy (accuracy.run) of the plain filter in each CDF mode and of a batch of
-    vehicles, against the scene's ground-truth boxes; one JSON line."""
+    """Tracking accuracy (accuracy.run) of the plain filter in each CDF mode, of a batch of vehicles
+    and of the same vehicles served from scans, in both ego modes and sensor frames, against the
+    scene's ground-truth boxes; one JSON line."""
     import accuracy
     say = lambda m: print(f"[accuracy] {m}", file=sys.stderr, flush=True)  # noqa: E731
     rows = []
+    k = max(1, min(args.batch, 8))
     for form, mode, veh, ego, frame in (("plain", "exact", 1, "reference", "world"),
                                         ("plain", "fp64", 1, "reference", "world"),
+                                        ("plain", "fp32c", 1, "reference", "world"),
                                         ("plain", "fp32", 1, "reference", "world"),
-                                        ("batched", "exact", args.batch, "reference", "world"),
+                                        ("batched", "exact", k, "reference", "world"),
+                                        ("serving", "exact", k, "reference", "world"),
                                         ("plain", "exact", 1, "metric", "world"),
                                         ("plain", "exact", 1, "reference", "vehicle"),
                                         ("plain", "exact", 1, "metric", "vehicle")):

@@ -17,7 +17,7 @@ import sys
 import numpy as np
 
 import scene as scene_mod
-from dogm_ctypes import DOGM, EXTRACT_VELOCITY, Batch, Evaluator, Params
+from dogm_ctypes import DOGM, EXTRACT_VELOCITY, Batch, Evaluator, Lidar, Params, Server
 
 MIN_OCC, MIN_VEL = 0.7, 4.0
 
@@ -26,7 +26,9 @@ def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5,
         form: str = "plain", cdf_mode: str = "exact", vehicles: int = 1, ego_mode: str = "reference",
         sensor_frame: str = "world") -> dict:
     """form "plain": one handle; "batched": `vehicles` vehicles of the same scene (instances 0..)
-    stepped together, all scored.  sensor_frame "vehicle" turns the lidar and its grid with the ego
+    stepped together, all scored; "serving": the same vehicles admitted to a server, every frame
+    submitted as a lidar scan (the grid is generated on the device inside the step) and the cells
+    with velocity extracted inside the step (exact CDF and the reference's ego compensation).  sensor_frame "vehicle" turns the lidar and its grid with the ego
     heading (the yaw is then passed to the filter; with a world-aligned sensor the filter gets yaw 0,
     as the reference demo passes, main.cpp:91)."""
     w = scene_mod.WORKLOADS[workload]
@@ -53,6 +55,29 @@ def run(workload: str = "plain_small", frames: int | None = None, skip: int = 5,
                 cl = ev.add_step(cells, sc.objects(f))
                 per_frame.append({"frame": f, "cells": int(cells.size), "clusters": int(cl.size)})
         g.close()
+    elif form == "serving":
+        if cdf_mode != "exact" or ego_mode != "reference":
+            raise ValueError("the serving form runs the exact CDF and the reference ego compensation")
+        srv = Server(hbm_budget_bytes=256 << 30)
+        srv.set_extraction(EXTRACT_VELOCITY, MIN_OCC, MIN_VEL)  # before any admission
+        ids = [srv.admit(p, 123456, i) for i in range(vehicles)]
+        assert all(v is not None for v in ids)
+        cs, sn = sc.beam_directions()
+        with Lidar(**sc.lidar_params(), beam_cos=cs, beam_sin=sn) as lidar:
+            for f in range(frames):
+                t = f * dt
+                r = sc.ranges(f)
+                for vid in ids:
+                    srv.submit_scan(vid, lidar, r, pose(f), dt, t_arrival=t)
+                while srv.queued():
+                    srv.step(t)
+                if f >= skip:
+                    for vid in ids:
+                        cells, _ = srv.get_cells_with_velocity(vid)
+                        cl = ev.add_step(cells, sc.objects(f))
+                        per_frame.append({"frame": f, "vehicle": vid, "cells": int(cells.size),
+                                          "clusters": int(cl.size)})
+        srv.close()
     else:
         b = Batch()
         ids = [b.add_vehicle(p, 123456, i) for i in range(vehicles)]
@@ -81,8 +106,8 @@ if __name__ == "__main__":
     ap.add_argument("--workload", default="plain_small", choices=sorted(scene_mod.WORKLOADS))
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--skip", type=int, default=5)
-    ap.add_argument("--form", default="plain", choices=["plain", "batched"])
-    ap.add_argument("--cdf-mode", default="exact", choices=["exact", "fp64", "fp32"])
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "serving"])
+    ap.add_argument("--cdf-mode", default="exact", choices=["exact", "fp64", "fp32c", "fp32"])
     ap.add_argument("--vehicles", type=int, default=1)
     ap.add_argument("--ego-mode", default="reference", choices=["reference", "metric"])
     ap.add_argument("--sensor-frame", default="world", choices=["world", "vehicle"])
