@@ -367,6 +367,9 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.shifted = (stages & ST_OCC) && v->pend_grid_shift ? 1 : 0;
         d.gx = v->pend_gx;
         d.gy = v->pend_gy;
+        d.xform_p = (stages & ST_PREDICT) && v->pend_xp ? 1 : 0;
+        d.xform_g = (stages & ST_OCC) && v->pend_xg ? 1 : 0;
+        d.xt = v->pend_xt;
         d.cdf_mode = v->cdf_mode;
         d.draw_mode = v->draw_mode;
         d.part[0] = v->part[0];
@@ -544,9 +547,9 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (stages & ST_OCC)
             std::swap(v->grid.f[GF_FREE], v->free_alt);
         if (stages & ST_PREDICT)
-            v->pend_part_shift = false;
+            v->pend_part_shift = v->pend_xp = false;
         if (stages & ST_OCC)
-            v->pend_grid_shift = false;
+            v->pend_grid_shift = v->pend_xg = false;
         if (stages & ST_ALL)
             v->nz_valid = zero_empty == 3;  // full updates rewrite the bits (k_mass_update)
         if (stages & ST_EXTRACT)
@@ -686,8 +689,23 @@ static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
 // Only whole cells of the translation are applied; `ref` advances by exactly the applied distance,
 // so the remainder carries into later frames.  The rotation is applied at every yaw change:
 // particle positions and velocities continuously, grid fields by nearest-cell resampling.
+// A metric transform still pending from a pose update whose step did not consume it is applied
+// here, eagerly, before another pose update.
+static void flush_pending_xform(Engine* E, Vehicle* v)
+{
+    if (v->pend_xp)
+        run_ego_particles(E, v, v->pend_xt);
+    if (v->pend_xg)
+    {
+        run_ego_field(E, v, v->grid.f[GF_FREE], v->field_tmp, v->pend_xt);
+        std::swap(v->grid.f[GF_FREE], v->field_tmp);
+    }
+    v->pend_xp = v->pend_xg = false;
+}
+
 static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
 {
+    flush_pending_xform(E, v);
     if (!v->first_pose)
     {
         v->ref_x = new_x;
@@ -713,6 +731,14 @@ static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, f
     if (dpsi == 0.0 && nx == 0.0 && ny == 0.0)
         return;
     const EgoXform t{0.5f * (float)v->gs, (float)std::cos(dpsi), (float)std::sin(dpsi), (float)(-nx), (float)(-ny)};
+    if (lazy)
+    {
+        // Inside a frame: particles in the step's predict, the free mass (the only grid field read
+        // before being rewritten in a frame) in its occupancy update.
+        v->pend_xp = v->pend_xg = true;
+        v->pend_xt = t;
+        return;
+    }
     run_ego_particles(E, v, t);
     auto xform = [&](void** field) {
         run_ego_field(E, v, *field, v->field_tmp, t);
@@ -720,12 +746,6 @@ static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, f
         *field = v->field_tmp;
         v->field_tmp = static_cast<float*>(old);
     };
-    if (lazy)
-    {
-        // Inside a frame only the free mass is read before being rewritten.
-        xform(reinterpret_cast<void**>(&v->grid.f[GF_FREE]));
-        return;
-    }
     v->nz_valid = false;
     xform(reinterpret_cast<void**>(&v->grid.start));
     xform(reinterpret_cast<void**>(&v->grid.end));
@@ -737,6 +757,7 @@ static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, f
 // occupancy update (free mass, the only grid field read before being rewritten in a frame).
 void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy)
 {
+    flush_pending_xform(E, v);
     if (v->ego_mode == DOGM_EGO_METRIC)
     {
         ego_metric_update(E, v, new_x, new_y, new_yaw, lazy);

@@ -87,6 +87,14 @@ constexpr long long MAX_SPACE_LEN = (long long)UNIT << TAB_VEH_SHIFT;  // elemen
 // What the kernels read about one vehicle of a step.  The current particle buffer is part[0]
 // at the start of the step and the current free-mass buffer is free[0]; stages that write out of
 // place flip to index 1 and the host swaps the vehicle's buffers after the step.
+// Metric ego compensation: rotation by -dpsi about the grid centre (c) plus a whole-cell translation.
+struct EgoXform
+{
+    float c;       // grid centre in cells (grid_size / 2)
+    float cs, sn;  // cos / sin of the yaw change
+    float tx, ty;  // translation in cells
+};
+
 struct VehDesc
 {
     int gs, C, N, B;
@@ -101,6 +109,9 @@ struct VehDesc
     float mx, my;
     int shifted;             // pending grid shift (fused into the occupancy update)
     int gx, gy;
+    int xform_p;             // pending metric transform of the particles (fused into predict)
+    int xform_g;             // ... and of the free mass (fused into the occupancy update)
+    EgoXform xt;
     int cdf_mode;            // resampling CDF arithmetic (dogm_num::CdfMode)
     int draw_mode;           // resampling draws (dogm_num::DrawMode)
     int u0[SP_COUNT];        // first table unit of this vehicle per space
@@ -279,6 +290,10 @@ struct Vehicle
     float pend_px = 0.0f, pend_py = 0.0f;
     bool pend_grid_shift = false;
     int pend_gx = 0, pend_gy = 0;
+    // Metric ego mode: the transform decided by a pose update inside an update step, applied inside
+    // predict (particles) and the occupancy update (free mass) of the same step.
+    bool pend_xp = false, pend_xg = false;
+    EgoXform pend_xt{};
 };
 
 // Stage selection of one step.
@@ -450,13 +465,6 @@ void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 
 void run_sort(Engine* E, int pph, bool hist_done, bool gather);  // gather: last pass writes the particles
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
-// Metric ego compensation: rotation by -dpsi about the grid centre (c) plus a whole-cell translation.
-struct EgoXform
-{
-    float c;       // grid centre in cells (grid_size / 2)
-    float cs, sn;  // cos / sin of the yaw change
-    float tx, ty;  // translation in cells
-};
 void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t);
 void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, const EgoXform& t);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);

@@ -109,29 +109,38 @@ void run_move_particles(Engine* E, const Vehicle* v, float mx, float my)
     launch_end(E, K_MOVE_PARTICLES);
 }
 
-// Metric ego compensation of the particles: positions rotated by -dpsi about the grid centre and
-// translated by whole cells, velocities rotated (a pure translation keeps positions exact).
+// Metric ego compensation of one particle: position rotated by -dpsi about the grid centre and
+// translated by whole cells, velocity rotated (a pure translation keeps positions exact).
+__device__ __forceinline__ float4 ego_particle(float4 s, const EgoXform& t)
+{
+    if (t.sn == 0.0f && t.cs == 1.0f)
+        return make_float4(s.x + t.tx, s.y + t.ty, s.z, s.w);
+    const float dx = s.x - t.c, dy = s.y - t.c;
+    const float rx = t.cs * dx + t.sn * dy;
+    const float ry = t.cs * dy - t.sn * dx;
+    const float vx = t.cs * s.z + t.sn * s.w;
+    const float vy = t.cs * s.w - t.sn * s.z;
+    return make_float4((t.c + rx) + t.tx, (t.c + ry) + t.ty, vx, vy);
+}
+
+// Source cell of destination cell c under the transform, by nearest cell: the cell containing
+// c + R(dpsi) ((x + 0.5, y + 0.5) - c - t); -1 outside the grid.
+__device__ __forceinline__ int ego_source_cell(int c, int gs, const EgoXform& t)
+{
+    const float qx = ((float)(c % gs) + 0.5f - t.c) - t.tx;
+    const float qy = ((float)(c / gs) + 0.5f - t.c) - t.ty;
+    const float sx = floorf(t.c + (t.cs * qx - t.sn * qy));
+    const float sy = floorf(t.c + (t.sn * qx + t.cs * qy));
+    const bool in = sx >= 0.0f && sx < (float)gs && sy >= 0.0f && sy < (float)gs;
+    return in ? (int)sx + gs * (int)sy : -1;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_ego_particles(DevParticles P, int n, EgoXform t)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n)
         return;
-    float4 s = P.state[i];
-    if (t.sn == 0.0f && t.cs == 1.0f)
-    {
-        s.x = s.x + t.tx;
-        s.y = s.y + t.ty;
-    }
-    else
-    {
-        const float dx = s.x - t.c, dy = s.y - t.c;
-        const float rx = t.cs * dx + t.sn * dy;
-        const float ry = t.cs * dy - t.sn * dx;
-        const float vx = t.cs * s.z + t.sn * s.w;
-        const float vy = t.cs * s.w - t.sn * s.z;
-        s = make_float4((t.c + rx) + t.tx, (t.c + ry) + t.ty, vx, vy);
-    }
-    P.state[i] = s;
+    P.state[i] = ego_particle(P.state[i], t);
 }
 
 void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t)
@@ -143,20 +152,15 @@ void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t)
     launch_end(E, K_EGO_PARTICLES);
 }
 
-// The same transform on a grid field, by nearest cell: destination cell (x, y) takes the source cell
-// containing c + R(dpsi) ((x + 0.5, y + 0.5) - c - t); sources outside the grid give 0.
+// The same transform on a grid field (nearest source cell; sources outside the grid give 0).
 __global__ __launch_bounds__(BLOCK) void k_ego_field(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                      int gs, EgoXform t)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= gs * gs)
         return;
-    const float qx = ((float)(c % gs) + 0.5f - t.c) - t.tx;
-    const float qy = ((float)(c / gs) + 0.5f - t.c) - t.ty;
-    const float sx = floorf(t.c + (t.cs * qx - t.sn * qy));
-    const float sy = floorf(t.c + (t.sn * qx + t.cs * qy));
-    const bool in = sx >= 0.0f && sx < (float)gs && sy >= 0.0f && sy < (float)gs;
-    dst[c] = in ? src[(int)sx + gs * (int)sy] : 0u;
+    const int s = ego_source_cell(c, gs, t);
+    dst[c] = s >= 0 ? src[s] : 0u;
 }
 
 void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, const EgoXform& t)
@@ -247,6 +251,10 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ d
             {
                 s.x = s.x - d.mx;
                 s.y = s.y - d.my;
+            }
+            else if (d.xform_p)
+            {
+                s = ego_particle(s, d.xt);
             }
             const u32x4 r = draw(d.seed, d.inst, TAG_PREDICT, d.ep_pred, (uint32_t)i);
             float n0, n1, n2, n3;
@@ -443,6 +451,11 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
         const int x = c % gs, y = c / gs;
         const int sx = x + d.gx, sy = y + d.gy;
         fprev = (sx > 0 && sx < gs && sy > 0 && sy < gs) ? free_src[sx + gs * sy] : 0.0f;
+    }
+    else if (d.xform_g)
+    {
+        const int sc = ego_source_cell(c, gs, d.xt);
+        fprev = sc >= 0 ? free_src[sc] : 0.0f;
     }
     else
     {
