@@ -108,14 +108,24 @@ def cpu_baseline(workload: str, frames: int, meas_list, poses, dt):
         o.update_grid(meas_list[f], x, y, yaw, dt)
         times.append(time.perf_counter() - t0)
     o.close()
-    steady = times[1:] if len(times) > 1 else times
+    warm = min(3, len(times) - 1)
+    steady = times[warm:]
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {
         "value": round(1.0 / float(np.mean(steady)), 4),
         "unit": "frames/s",
         "cores": oracle_ctypes.threads(),
         "kind": "port",
-        "sample": f"{frames} frames of the same workload (first frame, which also initialises the particles, "
-                  f"excluded from the rate); median {1e3 * float(np.median(steady)):.1f} ms/frame",
+        "sample": f"{frames} frames of the same workload, the first {warm} (warm-up, particle initialisation) "
+                  f"excluded; median {1e3 * float(np.median(steady)):.1f} ms/frame, "
+                  f"p99 {1e3 * float(np.percentile(steady, 99)):.1f} ms/frame",
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
     }
 
 
