@@ -268,3 +268,43 @@ def test_full_updates_mixed_with_grid_writers():
     for _ in range(3):
         full(f)
         f += 1
+
+
+def test_hostile_weights_in_resampling():
+    """Resampling from injected weights with negative values, NaN, zeros, subnormals and a wide
+    dynamic range (persistent and birth weights): negative and NaN weights count as zero in the CDF on
+    both sides, so ancestors and particles stay bit-exact."""
+    p = small_params(gs=40, res=0.5, n=12000, b=1500)
+    sc = small_scene(gs=40, res=0.5)
+    gpu = DOGM(p, seed=17)
+    orc = OracleDOGM(p, seed=17)
+    rng = np.random.default_rng(8)
+    for f in range(2):
+        x, y, yaw = sc.ego_pose(f)
+        m = sc.measurement(f)
+        gpu.update_grid(m, x, y, yaw, 0.1)
+        orc.update_grid(m, x, y, yaw, 0.1)
+
+    def hostile(n):
+        w = (10.0 ** rng.uniform(-12, 2, n)).astype(np.float32)
+        k = rng.integers(0, 6, n)
+        w[k == 0] = -w[k == 0]
+        w[k == 1] = np.nan
+        w[k == 2] = 0.0
+        w[k == 3] = np.float32(1e-40)  # subnormal
+        return w
+
+    for rep in range(2):
+        _say(f"hostile weights round {rep}")
+        for o in (gpu, orc):
+            o.particle_prediction(0.1)
+            o.particle_assignment()
+        w, bw = hostile(p.particle_count), hostile(p.new_born_particle_count)
+        gpu.set_weight_array(w)
+        orc.set_weight_array(w)
+        gpu.set_birth_particles(weight=bw)
+        orc.set_birth_particles(weight=bw)
+        gpu.resampling()
+        orc.resampling()
+        assert_int_equal("ancestors", gpu.get_resampled_indices(), orc.get_resampled_indices())
+        assert_particles_equal(f"round {rep}", gpu.get_particles(), orc.get_particles())
