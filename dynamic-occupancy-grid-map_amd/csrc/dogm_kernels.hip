@@ -315,17 +315,32 @@ void run_predict(Engine* E, int pph, bool hist)
 // ---------------------------------------------------------------------------------------------
 // Assignment (dogm/src/dogm.cu:262-281, init.cu:86-93, particle_to_grid.cu:16-44)
 // ---------------------------------------------------------------------------------------------
+// start = end = -1 for every cell, four cells per thread with 16-byte stores.  With valid occupancy
+// bits only groups of four containing a cell that had particles are written (the others still hold
+// -1, and rewriting -1 is harmless).
 __global__ __launch_bounds__(BLOCK) void k_reset_bounds(const VehDesc* __restrict__ descs,
                                                         const uint32_t* __restrict__ tab)
 {
-    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const TileRef tr = tile_ref(tab, 4, blockIdx.x);
     const VehDesc& d = descs[tr.v];
-    const int c = tr.base + threadIdx.x;
-    // With valid occupancy bits, cells that had no particles still hold -1.
-    if (c < d.C && (!d.nz_valid || ((d.nz[c >> 6] >> (c & 63)) & 1ull)))
+    const int c = tr.base + 4 * threadIdx.x;
+    if (c >= d.C)
+        return;
+    if (d.nz_valid && ((d.nz[c >> 6] >> (c & 63)) & 0xfull) == 0)
+        return;
+    if (c + 4 <= d.C)
     {
-        d.start[c] = -1;
-        d.end[c] = -1;
+        const int4 m1 = make_int4(-1, -1, -1, -1);
+        *reinterpret_cast<int4*>(d.start + c) = m1;
+        *reinterpret_cast<int4*>(d.end + c) = m1;
+    }
+    else
+    {
+        for (int k = c; k < d.C; ++k)
+        {
+            d.start[k] = -1;
+            d.end[k] = -1;
+        }
     }
 }
 
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
 void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds)
 {
     launch_begin(E, K_RESET_BOUNDS);
-    hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
+    hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 4)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
     launch_end(E, K_RESET_BOUNDS);
     run_sort(E, pph, hist_done, defer_bounds);
     if (defer_bounds)
