@@ -723,17 +723,37 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
     const int per = (ntiles + BLOCK - 1) / BLOCK;
     const int b0 = t0 + min(ntiles, (int)threadIdx.x * per);
     const int b1 = t0 + min(ntiles, (int)threadIdx.x * per + per);
+    // Chunks of four independent loads (a loop-carried load per tile would wait for each).
     fx_t s = 0;
-    for (int t = b0; t < b1; ++t)
-        s += tile_sum[t];
+    for (int tc = b0; tc < b1; tc += 4)
+    {
+        fx_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            x[k] = tc + k < b1 ? tile_sum[tc + k] : (fx_t)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            s += x[k];
+    }
     fx_t carry;
     fx_t run = block_exclusive_scan_fx(s, &carry);
-    for (int t = b0; t < b1; ++t)
+    for (int tc = b0; tc < b1; tc += 4)
     {
-        tile_pre[t] = run;
-        run += tile_sum[t];
-        if (tile_incl)
-            tile_incl[t] = run;
+        fx_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            x[k] = tc + k < b1 ? tile_sum[tc + k] : (fx_t)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            if (tc + k < b1)
+            {
+                tile_pre[tc + k] = run;
+                if (tile_incl)
+                    tile_incl[tc + k] = run + x[k];
+            }
+            run += x[k];
+        }
     }
     if (threadIdx.x == 0)
     {

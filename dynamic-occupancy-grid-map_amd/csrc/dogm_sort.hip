@@ -113,8 +113,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
     if (threadIdx.x < WAVE)
     {
         unsigned b = 0;
-        for (int e = lane; e < d; e += WAVE)
-            b += ghist_pass[e];
+#pragma unroll
+        for (int k = 0; k < RADIX / WAVE; ++k)  // independent loads
+        {
+            const int e = lane + k * WAVE;
+            b += e < d ? ghist_pass[e] : 0u;
+        }
 #pragma unroll
         for (int s = WAVE / 2; s > 0; s >>= 1)
             b += __shfl_xor(b, s, WAVE);
@@ -127,9 +131,18 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
     const int per = (nt + BLOCK - 1) / BLOCK;
     const int b0 = min(nt, (int)threadIdx.x * per);
     const int b1 = min(nt, b0 + per);
+    // Chunks of eight independent loads (a loop-carried load per tile would wait for each).
     unsigned sum = 0;
-    for (int t = b0; t < b1; ++t)
-        sum += row[t];
+    for (int tc = b0; tc < b1; tc += 8)
+    {
+        unsigned x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            x[k] = tc + k < b1 ? row[tc + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            sum += x[k];
+    }
     unsigned incl = sum;
 #pragma unroll
     for (int s = 1; s < WAVE; s <<= 1)
@@ -146,11 +159,19 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
     for (int w = 0; w < SORT_WAVES; ++w)
         if (w < wid)
             run += s_w[w];
-    for (int t = b0; t < b1; ++t)
+    for (int tc = b0; tc < b1; tc += 8)
     {
-        const unsigned x = row[t];
-        row[t] = run;
-        run += x;
+        unsigned x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            x[k] = tc + k < b1 ? row[tc + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            if (tc + k < b1)
+                row[tc + k] = run;
+            run += x[k];
+        }
     }
 }
 
