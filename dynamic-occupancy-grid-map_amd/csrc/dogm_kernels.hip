@@ -206,31 +206,34 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
     return 0;
 }
 
-constexpr int PRED_ITEMS = 8;  // particles per thread (strided by BLOCK)
+// 512-thread workgroups of 4 particles per thread: a single 2e6-particle grid still gets about 8
+// waves per SIMD to hide the Philox / Box-Muller latency.
+constexpr int PRED_ITEMS = 4;  // particles per thread (strided by PRED_BLOCK)
+constexpr int PRED_BLOCK = 512;
 
 // With passes > 0 the workgroup also counts the radix digits of the new cell indices for every
 // sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram), and
 // stores the first pass's digit counts of its tile as thist[digit * tiles + tile]: a workgroup
-// covers exactly one sort tile (PRED_ITEMS * BLOCK particles), so the sort skips that upsweep.
-__global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
+// covers exactly one sort tile (PRED_ITEMS * PRED_BLOCK particles), so the sort skips that upsweep.
+__global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
                                                    int pph, int passes, unsigned* __restrict__ ghist,
                                                    unsigned* __restrict__ thist, int tiles)
 {
-    static_assert(PRED_ITEMS * BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
+    static_assert(PRED_ITEMS * PRED_BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
     __shared__ unsigned s_h[4][RADIX];
-    const TileRef tr = tile_ref(tab, PRED_ITEMS, blockIdx.x);
+    const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
     const VehDesc& d = descs[tr.v];
     const int n = d.N;
     if (tr.base >= n)
     {
         if (passes > 0)
-            for (int i = threadIdx.x; i < RADIX; i += BLOCK)
+            for (int i = threadIdx.x; i < RADIX; i += PRED_BLOCK)
                 thist[(size_t)i * tiles + blockIdx.x] = 0u;
         return;
     }
     if (passes > 0)
     {
-        for (int i = threadIdx.x; i < 4 * RADIX; i += BLOCK)
+        for (int i = threadIdx.x; i < 4 * RADIX; i += PRED_BLOCK)
             (&s_h[0][0])[i] = 0u;
         __syncthreads();
     }
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ d
     const float zero = 0.0f, one = 1.0f;
     for (int it = 0; it < PRED_ITEMS; ++it)
     {
-        const int i = tr.base + it * BLOCK + threadIdx.x;
+        const int i = tr.base + it * PRED_BLOCK + threadIdx.x;
         const bool valid = i < n;
         int cell = 0;
         if (valid)
@@ -288,26 +291,26 @@ __global__ __launch_bounds__(BLOCK) void k_predict(const VehDesc* __restrict__ d
     {
         __syncthreads();
         unsigned* gh = ghist + (size_t)tr.v * 4 * RADIX;
-        for (int i = threadIdx.x; i < passes * RADIX; i += BLOCK)
+        for (int i = threadIdx.x; i < passes * RADIX; i += PRED_BLOCK)
         {
             const unsigned v = (&s_h[0][0])[i];
             if (v)
                 atomicAdd(&gh[i], v);
         }
-        for (int i = threadIdx.x; i < RADIX; i += BLOCK)
+        for (int i = threadIdx.x; i < RADIX; i += PRED_BLOCK)
             thist[(size_t)i * tiles + blockIdx.x] = s_h[0][i];
     }
 }
 
 void run_predict(Engine* E, int pph, bool hist)
 {
-    const int blocks = E->tiles(SP_P, PRED_ITEMS);
+    const int blocks = E->tiles(SP_P, SORT_Q);
     if (blocks <= 0)
         return;
     if (hist)
         run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
     launch_begin(E, K_PREDICT);
-    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(PRED_BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
                        hist ? E->passes : 0, E->sort_ghist, E->sort_thist, blocks);
     launch_end(E, K_PREDICT);
 }
