@@ -245,8 +245,10 @@ static int engine_reserve(Engine* E, int m, std::string* why)
             (st = dalloc(&E->sort_ghist, (size_t)cap * 4 * 256, "sort histograms", why)) ||
             (st = dalloc(&E->scalars, cap, "step scalars", why)))
             return st;
-        if (hipMemset(E->scalars, 0, sizeof(DevScalars) * cap) != hipSuccess)
+        if (hipMemset(E->scalars, 0, sizeof(DevScalars) * cap) != hipSuccess ||
+            hipMemset(E->sort_ghist, 0, sizeof(unsigned) * (size_t)cap * 4 * 256) != hipSuccess)
             return set_err(why, DOGM_ERR_HIP, "clearing step scalars");
+        E->ghist_clean = true;
         E->m_cap = cap;
     }
     return DOGM_OK;

@@ -349,6 +349,9 @@ struct Engine
     SegCarry seg{};
     long long sort_tiles_cap = 0;
     unsigned* sort_thist = nullptr;  // [256 * sort tiles]
+    // sort_ghist is all zero: every sort clears the histograms it consumed in its last pass, so the
+    // next step's predict can count into them without a fill (false after a step that stopped early).
+    bool ghist_clean = false;
     int ghist_cap = 0;
     unsigned* sort_ghist = nullptr;  // [m * 4 * 256]
     int sc_cap = 0;

@@ -307,8 +307,9 @@ void run_predict(Engine* E, int pph, bool hist)
     const int blocks = E->tiles(SP_P, SORT_Q);
     if (blocks <= 0)
         return;
-    if (hist)
+    if (hist && !E->ghist_clean)
         run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
+    E->ghist_clean = false;  // until the sort of this step has cleared them again
     launch_begin(E, K_PREDICT);
     hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(PRED_BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
                        hist ? E->passes : 0, E->sort_ghist, E->sort_thist, blocks);

@@ -184,7 +184,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
-                                                          const unsigned* __restrict__ goff)
+                                                          const unsigned* __restrict__ goff, int passes,
+                                                          unsigned* __restrict__ ghist)
 {
     __shared__ unsigned s_wcnt[SORT_WAVES][RADIX];
     __shared__ unsigned s_digoff[RADIX];
@@ -200,6 +201,13 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
     if (tbase >= n)
         return;
     const int shift = ps * RADIX_BITS;
+    if (ps == passes - 1 && tbase == 0)
+    {
+        // The digit scans of every pass have run: clear this vehicle's histograms for the next sort.
+        unsigned* gh = ghist + (size_t)tr.v * 4 * RADIX;
+        for (int i = threadIdx.x; i < 4 * RADIX; i += BLOCK)
+            gh[i] = 0u;
+    }
     const int* keys_in = sort_keys_in(dv, pph, ps);
     const int* vals_in = ps == 0 ? nullptr : dv.sort_vals[(ps - 1) & 1];
     int* keys_out = dv.sort_keys[ps & 1];
@@ -316,7 +324,9 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
         return;
     if (!hist_done)
     {
-        run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
+        if (!E->ghist_clean)
+            run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
+        E->ghist_clean = false;
         launch_begin(E, K_SORT_HIST);
         hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph, passes,
                            E->sort_ghist);
@@ -342,12 +352,13 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
         launch_begin(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
         if (g)
             hipLaunchKernelGGL(k_sort_downsweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
-                               pph, ps, tiles, E->sort_thist);
+                               pph, ps, tiles, E->sort_thist, passes, E->sort_ghist);
         else
             hipLaunchKernelGGL(k_sort_downsweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
-                               E->tab(SP_P), pph, ps, tiles, E->sort_thist);
+                               E->tab(SP_P), pph, ps, tiles, E->sort_thist, passes, E->sort_ghist);
         launch_end(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
     }
+    E->ghist_clean = true;  // the last pass cleared the histograms
 }
 
 }  // namespace dogm_hip
