@@ -370,6 +370,8 @@ __device__ __forceinline__ TileRef tile_ref(const uint32_t* __restrict__ tab, in
 //   __device__ const int* keys(const VehDesc& d) const;                 // cell index per particle
 //   __device__ void head(const VehDesc& d, int key, int p) const;       // p starts the segment of key
 //   __device__ void end(const VehDesc& d, int key, int p) const;        // p ends the segment of key
+//   __device__ void cont(const VehDesc& d, int key) const;              // key's segment started in this
+//                                                                       // tile and continues past it
 //   __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS],
 //                        float (&v)[ITEMS][NCH]) const;
 //   __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[NCH]) const;
@@ -545,6 +547,8 @@ __global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const VehDesc* __res
                 uint8_t fl = in_tile ? 0 : SEG_NOHEAD;
                 if (!is_end)
                 {
+                    if (in_tile)
+                        op.cont(d, key[i]);
                     fl |= SEG_TAIL_CONT;
 #pragma unroll
                     for (int c = 0; c < NCH; ++c)
@@ -607,8 +611,9 @@ __global__ __launch_bounds__(BLOCK) void k_segfix(Op op, const VehDesc* __restri
     op.finalize(d, op.keys(d)[last], s);
 }
 
+// fix = false: the next kernel finalises the segments that cross tiles itself (from E->seg).
 template <class Op>
-inline void launch_segreduce(Engine* E, int kid, int kid_fix, const Op& op)
+inline void launch_segreduce(Engine* E, int kid, int kid_fix, const Op& op, bool fix = true)
 {
     constexpr int TILE = BLOCK * Op::ITEMS;
     const int tiles = E->tiles(SP_P, TILE / UNIT);
@@ -617,7 +622,7 @@ inline void launch_segreduce(Engine* E, int kid, int kid_fix, const Op& op)
     launch_begin(E, kid);
     hipLaunchKernelGGL(k_segreduce<Op>, dim3(tiles), dim3(BLOCK), 0, E->stream, op, E->d_desc, E->tab(SP_P), E->seg);
     launch_end(E, kid);
-    if (tiles > 1)
+    if (fix && tiles > 1)
     {
         launch_begin(E, kid_fix);
         hipLaunchKernelGGL(k_segfix<Op>, dim3(div_up(tiles, BLOCK)), dim3(BLOCK), 0, E->stream, op, E->d_desc,

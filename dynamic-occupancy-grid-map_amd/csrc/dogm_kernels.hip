@@ -419,6 +419,9 @@ struct OccOp
             v[i][0] = w[i];
     }
     __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[1]) const { d.m_raw[key] = fx_to_f32(s[0]); }
+    // A cell crossing tiles is marked with NaN (an exact sum is never NaN); the occupancy update
+    // sums its tile partials.
+    __device__ void cont(const VehDesc& d, int key) const { d.m_raw[key] = __builtin_nanf(""); }
 };
 
 // zero_empty: bit 0 writes mu_A = mu_UA = 0, bit 1 the five velocity moments = 0 for cells without
@@ -426,8 +429,12 @@ struct OccOp
 // in the same step.  With both (a full update) the kernel also writes the occupancy bits, and with
 // valid bits from the previous full update it skips the zeros of cells that were empty then (their
 // fields are still zero in memory).
+// Cells whose particles cross reduction tiles get their weight sum here from the reduction's tile
+// partials (the first tile's tail, whole middle tiles, the last tile's head), as the fix-up pass
+// would, and m_raw is written for them.
 __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
-                                                       const uint32_t* __restrict__ tab, int zero_empty)
+                                                       const uint32_t* __restrict__ tab, int zero_empty,
+                                                       SegCarry carry)
 {
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -450,7 +457,21 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
             d.nz[c >> 6] = now;
     }
     if (occupied)
+    {
+        constexpr int OT = BLOCK * OccOp::ITEMS;  // particles per reduction tile
         m_occ_pred = d.m_raw[c];
+        if (m_occ_pred != m_occ_pred)  // marked: the cell crosses reduction tiles
+        {
+            const int ts = d.start[c] / OT, te = d.end[c] / OT;
+            const int t0 = d.u0[SP_P] / (OT / UNIT);
+            fx_t sum = carry.tail_partial[t0 + ts];
+            for (int t = ts + 1; t < te; ++t)
+                sum += carry.tail_partial[t0 + t];
+            sum += carry.head_partial[t0 + te];
+            m_occ_pred = fx_to_f32(sum);
+            d.m_raw[c] = m_occ_pred;
+        }
+    }
     else if (zero_fields)
     {
         if (zero_empty & 1)
@@ -521,10 +542,10 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
 
 void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds)
 {
-    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph, bounds ? 1 : 0});
+    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph, bounds ? 1 : 0}, false);
     launch_begin(E, K_MASS_UPDATE);
     hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       zero_empty);
+                       zero_empty, E->seg);
     launch_end(E, K_MASS_UPDATE);
     if (defer_normalize)
         return;
@@ -550,6 +571,7 @@ struct PersOp
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
     __device__ void head(const VehDesc&, int, int) const {}
     __device__ void end(const VehDesc&, int, int) const {}
+    __device__ void cont(const VehDesc&, int) const {}
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS], lw[ITEMS];
@@ -865,6 +887,7 @@ struct MomOp
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
     __device__ void head(const VehDesc&, int, int) const {}
     __device__ void end(const VehDesc&, int, int) const {}
+    __device__ void cont(const VehDesc&, int) const {}
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
         float w[ITEMS];
