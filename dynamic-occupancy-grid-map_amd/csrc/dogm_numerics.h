@@ -273,27 +273,22 @@ DOGM_HD int clz128(ufx_t v)
 DOGM_HD fx_t fx_from_f32(float v)
 {
     const uint32_t b = f32_bits(v);
-    const uint32_t E = (b >> 23) & 0xffu;
-    const int sh = (int)E - 150 + FX_FRAC;  // value = m * 2^(E - 150), fixed point = m << sh
+    const int sh = (int)((b >> 23) & 0xffu) - 150 + FX_FRAC;  // value = m * 2^(E - 150), fixed point = m << sh
     const uint64_t m = (uint64_t)((b & 0x007fffffu) | 0x00800000u);
-    // Candidates for the three shift ranges (every shift count kept in [0, 63]); E >= 1 gives
-    // sh >= -61, and E = 0 is cleared below.
+    // Candidates for the three shift ranges (every shift count kept in [0, 63]).  E = 0 (zeros,
+    // subnormals) gives sh = -62, where m >> 62 = 0 clears the value without a test of its own.
     const uint64_t hi_a = m << ((sh - 64) & 63);   // 64 <= sh <= 103
     const uint64_t lo_b = m << (sh & 63);          // 0 <= sh < 64
     const uint64_t hi_b = m >> ((64 - sh) & 63);   // 41 <= sh < 64
     const uint64_t lo_c = m >> ((-sh) & 63);       // sh < 0
     uint64_t lo = sh >= 64 ? 0 : (sh >= 0 ? lo_b : lo_c);
     uint64_t hi = sh >= 64 ? hi_a : (sh > 40 ? hi_b : 0);
-    const bool sat = E == 0xffu || sh > 103;
+    const bool sat = sh > 103;  // |v| >= 2^40, infinities and NaNs (E = 255 gives sh = 193)
     lo = sat ? 0 : lo;
     hi = sat ? (1ull << 62) : hi;  // 2^126
-    lo = E == 0 ? 0 : lo;          // zero and subnormals are < 2^-126 < 2^-88
-    hi = E == 0 ? 0 : hi;
-    // Two's-complement negation for negative v: (x ^ s) - s with s = all ones.
-    const uint64_t sgn = 0ull - (uint64_t)(b >> 31);
-    const uint64_t nlo = (lo ^ sgn) - sgn;
-    const uint64_t nhi = (hi ^ sgn) - sgn - ((lo ^ sgn) < sgn ? 1u : 0u);
-    return (fx_t)(((ufx_t)nhi << 64) | (ufx_t)nlo);
+    // Two's-complement negation for negative v: (x ^ s) - s with s = all ones, on 128 bits.
+    const ufx_t sgn = (ufx_t)0 - (ufx_t)(b >> 31);
+    return (fx_t)(((((ufx_t)hi << 64) | (ufx_t)lo) ^ sgn) - sgn);
 }
 
 DOGM_HD fx_t fx_from_f64(double v)
@@ -356,15 +351,18 @@ DOGM_HD void fx_round(ufx_t mag, int mbits, uint64_t* mant_out, int* exp_out)
 }
 
 // Round-to-nearest-even of raw * 2^-88 to f32: the magnitude is normalised into one 64-bit word
-// (leading bit at 63) plus a sticky bit for what falls below it; the significand is its top 24 bits.
+// (leading bit at 63) plus a sticky bit for what falls below it.  Its top 32 bits, with every lower
+// bit folded into the lowest one, keep the round bit and a sticky bit below the 24-bit significand,
+// so the (round-to-nearest-even) u32 -> f32 conversion rounds them exactly as the whole magnitude;
+// the result's exponent is then set on the bit pattern (every non-zero fixed-point value is a
+// normal f32: 2^-88 <= |v| < 2^40).
 DOGM_HD float fx_to_f32(fx_t v)
 {
     const ufx_t u = (ufx_t)v;
-    uint64_t hi = (uint64_t)(u >> 64), lo = (uint64_t)u;
-    const uint64_t sgn = 0ull - (hi >> 63);
-    const uint64_t borrow = (lo ^ sgn) < sgn ? 1u : 0u;
-    lo = (lo ^ sgn) - sgn;
-    hi = (hi ^ sgn) - sgn - borrow;
+    const uint32_t neg = (uint32_t)(u >> 127);
+    const ufx_t sgn = (ufx_t)0 - (ufx_t)neg;
+    const ufx_t mag = (u ^ sgn) - sgn;
+    const uint64_t hi = (uint64_t)(mag >> 64), lo = (uint64_t)mag;
     if ((hi | lo) == 0)
         return 0.0f;
     uint64_t t;
@@ -384,21 +382,10 @@ DOGM_HD float fx_to_f32(fx_t v)
         sticky = false;
         msb = 63 - z;
     }
-    int e = msb - FX_FRAC;  // in [-88, 39]: always a normal f32
-    uint32_t mant = (uint32_t)(t >> 40);
-    const uint64_t rem = t & ((1ull << 40) - 1);
-    const uint64_t half = 1ull << 39;
-    if (rem > half || (rem == half && (sticky || (mant & 1u))))
-    {
-        mant += 1;
-        if (mant == (1u << 24))
-        {
-            mant >>= 1;
-            e += 1;
-        }
-    }
-    const uint32_t bits = ((uint32_t)(sgn & 1u) << 31) | ((uint32_t)(e + 127) << 23) | (mant & 0x007fffffu);
-    return f32_from_bits(bits);
+    const uint32_t w = (uint32_t)(t >> 32) | (((uint32_t)t != 0u || sticky) ? 1u : 0u);
+    // (float)w lies in [2^31, 2^32]; scale it by 2^(msb - 88 - 31) on the exponent field.
+    const uint32_t bits = f32_bits((float)w) + ((uint32_t)(msb - FX_FRAC - 31) << 23);
+    return f32_from_bits(bits | (neg << 31));
 }
 
 DOGM_HD double fx_to_f64(fx_t v)
