@@ -367,6 +367,7 @@ __device__ __forceinline__ TileRef tile_ref(const uint32_t* __restrict__ tab, in
 //
 // Op interface:
 //   static constexpr int NCH, ITEMS;                                    // ITEMS even
+//   static constexpr int WAVES;                                         // waves per SIMD to fit
 //   __device__ const int* keys(const VehDesc& d) const;                 // cell index per particle
 //   __device__ void head(const VehDesc& d, int key, int p) const;       // p starts the segment of key
 //   __device__ void end(const VehDesc& d, int key, int p) const;        // p ends the segment of key
@@ -377,7 +378,8 @@ __device__ __forceinline__ TileRef tile_ref(const uint32_t* __restrict__ tab, in
 //   __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[NCH]) const;
 // ---------------------------------------------------------------------------------------------
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_segreduce(Op op, const VehDesc* __restrict__ descs,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(Op::WAVES)))
+void k_segreduce(Op op, const VehDesc* __restrict__ descs,
                                                      const uint32_t* __restrict__ tab, SegCarry carry)
 {
     constexpr int NCH = Op::NCH;

@@ -397,6 +397,7 @@ struct OccOp
 {
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
+    static constexpr int WAVES = 6;  // register budget of six waves per SIMD (no spills)
     int pph;
     int bounds;  // record start_idx / end_idx (dogm.cu:262-281) when the assignment deferred them
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
@@ -566,6 +567,7 @@ struct PersOp
 {
     static constexpr int NCH = 1;
     static constexpr int ITEMS = 8;
+    static constexpr int WAVES = 6;  // register budget of six waves per SIMD (no spills)
     int pph;
     int norm;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
@@ -882,6 +884,7 @@ struct MomOp
 {
     static constexpr int NCH = 5;
     static constexpr int ITEMS = 4;
+    static constexpr int WAVES = 3;  // 166 registers: three waves per SIMD
     int pph;
     int post;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
