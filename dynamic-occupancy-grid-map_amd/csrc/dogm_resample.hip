@@ -324,8 +324,11 @@ __device__ int find_cdf_tile(const void* __restrict__ ci, int t0, int T, typenam
     return t;
 }
 
+// Seven waves per SIMD for the one-tile exact form (72 registers; the few values spilled cost less
+// than the latency the extra wave hides); the two-tile form is bounded by its LDS.
 template <class Tr, int RS_SLICE_TILES>
-__global__ __launch_bounds__(BLOCK) void k_rs_merge(const VehDesc* __restrict__ descs,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RS_SLICE_TILES == 1 ? 7 : 4)))
+void k_rs_merge(const VehDesc* __restrict__ descs,
                                                     const uint32_t* __restrict__ tab, int pph,
                                                     const fx_t* __restrict__ exp_tile_pre,
                                                     const void* __restrict__ cdf_incl,
