@@ -187,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
                                                           const unsigned* __restrict__ goff, int passes,
                                                           unsigned* __restrict__ ghist)
 {
-    __shared__ unsigned s_wcnt[SORT_WAVES][RADIX];
+    __shared__ uint16_t s_wcnt[SORT_WAVES][RADIX];  // counts <= SORT_TILE; 16 bits keep 8 workgroups per CU
     __shared__ unsigned s_digoff[RADIX];
     __shared__ int s_keys[SORT_TILE];
     __shared__ int s_vals[SORT_TILE];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += BLOCK)
-        (&s_wcnt[0][0])[i] = 0u;
+        (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
 
     int key[SORT_ITEMS], val[SORT_ITEMS];
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
             const unsigned before = s_wcnt[wid][d];
             r = before + (unsigned)__popcll(peers & lt);
             if ((peers & lt) == 0)
-                s_wcnt[wid][d] = before + (unsigned)__popcll(peers);
+                s_wcnt[wid][d] = (uint16_t)(before + (unsigned)__popcll(peers));
         }
         rank[j] = r;
     }
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         for (int w = 0; w < SORT_WAVES; ++w)
         {
             const unsigned c = s_wcnt[w][d];
-            s_wcnt[w][d] = run;
+            s_wcnt[w][d] = (uint16_t)run;
             run += c;
         }
         unsigned incl = run;
