@@ -284,23 +284,22 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Adds one to s_h[d] for every active lane.  A wave whose keys share the digit (the usual case
-// for the upper digits: resampled particles arrive in cell order) adds once; otherwise lanes add
-// individually.
+// Adds one to s_h[d] for every active lane, with one LDS atomic per run of equal digits in
+// consecutive lanes (keys arrive in cell order, so runs are long: a cell holds about 20 particles
+// of the full workload).  Counts only: the order in which runs are added does not matter.
 __device__ __forceinline__ void wave_count(unsigned* s_h, unsigned d, bool valid)
 {
+    const int lane = threadIdx.x & (WAVE - 1);
     const uint64_t active = __ballot(valid);
-    if (active == 0)
-        return;
-    const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
-    if ((__ballot(d == first) & active) == active)
+    const unsigned prev = __shfl_up(d, 1, WAVE);
+    const bool prev_valid = lane > 0 && ((active >> (lane - 1)) & 1ull);
+    const bool head = valid && (!prev_valid || prev != d);
+    const uint64_t heads = __ballot(head);
+    if (head)
     {
-        if (valid && (active & lanemask_lt()) == 0)
-            atomicAdd(&s_h[d], (unsigned)__popcll(active));
-    }
-    else if (valid)
-    {
-        atomicAdd(&s_h[d], 1u);
+        const uint64_t above = lane == WAVE - 1 ? 0ull : heads & (~0ull << (lane + 1));
+        const uint64_t upto = above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull;
+        atomicAdd(&s_h[d], (unsigned)__popcll(active & upto & (~0ull << lane)));
     }
 }
 

@@ -87,12 +87,18 @@ __global__ __launch_bounds__(BLOCK) void k_sort_upsweep(const VehDesc* __restric
     for (int i = threadIdx.x; i < RADIX; i += BLOCK)
         s_h[i] = 0u;
     __syncthreads();
+    // Every key is loaded before the first count: the compiler keeps a load behind the LDS atomics
+    // of the previous item, which serialises the tile's loads.
+    unsigned kd[SORT_ITEMS];
+#pragma unroll
     for (int i = 0; i < SORT_ITEMS; ++i)
     {
         const int p = tr.base + i * BLOCK + threadIdx.x;
-        const bool valid = p < n;
-        wave_count(s_h, valid ? ((unsigned)keys[p] >> shift) & (RADIX - 1) : 0u, valid);
+        kd[i] = p < n ? (unsigned)keys[p] : 0xffffffffu;
     }
+#pragma unroll
+    for (int i = 0; i < SORT_ITEMS; ++i)
+        wave_count(s_h, (kd[i] >> shift) & (RADIX - 1), tr.base + i * BLOCK + (int)threadIdx.x < n);
     __syncthreads();
     for (int d = threadIdx.x; d < RADIX; d += BLOCK)
         thist[(size_t)d * tiles + blockIdx.x] = s_h[d];
