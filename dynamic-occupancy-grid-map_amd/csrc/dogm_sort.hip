@@ -295,27 +295,43 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
     __syncthreads();
 
     const int cnt = min(SORT_TILE, n - tbase);
-    for (int s = threadIdx.x; s < cnt; s += BLOCK)
+    // Every destination offset of the thread's elements is loaded before the first store (the output
+    // arrays may alias the offsets for the compiler, which would keep each load behind the previous
+    // element's stores).
+    constexpr int GB = SORT_TILE / BLOCK;
+    int kk[GB], vv[GB];
+    unsigned gg[GB];
+#pragma unroll
+    for (int b = 0; b < GB; ++b)
     {
-        const int k = s_keys[s];
-        const unsigned d = ((unsigned)k >> shift) & (RADIX - 1);
-        const unsigned g = goff[(size_t)d * tiles + tile] + ((unsigned)s - s_digoff[d]);
+        const int s = min((int)threadIdx.x + b * BLOCK, cnt - 1);
+        kk[b] = s_keys[s];
+        vv[b] = s_vals[s];
+        const unsigned d = ((unsigned)kk[b] >> shift) & (RADIX - 1);
+        gg[b] = goff[(size_t)d * tiles + tile] + ((unsigned)s - s_digoff[d]);
+    }
+#pragma unroll
+    for (int b = 0; b < GB; ++b)
+    {
+        if ((int)threadIdx.x + b * BLOCK >= cnt)
+            break;
+        const unsigned g = gg[b];
         if constexpr (GATHER)
         {
             const DevParticles src = dv.part[pph];
             const DevParticles dst = dv.part[pph ^ 1];
-            const int o = sort_value_index(s_vals[s]);
+            const int o = sort_value_index(vv[b]);
             const float w = src.weight[o];
             dst.state[g] = src.state[o];
             dst.weight[g] = w;
-            dst.assoc[g] = sort_value_assoc(s_vals[s]);
-            dst.idx[g] = k;
+            dst.assoc[g] = sort_value_assoc(vv[b]);
+            dst.idx[g] = kk[b];
             dv.weight_array[g] = w;
         }
         else
         {
-            keys_out[g] = k;
-            vals_out[g] = s_vals[s];
+            keys_out[g] = kk[b];
+            vals_out[g] = vv[b];
         }
     }
 }
