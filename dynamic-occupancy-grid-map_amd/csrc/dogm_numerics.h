@@ -388,18 +388,40 @@ DOGM_HD float fx_to_f32(fx_t v)
     return f32_from_bits(bits | (neg << 31));
 }
 
+// Round-to-nearest-even of raw * 2^-88 to f64, as fx_to_f32: the magnitude is normalised into one 64-bit
+// word with every lower bit folded into its lowest one (11 bits below the 53-bit significand keep the round
+// and sticky information), converted with one rounding (hi * 2^32 + lo, both exact), and the exponent is set
+// on the bit pattern (every non-zero fixed-point value is a normal f64).
 DOGM_HD double fx_to_f64(fx_t v)
 {
-    if (v == 0)
+    const ufx_t u = (ufx_t)v;
+    const uint64_t neg = (uint64_t)(u >> 127);
+    const ufx_t sgn = (ufx_t)0 - (ufx_t)neg;
+    const ufx_t mag = (u ^ sgn) - sgn;
+    const uint64_t hi = (uint64_t)(mag >> 64), lo = (uint64_t)mag;
+    if ((hi | lo) == 0)
         return 0.0;
-    const bool neg = v < 0;
-    const ufx_t mag = neg ? (ufx_t)0 - (ufx_t)v : (ufx_t)v;
-    uint64_t mant;
-    int e;
-    fx_round(mag, 53, &mant, &e);
-    const uint64_t b =
-        ((neg ? 1ull : 0ull) << 63) | ((uint64_t)(e + 1023) << 52) | (mant & 0x000fffffffffffffull);
-    return f64_from_bits(b);
+    uint64_t t;
+    bool sticky;
+    int msb;
+    if (hi != 0)
+    {
+        const int z = __builtin_clzll(hi);
+        t = z ? (hi << z) | (lo >> (64 - z)) : hi;
+        sticky = (z ? (lo << z) : lo) != 0;
+        msb = 127 - z;
+    }
+    else
+    {
+        const int z = __builtin_clzll(lo);
+        t = lo << z;
+        sticky = false;
+        msb = 63 - z;
+    }
+    const uint64_t w = t | (sticky ? 1ull : 0ull);
+    const double f = (double)(uint32_t)(w >> 32) * 4294967296.0 + (double)(uint32_t)w;  // in [2^63, 2^64]
+    const uint64_t bits = f64_bits(f) + ((uint64_t)(int64_t)(msb - FX_FRAC - 63) << 52);
+    return f64_from_bits(bits | (neg << 63));
 }
 
 // floor(total * u / 2^64) for total >= 0: maps a 64-bit uniform onto [0, total).
