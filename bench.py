@@ -86,6 +86,32 @@ def measured_traffic(kernel: str, workload: str, vehicles: int) -> tuple[int | N
     return profiling.lookup_traffic(str(path), name), str(path.relative_to(REPO))
 
 
+# Vector-ALU issue: a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles; 256 CUs x 4 SIMDs
+# at the 2.4 GHz peak engine clock.
+SIMDS = 1024
+ENGINE_HZ = 2.4e9
+
+
+def valu_issue(kernel: str, workload: str, vehicles: int, avg_launch_s: float) -> dict | None:
+    """Fraction of the VALU issue rate a launch of `kernel` uses: (waves x VALU instructions per wave
+    x 4 cycles) / (SIMDs x clock x launch time), with the instruction count from the committed counter
+    summary (SQ_INSTS_VALU / SQ_WAVES at the largest grid recorded, the batched launch)."""
+    import profiling
+    path = REPO / "profiles" / f"hbm_traffic_{workload}_batch{vehicles}.md"
+    name = PROFILED_NAME.get(kernel)
+    if name is None or not path.exists() or not avg_launch_s:
+        return None
+    row = profiling.lookup_valu(str(path), name)
+    if row is None:
+        return None
+    grid, valu = row
+    waves = grid // 64
+    cycles = waves * valu * 4.0
+    return {"valu_per_wave": valu, "waves": waves,
+            "frac": round(cycles / (SIMDS * ENGINE_HZ * avg_launch_s), 4),
+            "source": str(path.relative_to(REPO))}
+
+
 def frame_bytes(N: int, B: int, C: int, shifted: bool) -> int:
     """Stage-boundary algorithmic bytes of one frame: 172 N + 29 B + 128 C (+16 N + 8 C with a
     grid shift)."""
@@ -894,6 +920,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "valu_issue": valu_issue(dominant, args.workload, kveh, dom_avg_s),
             "avg_launch_us": round(dom_avg_s * 1e6, 3),
             "algorithmic_bytes_per_launch": dbytes,
             "measured_on": (f"{roof['steps']} steps of the same {kveh}-vehicle batch on one lane after the timed "

@@ -126,6 +126,27 @@ def lookup_traffic(summary_md: str, kernel_prefix: str) -> int | None:
     return int(sum(e[1] * e[2] for e in sel) / calls)
 
 
+def lookup_valu(summary_md: str, kernel_name: str) -> tuple[int, float] | None:
+    """(grid, VALU instructions per wave) of a kernel's row at the largest grid in a counter summary
+    table (kernel | grid | calls | avg us | read MB | write MB | GB/s | valu/wave | ...)."""
+    try:
+        lines = open(summary_md).read().splitlines()
+    except OSError:
+        return None
+    best = None
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) < 8 or cells[0].strip("`") != kernel_name:
+            continue
+        try:
+            grid, valu = int(cells[1]), float(cells[7])
+        except ValueError:
+            continue
+        if best is None or grid > best[0]:
+            best = (grid, valu)
+    return best
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 5 and sys.argv[1] == "trace":
         print(trace_summary(sys.argv[2], sys.argv[3], sys.argv[4]), end="")
