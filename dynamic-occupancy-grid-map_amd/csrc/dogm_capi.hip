@@ -123,19 +123,53 @@ int get_grid_cells(Engine* E, Vehicle* v, dogm_grid_cell* out)
     return st ? st : copy_out(E, out, v->aos_tmp, sizeof(dogm_grid_cell) * (size_t)v->C);
 }
 
-int get_set(Engine* E, const DevParticles& p, int n, dogm_vec4* state, int* idx, float* weight, uint8_t* assoc)
+// Device buffer of n (x, y, vx, vy) records for packing: the second particle buffer, which no step
+// reads before writing it, or a lazy buffer for a birth set larger than N.
+int state_staging(Engine* E, Vehicle* v, int n, float4** out)
+{
+    if (n <= v->N)
+    {
+        *out = reinterpret_cast<float4*>(v->part[1].pos);
+        return DOGM_OK;
+    }
+    if (!v->state_tmp)
+    {
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&v->state_tmp), sizeof(float4) * (size_t)v->B);
+        if (e != hipSuccess)
+        {
+            v->state_tmp = nullptr;
+            return fail(E, e == hipErrorOutOfMemory ? DOGM_ERR_OUT_OF_MEMORY : DOGM_ERR_HIP,
+                        std::string("allocating the particle packing buffer: ") + hipGetErrorString(e));
+        }
+    }
+    *out = v->state_tmp;
+    return DOGM_OK;
+}
+
+}  // namespace
+
+int dogm_hip::read_particles(Engine* E, Vehicle* v, const DevParticles& p, int n, dogm_vec4* state, int* idx,
+                             float* weight, uint8_t* assoc)
 {
     int st;
-    if ((state && (st = copy_out(E, state, p.state, sizeof(float4) * (size_t)n))) ||
-        (idx && (st = copy_out(E, idx, p.idx, sizeof(int) * (size_t)n))) ||
+    if (state)
+    {
+        float4* tmp = nullptr;
+        if ((st = state_staging(E, v, n, &tmp)))
+            return st;
+        run_pack_state(E, p, n, tmp);
+        if ((st = check_launch(E, "packing particle state")) || (st = copy_out(E, state, tmp, sizeof(float4) * (size_t)n)))
+            return st;
+    }
+    if ((idx && (st = copy_out(E, idx, p.idx, sizeof(int) * (size_t)n))) ||
         (weight && (st = copy_out(E, weight, p.weight, sizeof(float) * (size_t)n))) ||
         (assoc && (st = copy_out(E, assoc, p.assoc, (size_t)n))))
         return st;
     return DOGM_OK;
 }
 
-int put_set(Engine* E, const Vehicle* v, const DevParticles& p, int n, const dogm_vec4* state, const int* idx,
-            const float* weight, const uint8_t* assoc)
+int dogm_hip::write_particles(Engine* E, Vehicle* v, const DevParticles& p, int n, const dogm_vec4* state,
+                              const int* idx, const float* weight, const uint8_t* assoc)
 {
     if (idx)
         for (int i = 0; i < n; ++i)
@@ -144,13 +178,24 @@ int put_set(Engine* E, const Vehicle* v, const DevParticles& p, int n, const dog
                             "grid_cell_idx[" + std::to_string(i) + "] = " + std::to_string(idx[i]) +
                                 " is outside [0, grid_size^2)");
     int st;
-    if ((state && (st = copy_in(E, p.state, state, sizeof(float4) * (size_t)n))) ||
-        (idx && (st = copy_in(E, p.idx, idx, sizeof(int) * (size_t)n))) ||
+    if (state)
+    {
+        float4* tmp = nullptr;
+        if ((st = state_staging(E, v, n, &tmp)) || (st = copy_in(E, tmp, state, sizeof(float4) * (size_t)n)))
+            return st;
+        run_unpack_state(E, p, n, tmp);
+        if ((st = check_launch(E, "unpacking particle state")))
+            return st;
+    }
+    if ((idx && (st = copy_in(E, p.idx, idx, sizeof(int) * (size_t)n))) ||
         (weight && (st = copy_in(E, p.weight, weight, sizeof(float) * (size_t)n))) ||
         (assoc && (st = copy_in(E, p.assoc, assoc, (size_t)n))))
         return st;
     return DOGM_OK;
 }
+
+namespace
+{
 
 void release(dogm_handle* h)
 {
@@ -350,7 +395,7 @@ int dogm_update_grid(dogm_handle* h, const dogm_measurement_cell* meas, float ne
     Vehicle* v = &h->veh;
     if (!meas)
         return fail(E, DOGM_ERR_INVALID_ARGUMENT, "measurement_grid is null");
-    int st = vehicle_set_measurement(E, v, meas, is_device);
+    int st = vehicle_take_measurement(E, v, meas, is_device);
     if (st)
         return st;
     vehicle_update_pose(E, v, new_x, new_y, new_yaw, true);
@@ -446,28 +491,28 @@ int dogm_get_measurement_cells(dogm_handle* h, dogm_measurement_cell* out)
 int dogm_get_particles(dogm_handle* h, dogm_vec4* state, int* grid_cell_idx, float* weight, uint8_t* associated)
 {
     NEED_H(h);
-    return get_set(&h->eng, h->veh.part[0], h->veh.N, state, grid_cell_idx, weight, associated);
+    return read_particles(&h->eng, &h->veh, h->veh.part[0], h->veh.N, state, grid_cell_idx, weight, associated);
 }
 
 int dogm_get_birth_particles(dogm_handle* h, dogm_vec4* state, int* grid_cell_idx, float* weight,
                              uint8_t* associated)
 {
     NEED_H(h);
-    return get_set(&h->eng, h->veh.birth, h->veh.B, state, grid_cell_idx, weight, associated);
+    return read_particles(&h->eng, &h->veh, h->veh.birth, h->veh.B, state, grid_cell_idx, weight, associated);
 }
 
 int dogm_set_particles(dogm_handle* h, const dogm_vec4* state, const int* grid_cell_idx, const float* weight,
                        const uint8_t* associated)
 {
     NEED_H(h);
-    return put_set(&h->eng, &h->veh, h->veh.part[0], h->veh.N, state, grid_cell_idx, weight, associated);
+    return write_particles(&h->eng, &h->veh, h->veh.part[0], h->veh.N, state, grid_cell_idx, weight, associated);
 }
 
 int dogm_set_birth_particles(dogm_handle* h, const dogm_vec4* state, const int* grid_cell_idx, const float* weight,
                              const uint8_t* associated)
 {
     NEED_H(h);
-    return put_set(&h->eng, &h->veh, h->veh.birth, h->veh.B, state, grid_cell_idx, weight, associated);
+    return write_particles(&h->eng, &h->veh, h->veh.birth, h->veh.B, state, grid_cell_idx, weight, associated);
 }
 
 int dogm_set_grid_cells(dogm_handle* h, const dogm_grid_cell* cells)
@@ -849,7 +894,7 @@ int dogm_batch_get_particles(dogm_batch* b, int vehicle_id, dogm_vec4* state, in
     Vehicle* v = find_vehicle(b, vehicle_id);
     if (!v)
         return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "unknown vehicle id " + std::to_string(vehicle_id));
-    return get_set(&b->eng, v->part[0], v->N, state, grid_cell_idx, weight, associated);
+    return read_particles(&b->eng, v, v->part[0], v->N, state, grid_cell_idx, weight, associated);
 }
 
 int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out)

@@ -18,7 +18,7 @@ const char* const kKernelNames[K_COUNT] = {
     "birth_particles", "init_cells",     "init_particles",  "copy_masses",     "seg_moments",
     "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
     "unpack_grid",    "build_tables",    "extract_mark",    "extract_scan",    "extract_compact",
-    "extract_gather", "ego_particles",   "ego_field",
+    "extract_gather", "ego_particles",   "ego_field",       "pack_state",      "unpack_state",
 };
 
 // ---- timing ------------------------------------------------------------------------------------
@@ -144,7 +144,7 @@ long long space_len(const Vehicle* v, int s)
 
 void free_particles(DevParticles& p)
 {
-    (void)hipFree(p.state);
+    (void)hipFree(p.pos);  // pos and vel share this allocation
     (void)hipFree(p.idx);
     (void)hipFree(p.weight);
     (void)hipFree(p.assoc);
@@ -154,12 +154,13 @@ void free_particles(DevParticles& p)
 int alloc_particles(DevParticles& p, int n, std::string* why, size_t* bytes)
 {
     int st;
-    if ((st = dalloc(&p.state, n, "particle state", why, bytes)) ||
+    if ((st = dalloc(&p.pos, 2 * (size_t)n, "particle state", why, bytes)) ||
         (st = dalloc(&p.idx, n, "particle index", why, bytes)) ||
         (st = dalloc(&p.weight, n, "particle weight", why, bytes)) ||
         (st = dalloc(&p.assoc, n, "particle flags", why, bytes)))
         return st;
-    if (n > 0 && (hipMemset(p.state, 0, sizeof(float4) * n) != hipSuccess ||
+    p.vel = p.pos ? p.pos + n : nullptr;
+    if (n > 0 && (hipMemset(p.pos, 0, sizeof(float4) * n) != hipSuccess ||
                   hipMemset(p.idx, 0, sizeof(int) * n) != hipSuccess ||
                   hipMemset(p.weight, 0, sizeof(float) * n) != hipSuccess || hipMemset(p.assoc, 0, n) != hipSuccess))
         return set_err(why, DOGM_ERR_HIP, "clearing particle buffers");
@@ -199,11 +200,15 @@ void engine_release(Engine* E)
         (void)hipHostFree(E->h_desc);
     if (E->h_upre)
         (void)hipHostFree(E->h_upre);
-    void* bufs[] = {E->d_desc,          E->d_upre,           E->d_tab,           E->scan_tile_sum, E->scan_tile_pre,
-                    E->cdf_tile_incl,   E->seg.flags,        E->seg.head_partial, E->seg.tail_partial,
-                    E->sort_thist,      E->sort_ghist,       E->scalars};
-    for (void* b : bufs)
-        (void)hipFree(b);
+    E->h_desc = nullptr;
+    E->h_upre = nullptr;
+    {
+        void* bufs[] = {E->d_desc,        E->d_upre,         E->d_tab,            E->scan_tile_sum,
+                        E->scan_tile_pre, E->cdf_tile_incl,  E->seg.flags,        E->seg.head_partial,
+                        E->seg.tail_partial, E->sort_thist,  E->sort_ghist,       E->scalars};
+        for (void* b : bufs)
+            (void)hipFree(b);
+    }
     for (auto& t : E->timed_pending)
     {
         (void)hipEventDestroy(t.a);
@@ -294,32 +299,36 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
             total += tot[s];
         }
         const long long scan_tiles = std::max({tot[SP_C], tot[SP_J], tot[SP_E]}) / 4;
-        const long long seg_tiles = tot[SP_P] / 2;  // segmented reductions take 512 or more particles per tile
+        // Segmented reductions take 512 or more particles per tile; the moments carry one entry of
+        // five doubles per 256-particle chunk (in the 16-byte partial slots).
+        const long long seg_tiles = tot[SP_P];
         const long long sort_tiles = tot[SP_P] / 8;
-        if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
-            return set_err(&E->last_error, st, why);
-        long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = E->scan_tiles_cap;
-        if ((st = grow(&E->scan_tile_sum, &c1, scan_tiles, "scan", &why)) ||
-            (st = grow(&E->scan_tile_pre, &c2, scan_tiles, "scan", &why)) ||
-            (st = grow(&E->cdf_tile_incl, &c3, scan_tiles, "cdf index", &why)))
-            return set_err(&E->last_error, st, why);
-        E->scan_tiles_cap = std::min({c1, c2, c3});
-        if (seg_tiles > E->seg_tiles_cap)
         {
-            long long a = 0, b = 0, c = 0;
-            (void)hipFree(E->seg.flags);
-            (void)hipFree(E->seg.head_partial);
-            (void)hipFree(E->seg.tail_partial);
-            E->seg = SegCarry{};
-            E->seg_tiles_cap = 0;
-            if ((st = grow(&E->seg.flags, &a, seg_tiles, "segments", &why)) ||
-                (st = grow(&E->seg.head_partial, &b, seg_tiles * 5, "segments", &why)) ||
-                (st = grow(&E->seg.tail_partial, &c, seg_tiles * 5, "segments", &why)))
+            if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
                 return set_err(&E->last_error, st, why);
-            E->seg_tiles_cap = seg_tiles;
+            long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = E->scan_tiles_cap;
+            if ((st = grow(&E->scan_tile_sum, &c1, scan_tiles, "scan", &why)) ||
+                (st = grow(&E->scan_tile_pre, &c2, scan_tiles, "scan", &why)) ||
+                (st = grow(&E->cdf_tile_incl, &c3, scan_tiles, "cdf index", &why)))
+                return set_err(&E->last_error, st, why);
+            E->scan_tiles_cap = std::min({c1, c2, c3});
+            if (seg_tiles > E->seg_tiles_cap)
+            {
+                long long a = 0, b = 0, c = 0;
+                (void)hipFree(E->seg.flags);
+                (void)hipFree(E->seg.head_partial);
+                (void)hipFree(E->seg.tail_partial);
+                E->seg = SegCarry{};
+                E->seg_tiles_cap = 0;
+                if ((st = grow(&E->seg.flags, &a, seg_tiles, "segments", &why)) ||
+                    (st = grow(&E->seg.head_partial, &b, seg_tiles * 5, "segments", &why)) ||
+                    (st = grow(&E->seg.tail_partial, &c, seg_tiles * 5, "segments", &why)))
+                    return set_err(&E->last_error, st, why);
+                E->seg_tiles_cap = seg_tiles;
+            }
+            if ((st = grow(&E->sort_thist, &E->sort_tiles_cap, sort_tiles * 256, "sort", &why)))
+                return set_err(&E->last_error, st, why);
         }
-        if ((st = grow(&E->sort_thist, &E->sort_tiles_cap, sort_tiles * 256, "sort", &why)))
-            return set_err(&E->last_error, st, why);
         E->passes = passes;
         E->m = m;
         ++E->plan_epoch;
@@ -384,6 +393,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.free[0] = v->grid.f[GF_FREE];
         d.free[1] = v->free_alt;
         d.meas = v->meas;
+        d.meas_in = (stages & ST_OCC) ? v->meas_src : nullptr;
         d.weight_array = v->weight_array;
         d.born = v->born;
         d.m_raw = v->m_raw;
@@ -439,7 +449,22 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
                 return set_err(&E->last_error, st, why);
         }
     }
+    for (int i = 0; i < m; ++i)
+    {
+        // A pending device grid that this step's stages do not copy is copied here.
+        Vehicle* v = veh[i];
+        if (v->meas_src && !(stages & ST_OCC))
+        {
+            const dogm_measurement_cell* src = v->meas_src;
+            v->meas_src = nullptr;
+            const int cs = vehicle_set_measurement(E, v, src, 1);
+            if (cs != DOGM_OK)
+                return cs;
+        }
+    }
     int st = engine_plan(E, veh, m, stages);
+    for (int i = 0; i < m; ++i)
+        veh[i]->meas_src = nullptr;  // consumed by the descriptors (or dropped with the failed step)
     if (st != DOGM_OK)
         return st;
     bool any_init = false;
@@ -583,7 +608,7 @@ int frame_enqueue(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool
         Vehicle* v = veh[i];
         if (in[i].meas)
         {
-            int st = vehicle_set_measurement(E, v, in[i].meas, in[i].is_device);
+            int st = vehicle_take_measurement(E, v, in[i].meas, in[i].is_device);
             if (st != DOGM_OK)
                 return st;
         }
@@ -666,6 +691,17 @@ int vehicle_set_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* 
         v->first_meas = true;
     }
     return DOGM_OK;
+}
+
+int vehicle_take_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device)
+{
+    if (is_device && v->first_meas && !v->init_pending)
+    {
+        // The step's occupancy update copies the grid as it reads it (no separate copy).
+        v->meas_src = m;
+        return DOGM_OK;
+    }
+    return vehicle_set_measurement(E, v, m, is_device);
 }
 
 static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
@@ -901,7 +937,7 @@ void vehicle_free(Vehicle* v)
         (void)hipFree(v->grid.f[f]);
     void* bufs[] = {v->free_alt, v->field_tmp,    v->meas,         v->weight_array, v->born,
                     v->m_raw,    v->pend,         v->ancestors,    v->sort_keys[0], v->sort_keys[1],
-                    v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp,
+                    v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp,      v->state_tmp,
                     v->x_occ,    v->x_img,        v->x_mask,       v->x_cnt,        v->x_off,
                     v->x_idx,    v->x_rec,        v->nz};
     for (void* b : bufs)

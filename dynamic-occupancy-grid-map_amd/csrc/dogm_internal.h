@@ -49,9 +49,12 @@ enum GridField
     GF_COUNT
 };
 
+// pos and vel share one allocation of 16 bytes per particle (pos first); read-backs pack them into
+// the reference's (x, y, vx, vy) records.
 struct DevParticles
 {
-    float4* state;  // (x, y, vx, vy), cells and cells/s
+    float2* pos;    // (x, y), cells
+    float2* vel;    // (vx, vy), cells/s
     int* idx;       // grid cell index
     float* weight;
     uint8_t* assoc;
@@ -123,6 +126,9 @@ struct VehDesc
     float* f[GF_COUNT];      // f[GF_FREE] is not used by step kernels: see free[]
     float* free[2];
     dogm_measurement_cell* meas;
+    // A device measurement grid of this step not yet copied into meas: the occupancy update reads it
+    // from here and writes the copy (dogm.cu:205-215) as it goes, instead of a separate copy.
+    const dogm_measurement_cell* meas_in;
     float* weight_array;
     float* born;
     float* m_raw;
@@ -210,6 +216,8 @@ enum KernelId
     K_EXTRACT_GATHER,
     K_EGO_PARTICLES,
     K_EGO_FIELD,
+    K_PACK_STATE,
+    K_UNPACK_STATE,
     K_COUNT
 };
 
@@ -245,6 +253,9 @@ struct Vehicle
     float* free_alt = nullptr;
     float* field_tmp = nullptr;  // scratch for eager grid shifts
     dogm_measurement_cell* meas = nullptr;
+    // Device measurement grid of the next full update step, copied into meas by that step's
+    // occupancy update (nullptr: meas already holds the grid).
+    const dogm_measurement_cell* meas_src = nullptr;
     float* weight_array = nullptr;
     float* born = nullptr;
     float* m_raw = nullptr;  // per-cell predicted occupancy before the >1 clamp
@@ -254,6 +265,9 @@ struct Vehicle
     int* sort_vals[2] = {nullptr, nullptr};
     fx_t* cdf = nullptr;
     dogm_grid_cell* aos_tmp = nullptr;  // packing buffer for grid read-backs / writes (lazy)
+    // Packing buffer of (x, y, vx, vy) records for particle read-backs / writes: part[1] (free between
+    // steps), or this lazy buffer for a birth set larger than N.
+    float4* state_tmp = nullptr;
     // Occupancy bits written by the occupancy update of every full update step.  While nz_valid
     // holds, a cell whose bit is clear has start = end = -1 and zero mu_A, mu_UA and velocity
     // moments in memory, so the next full update skips rewriting them.  Anything else that writes
@@ -391,6 +405,7 @@ inline int div_up(long long a, long long b)
 
 // ---- engine (dogm_engine.hip) ----
 long long units_for(long long len);  // table units of a space of `len` elements (padded)
+
 int engine_init(Engine* E, std::string* why);
 void engine_release(Engine* E);
 // Builds descriptors and tables for `veh` (reusing the tables while the vehicle list is
@@ -407,6 +422,9 @@ void vehicle_free(Vehicle* v);
 // Queues the copy of a frame's measurement grid (dogm.cu:205-215); the first one arms the particle
 // initialisation of the next step.
 int vehicle_set_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device);
+// The same for a frame whose full update step follows at once: a device grid after the first is
+// copied by that step's occupancy update (Vehicle::meas_src) instead of a separate copy.
+int vehicle_take_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell* m, int is_device);
 // dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
 // occupancy update (free mass); otherwise at once.
 void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy);
@@ -473,6 +491,15 @@ void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, cons
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);
 void run_pack_grid(Engine* E, const Vehicle* v);
 void run_unpack_grid(Engine* E, const Vehicle* v);
+// (x, y, vx, vy) records <-> the pos / vel arrays of n particles.
+void run_pack_state(Engine* E, const DevParticles& p, int n, float4* out);
+void run_unpack_state(Engine* E, const DevParticles& p, int n, const float4* in);
+// Particle read-back / write of one of v's sets (part[0] or birth) in the reference's layout:
+// (x, y, vx, vy) records, cell index, weight, association; null outputs are skipped.
+int read_particles(Engine* E, Vehicle* v, const DevParticles& p, int n, dogm_vec4* state, int* idx, float* weight,
+                   uint8_t* assoc);
+int write_particles(Engine* E, Vehicle* v, const DevParticles& p, int n, const dogm_vec4* state, const int* idx,
+                    const float* weight, const uint8_t* assoc);
 // Result extraction over the step's vehicles (dogm_extract.hip); fsel selects the current free mass
 // buffer of the step's descriptors (1 after an occupancy update in the same step).
 void run_extract(Engine* E, int fsel);

@@ -95,10 +95,10 @@ __global__ __launch_bounds__(BLOCK) void k_move_particles(DevParticles P, int n,
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n)
         return;
-    float4 s = P.state[i];
+    float2 s = P.pos[i];
     s.x = s.x - mx;
     s.y = s.y - my;
-    P.state[i] = s;
+    P.pos[i] = s;
 }
 
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my)
@@ -140,7 +140,10 @@ __global__ __launch_bounds__(BLOCK) void k_ego_particles(DevParticles P, int n, 
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n)
         return;
-    P.state[i] = ego_particle(P.state[i], t);
+    const float2 p = P.pos[i], v = P.vel[i];
+    const float4 s = ego_particle(make_float4(p.x, p.y, v.x, v.y), t);
+    P.pos[i] = make_float2(s.x, s.y);
+    P.vel[i] = make_float2(s.z, s.w);
 }
 
 void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t)
@@ -248,7 +251,8 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
         int cell = 0;
         if (valid)
         {
-            float4 s = P.state[i];
+            const float2 ps = P.pos[i], vs = P.vel[i];
+            float4 s = make_float4(ps.x, ps.y, vs.x, vs.y);
             const float w_in = P.weight[i];
             if (d.move)
             {
@@ -280,7 +284,8 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
                 w = zero;
             cell = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
-            P.state[i] = o;
+            P.pos[i] = make_float2(o.x, o.y);
+            P.vel[i] = make_float2(o.z, o.w);
             P.weight[i] = w;
             P.idx[i] = cell;
         }
@@ -365,7 +370,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
     const int o = sort_value_index(sv);
     const int key = skeys[k];
     const float w = src.weight[o];
-    dst.state[k] = src.state[o];
+    dst.pos[k] = src.pos[o];
+    dst.vel[k] = src.vel[o];
     dst.weight[k] = w;
     dst.assoc[k] = sort_value_assoc(sv);
     dst.idx[k] = key;
@@ -502,7 +508,11 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
     {
         fprev = free_src[c];
     }
-    const dogm_measurement_cell mc = d.meas[c];
+    // The step's measurement grid, copied into the vehicle's buffer on the way when it comes from
+    // the caller's device buffer (dogm.cu:205-215).
+    const dogm_measurement_cell mc = d.meas_in ? d.meas_in[c] : d.meas[c];
+    if (d.meas_in)
+        d.meas[c] = mc;
     const float pB = d.pB;
     const float m_free_pred = fminf(d.alpha * fprev, one - m_occ_pred);
     const float unknown_pred = one - m_occ_pred - m_free_pred;
@@ -800,12 +810,8 @@ __global__ __launch_bounds__(BLOCK) void k_birth_particles(const VehDesc* __rest
     float n0, n1;
     box_muller(r.x, r.y, &n0, &n1);
     const int gs = d.gs;
-    float4 st;
-    st.x = (float)(j % gs) + 0.5f;
-    st.y = (float)j / (float)gs + 0.5f;
-    st.z = normal_scale(n0, 0.0f, d.sd_birth);
-    st.w = normal_scale(n1, 0.0f, d.sd_birth);
-    d.birth.state[k] = st;
+    d.birth.pos[k] = make_float2((float)(j % gs) + 0.5f, (float)j / (float)gs + 0.5f);
+    d.birth.vel[k] = make_float2(normal_scale(n0, 0.0f, d.sd_birth), normal_scale(n1, 0.0f, d.sd_birth));
     d.birth.idx[k] = j;
     d.birth.weight[k] = assoc ? d.f[GF_W_A][j] : d.f[GF_W_UA][j];
     d.birth.assoc[k] = assoc ? 1 : 0;
@@ -850,13 +856,9 @@ __global__ __launch_bounds__(BLOCK) void k_init_particles(const VehDesc* __restr
     const int j = upper_bound_i32(d.pend, d.C, k);
     const u32x4 r = draw(d.seed, d.inst, TAG_INIT_PARTICLES, d.ep_init, (uint32_t)k);
     const float v = d.vmax;
-    float4 st;
-    st.x = (float)(j % gs) + 0.5f;
-    st.y = (float)(j / gs) + 0.5f;
-    st.z = -v + (v - -v) * uniform01(r.x);
-    st.w = -v + (v - -v) * uniform01(r.y);
     const DevParticles P = d.part[0];
-    P.state[k] = st;
+    P.pos[k] = make_float2((float)(j % gs) + 0.5f, (float)(j / gs) + 0.5f);
+    P.vel[k] = make_float2(-v + (v - -v) * uniform01(r.x), -v + (v - -v) * uniform01(r.y));
     P.idx[k] = j;
     P.weight[k] = 1.0f / (float)d.N;  // dogm.cu:228
     P.assoc[k] = 0;
@@ -877,76 +879,268 @@ void run_init_particles(Engine* E)
 // ---------------------------------------------------------------------------------------------
 // Statistical moments (dogm/src/dogm.cu:350-384, statistical_moments.cu:16-106)
 // ---------------------------------------------------------------------------------------------
-// post: the posterior weights (update_persistent_particles.cu:76-87) are formed here, when the
-// persistent update ran in the same step: w = p_A mu_A lw + (1 - p_A) mu_UA w_prior, written to
-// weight_array and used for the moments.
-struct MomOp
+// Per-cell sums of the five velocity channels w vx, w vy, w vx^2, w vy^2, w vx vy (f32 products
+// widened to f64) in one fixed association, independent of the launch shape and of the other
+// vehicles of the step: a grid's sorted particles are cut into chunks of 256 (one wave64 per chunk,
+// 4 consecutive particles per lane).
+//   lane:   sequential fold restarting where a cell starts (the lane's first particle always starts it);
+//   wave:   segmented inclusive scan of the lanes' last folds in six DPP steps (row shifts 1, 2, 4, 8
+//           inside 16-lane rows, then lane 15 of rows 0 / 2 into rows 1 / 3, lane 31 into rows 2 / 3),
+//           each adding the moved value only while no cell starts in between;
+//   cell:   ending in lane l: its lane-local fold if it starts in lane l, else (scan of lane l - 1) +
+//           fold;
+//   chunks: a cell crossing chunks is the left fold ((first piece + whole middle chunk) + ...) + last
+//           piece, summed by k_moments_fix from the chunks' carries.
+// The float sums are then rounded once to f32 and finalised as in statistical_moments.cu:58-106.
+// With post set (the persistent update ran in the same step) the posterior weights
+// (update_persistent_particles.cu:76-87) are formed in the load: w = p_A mu_A lw + (1 - p_A) mu_UA
+// w_prior, written to weight_array and used for the moments.
+constexpr int MOM_CH = 5;
+constexpr int MOM_ITEMS = 4;                  // consecutive particles per lane
+constexpr int MOM_CHUNK = WAVE * MOM_ITEMS;   // particles per wave
+static_assert(MOM_CHUNK == UNIT, "a moments chunk is one work-table unit");
+constexpr uint8_t MC_CONT = 1;    // the chunk's last cell continues into the next chunk
+constexpr uint8_t MC_NOHEAD = 2;  // no cell starts in the chunk
+
+__device__ __forceinline__ void mom_finalize(const VehDesc& d, int key, const float (&s)[MOM_CH])
 {
-    static constexpr int NCH = 5;
-    static constexpr int ITEMS = 4;
-    static constexpr int WAVES = 3;  // 166 registers: three waves per SIMD
-    int pph;
-    int post;
-    __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
-    __device__ void head(const VehDesc&, int, int) const {}
-    __device__ void end(const VehDesc&, int, int) const {}
-    __device__ void cont(const VehDesc&, int) const {}
-    __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
+    const float rho_p = d.f[GF_PERS][key];
+    float mx = 0.0f, my = 0.0f, vx = 0.0f, vy = 0.0f, cv = 0.0f;
+    if (rho_p > 0.0f)
     {
-        float w[ITEMS];
-        load_run<ITEMS>(d.weight_array, p0, cnt, w);
-        if (post)
-        {
-            float wp[ITEMS];
-            load_run<ITEMS>(d.part[pph].weight, p0, cnt, wp);
+        const float inv = 1.0f / rho_p;
+        mx = inv * s[0];
+        my = inv * s[1];
+        vx = inv * s[2] - mx * mx;
+        vy = inv * s[3] - my * my;
+        cv = inv * s[4] - mx * my;
+    }
+    d.f[GF_MEAN_X][key] = mx;
+    d.f[GF_MEAN_Y][key] = my;
+    d.f[GF_VAR_X][key] = vx;
+    d.f[GF_VAR_Y][key] = vy;
+    d.f[GF_COVAR][key] = cv;
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, ROW_MASK, 0xf, false);
+}
+
+// carry.flags[unit], and per unit MOM_CH doubles in carry.head_partial (piece of the chunk's first
+// cell when that cell started in an earlier chunk) and carry.tail_partial (piece of its last cell).
+template <int WAVES_PER_WG>
+__global__ __launch_bounds__(WAVES_PER_WG* WAVE) void k_moments(const VehDesc* __restrict__ descs,
+                                                                const uint32_t* __restrict__ tab, int pph, int post,
+                                                                SegCarry carry)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int unit = xcd_tile(blockIdx.x, gridDim.x) * WAVES_PER_WG + (int)threadIdx.x / WAVE;
+    const TileRef tr = tile_ref(tab, 1, unit);
+    const VehDesc& d = descs[tr.v];
+    const int n = d.N;
+    const int base = tr.base;
+    if (base >= n)
+    {
+        if (lane == 0)
+            carry.flags[unit] = 0;  // padding chunk past the vehicle's particles
+        return;                     // wave-uniform
+    }
+    const int p0 = base + lane * MOM_ITEMS;
+    const int cnt = max(0, min(MOM_ITEMS, n - p0));
+    const int* keys = d.part[pph].idx;
+    int key[MOM_ITEMS];
+    load_run<MOM_ITEMS>(keys, p0, cnt, key, -1);
+    int kprev = __shfl_up(key[MOM_ITEMS - 1], 1, WAVE);
+    int knext = __shfl_down(key[0], 1, WAVE);
+    if (lane == 0)
+        kprev = base > 0 ? keys[base - 1] : -1;
+    if (lane == WAVE - 1)
+        knext = p0 + MOM_ITEMS < n ? keys[p0 + MOM_ITEMS] : -1;
+    bool head[MOM_ITEMS], end[MOM_ITEMS], started[MOM_ITEMS];
+    bool lh = false;
 #pragma unroll
-            for (int i = 0; i < ITEMS; ++i)
+    for (int i = 0; i < MOM_ITEMS; ++i)
+    {
+        const int kp = i == 0 ? kprev : key[i - 1];
+        const int kn = i == MOM_ITEMS - 1 ? knext : key[i + 1];
+        head[i] = i < cnt && key[i] != kp;
+        end[i] = i < cnt && key[i] != kn;
+        lh = lh || head[i];
+        started[i] = lh;
+    }
+
+    float w[MOM_ITEMS];
+    load_run<MOM_ITEMS>(d.weight_array, p0, cnt, w);
+    if (post)
+    {
+        float wp[MOM_ITEMS];
+        load_run<MOM_ITEMS>(d.part[pph].weight, p0, cnt, wp);
+#pragma unroll
+        for (int i = 0; i < MOM_ITEMS; ++i)
+        {
+            if (i < cnt)
             {
-                if (i < cnt)
-                {
-                    const int c = key[i];
-                    const float pA = d.meas[c].p_A;
-                    w[i] = pA * d.f[GF_MU_A][c] * w[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * wp[i];
-                }
+                const int c = key[i];
+                const float pA = d.meas[c].p_A;
+                w[i] = pA * d.f[GF_MU_A][c] * w[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * wp[i];
             }
-            store_run<ITEMS>(d.weight_array, p0, cnt, w);
         }
-        const float4* state = d.part[pph].state;
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i)
-        {
-            const float4 s = i < cnt ? state[p0 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            v[i][0] = w[i] * s.z;
-            v[i][1] = w[i] * s.w;
-            v[i][2] = w[i] * s.z * s.z;
-            v[i][3] = w[i] * s.w * s.w;
-            v[i][4] = w[i] * s.z * s.w;
-        }
+        store_run<MOM_ITEMS>(d.weight_array, p0, cnt, w);
     }
-    __device__ void finalize(const VehDesc& d, int key, const fx_t (&s)[5]) const
+    float vx[MOM_ITEMS], vy[MOM_ITEMS];
     {
-        const float rho_p = d.f[GF_PERS][key];
-        float mx = 0.0f, my = 0.0f, vx = 0.0f, vy = 0.0f, cv = 0.0f;
-        if (rho_p > 0.0f)
+        const float2* vel = d.part[pph].vel;
+        if (cnt == MOM_ITEMS)
         {
-            const float inv = 1.0f / rho_p;
-            mx = inv * fx_to_f32(s[0]);
-            my = inv * fx_to_f32(s[1]);
-            vx = inv * fx_to_f32(s[2]) - mx * mx;
-            vy = inv * fx_to_f32(s[3]) - my * my;
-            cv = inv * fx_to_f32(s[4]) - mx * my;
+            const float4 a = *reinterpret_cast<const float4*>(vel + p0);
+            const float4 b = *reinterpret_cast<const float4*>(vel + p0 + 2);
+            vx[0] = a.x; vy[0] = a.y; vx[1] = a.z; vy[1] = a.w;
+            vx[2] = b.x; vy[2] = b.y; vx[3] = b.z; vy[3] = b.w;
         }
-        d.f[GF_MEAN_X][key] = mx;
-        d.f[GF_MEAN_Y][key] = my;
-        d.f[GF_VAR_X][key] = vx;
-        d.f[GF_VAR_Y][key] = vy;
-        d.f[GF_COVAR][key] = cv;
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < MOM_ITEMS; ++i)
+            {
+                const float2 v = i < cnt ? vel[p0 + i] : make_float2(0.0f, 0.0f);
+                vx[i] = v.x;
+                vy[i] = v.y;
+            }
+        }
     }
-};
+
+    // Combine predicates of the six scan steps (the same for every channel): the source lane is in
+    // range and no cell starts between it and this lane.
+    bool comb[6];
+    {
+        const int row = lane >> 4, rl = lane & 15;
+        int f = lh ? 1 : 0;
+        int g;
+        g = dpp_i<0x111, 0xf>(f); comb[0] = rl >= 1 && !f; f = rl >= 1 ? (f | g) : f;
+        g = dpp_i<0x112, 0xf>(f); comb[1] = rl >= 2 && !f; f = rl >= 2 ? (f | g) : f;
+        g = dpp_i<0x114, 0xf>(f); comb[2] = rl >= 4 && !f; f = rl >= 4 ? (f | g) : f;
+        g = dpp_i<0x118, 0xf>(f); comb[3] = rl >= 8 && !f; f = rl >= 8 ? (f | g) : f;
+        const bool b15 = row == 1 || row == 3, b31 = row >= 2;
+        g = dpp_i<0x142, 0xa>(f); comb[4] = b15 && !f; f = b15 ? (f | g) : f;
+        comb[5] = b31 && !f;
+    }
+    // The chunk's first cell started in an earlier chunk (lane 0's first particle is no cell start);
+    // its first end is lane first_end_lane's particle first_i.
+    const bool from_before = base > 0 && !(__ballot(head[0]) & 1ull);
+    const uint64_t endlanes = __ballot(end[0] || end[1] || end[2] || end[3]);
+    const int first_end_lane = endlanes ? __builtin_ctzll(endlanes) : WAVE;
+    const int first_i = end[0] ? 0 : end[1] ? 1 : end[2] ? 2 : 3;
+    const bool nohead = __ballot(lh) == 0;
+    // The chunk's last cell continues into the next chunk: lane 63 holds 4 particles and the last
+    // is no cell end.
+    const bool cont = (__ballot(cnt == MOM_ITEMS && !end[MOM_ITEMS - 1]) >> (WAVE - 1)) & 1ull;
+
+    float s32[MOM_ITEMS][MOM_CH];
+#pragma unroll
+    for (int c = 0; c < MOM_CH; ++c)
+    {
+        double r[MOM_ITEMS];
+#pragma unroll
+        for (int i = 0; i < MOM_ITEMS; ++i)
+        {
+            const float wv = w[i] * vx[i], wy = w[i] * vy[i];
+            const float xf = c == 0 ? wv : c == 1 ? wy : c == 2 ? wv * vx[i] : c == 3 ? wy * vy[i] : wv * vy[i];
+            const double x = (double)xf;
+            if (i == 0)
+                r[i] = x;
+            else if (i < cnt)
+                r[i] = head[i] ? x : r[i - 1] + x;
+            else
+                r[i] = r[i - 1];
+        }
+        double S = r[MOM_ITEMS - 1];
+        { const double y = dpp_val<0x111, 0xf>(S); S = comb[0] ? y + S : S; }
+        { const double y = dpp_val<0x112, 0xf>(S); S = comb[1] ? y + S : S; }
+        { const double y = dpp_val<0x114, 0xf>(S); S = comb[2] ? y + S : S; }
+        { const double y = dpp_val<0x118, 0xf>(S); S = comb[3] ? y + S : S; }
+        { const double y = dpp_val<0x142, 0xa>(S); S = comb[4] ? y + S : S; }
+        { const double y = dpp_val<0x143, 0xc>(S); S = comb[5] ? y + S : S; }
+        const double E = __shfl_up(S, 1, WAVE);
+#pragma unroll
+        for (int i = 0; i < MOM_ITEMS; ++i)
+        {
+            const double v = (started[i] || lane == 0) ? r[i] : E + r[i];
+            s32[i][c] = (float)v;
+            if (end[i] && from_before && lane == first_end_lane && i == first_i)
+                reinterpret_cast<double*>(carry.head_partial)[(size_t)unit * MOM_CH + c] = v;
+        }
+        if (lane == WAVE - 1 && cont)
+            reinterpret_cast<double*>(carry.tail_partial)[(size_t)unit * MOM_CH + c] = S;
+    }
+    if (lane == 0)
+        carry.flags[unit] = (uint8_t)((cont ? MC_CONT : 0) | (nohead ? MC_NOHEAD : 0));
+#pragma unroll
+    for (int i = 0; i < MOM_ITEMS; ++i)
+    {
+        if (!end[i])
+            continue;
+        if (from_before && lane == first_end_lane && i == first_i)
+            continue;  // the cell started in an earlier chunk: k_moments_fix
+        float s[MOM_CH];
+#pragma unroll
+        for (int c = 0; c < MOM_CH; ++c)
+            s[c] = s32[i][c];
+        mom_finalize(d, key[i], s);
+    }
+}
+
+// Cells crossing chunks: one thread per chunk whose last cell starts in it and continues.
+__global__ __launch_bounds__(BLOCK) void k_moments_fix(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab, int units, int pph,
+                                                       SegCarry carry)
+{
+    const int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= units)
+        return;
+    const uint8_t fl = carry.flags[u];
+    if (!(fl & MC_CONT) || (fl & MC_NOHEAD))
+        return;
+    const double* tail = reinterpret_cast<const double*>(carry.tail_partial);
+    const double* headp = reinterpret_cast<const double*>(carry.head_partial);
+    double acc[MOM_CH];
+#pragma unroll
+    for (int c = 0; c < MOM_CH; ++c)
+        acc[c] = tail[(size_t)u * MOM_CH + c];
+    int j = u + 1;
+    while ((carry.flags[j] & (MC_CONT | MC_NOHEAD)) == (MC_CONT | MC_NOHEAD))
+    {
+#pragma unroll
+        for (int c = 0; c < MOM_CH; ++c)
+            acc[c] = acc[c] + tail[(size_t)j * MOM_CH + c];
+        ++j;
+    }
+    float s[MOM_CH];
+#pragma unroll
+    for (int c = 0; c < MOM_CH; ++c)
+        s[c] = (float)(acc[c] + headp[(size_t)j * MOM_CH + c]);
+    const TileRef tr = tile_ref(tab, 1, u);
+    const VehDesc& d = descs[tr.v];
+    mom_finalize(d, d.part[pph].idx[tr.base + MOM_CHUNK - 1], s);
+}
+
+constexpr int MOM_WAVES_PER_WG = 4;
 
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done)
 {
-    launch_segreduce(E, K_SEG_MOM, K_SEG_MOM_FIX, MomOp{pph, fused_post ? 1 : 0});
+    const int units = (int)E->units[SP_P];
+    if (units > 0)
+    {
+        launch_begin(E, K_SEG_MOM);
+        hipLaunchKernelGGL(k_moments<MOM_WAVES_PER_WG>, dim3(units / MOM_WAVES_PER_WG), dim3(MOM_WAVES_PER_WG * WAVE),
+                           0, E->stream, E->d_desc, E->tab(SP_P), pph, fused_post ? 1 : 0, E->seg);
+        launch_end(E, K_SEG_MOM);
+        launch_begin(E, K_SEG_MOM_FIX);
+        hipLaunchKernelGGL(k_moments_fix, dim3(div_up(units, BLOCK)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                           E->tab(SP_P), units, pph, E->seg);
+        launch_end(E, K_SEG_MOM_FIX);
+    }
     if (empty_done)
         return;
     launch_begin(E, K_EMPTY_CELLS);
@@ -1005,6 +1199,43 @@ __global__ __launch_bounds__(BLOCK) void k_unpack_grid(DevGrid G, const dogm_gri
     G.f[GF_VAR_X][c] = g.var_x_vel;
     G.f[GF_VAR_Y][c] = g.var_y_vel;
     G.f[GF_COVAR][c] = g.covar_xy_vel;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pack_state(DevParticles P, int n, float4* __restrict__ out)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    const float2 p = P.pos[i], v = P.vel[i];
+    out[i] = make_float4(p.x, p.y, v.x, v.y);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_unpack_state(DevParticles P, int n, const float4* __restrict__ in)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 s = in[i];
+    P.pos[i] = make_float2(s.x, s.y);
+    P.vel[i] = make_float2(s.z, s.w);
+}
+
+void run_pack_state(Engine* E, const DevParticles& p, int n, float4* out)
+{
+    if (n <= 0)
+        return;
+    launch_begin(E, K_PACK_STATE);
+    hipLaunchKernelGGL(k_pack_state, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, E->stream, p, n, out);
+    launch_end(E, K_PACK_STATE);
+}
+
+void run_unpack_state(Engine* E, const DevParticles& p, int n, const float4* in)
+{
+    if (n <= 0)
+        return;
+    launch_begin(E, K_UNPACK_STATE);
+    hipLaunchKernelGGL(k_unpack_state, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, E->stream, p, n, in);
+    launch_end(E, K_UNPACK_STATE);
 }
 
 void run_pack_grid(Engine* E, const Vehicle* v)

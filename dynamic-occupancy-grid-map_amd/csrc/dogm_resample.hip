@@ -525,10 +525,12 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         d.ancestors[k] = a;
         const bool from_p = a < n;
         const int o = from_p ? a : a - n;
-        const float4* st = from_p ? P.state : Bp.state;
+        const float2* ps = from_p ? P.pos : Bp.pos;
+        const float2* vs = from_p ? P.vel : Bp.vel;
         const int* ix = from_p ? P.idx : Bp.idx;
         const uint8_t* as = from_p ? P.assoc : Bp.assoc;
-        out.state[k] = st[o];
+        out.pos[k] = ps[o];
+        out.vel[k] = vs[o];
         out.idx[k] = ix[o];
         out.assoc[k] = as[o];
         out.weight[k] = nw;

@@ -547,20 +547,9 @@ int dogm_server_get_particles(dogm_server* s, int vehicle_id, dogm_vec4* state, 
     }
     std::lock_guard<std::mutex> ge(s->eng_mu);
     Engine* E = &s->eng;
-    const DevParticles& p = v->part[0];
-    const size_t n = (size_t)v->N;
-    struct Part
-    {
-        void* dst;
-        const void* src;
-        size_t bytes;
-    } parts[] = {{state, p.state, 16 * n}, {grid_cell_idx, p.idx, 4 * n}, {weight, p.weight, 4 * n},
-                 {associated, p.assoc, n}};
-    for (const Part& q : parts)
-        if (q.dst && hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost, E->stream) != hipSuccess)
-            return sfail(s, DOGM_ERR_HIP, "reading back particles");
-    if (hipStreamSynchronize(E->stream) != hipSuccess)
-        return sfail(s, DOGM_ERR_HIP, "reading back particles");
+    const int st = read_particles(E, v, v->part[0], v->N, state, grid_cell_idx, weight, associated);
+    if (st != DOGM_OK)
+        return sfail(s, st, "reading back particles: " + E->last_error);
     return DOGM_OK;
 }
 

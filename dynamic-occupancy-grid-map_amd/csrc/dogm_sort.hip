@@ -322,7 +322,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
             const DevParticles dst = dv.part[pph ^ 1];
             const int o = sort_value_index(vv[b]);
             const float w = src.weight[o];
-            dst.state[g] = src.state[o];
+            dst.pos[g] = src.pos[o];
+            dst.vel[g] = src.vel[o];
             dst.weight[g] = w;
             dst.assoc[g] = sort_value_assoc(vv[b]);
             dst.idx[g] = kk[b];

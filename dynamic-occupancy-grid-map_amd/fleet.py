@@ -49,15 +49,26 @@ class FleetVehicle:
         return (r * math.sin(th), r * (1.0 - math.cos(th)), th)
 
 
-def make_fleet(count: int, seed: int = 7, sides=SIDES) -> list[FleetVehicle]:
+def make_fleet(count: int, seed: int = 7, sides=SIDES, min_speed: float = 0.0) -> list[FleetVehicle]:
+    """`count` vehicles with sides drawn from `sides`; speeds uniform in [min_speed, 15) m/s (above
+    1 m/s every 0.1 s frame moves the grid by at least one 0.1 m cell)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
         side = int(rng.choice(sides))
-        out.append(FleetVehicle(side=side, params=fleet_params(side), speed=float(rng.uniform(0.0, 15.0)),
+        out.append(FleetVehicle(side=side, params=fleet_params(side), speed=float(rng.uniform(min_speed, 15.0)),
                                 yaw_rate=float(rng.uniform(-0.2, 0.2)), phase=float(rng.uniform(0.0, 0.1)),
                                 boxes=int(rng.integers(5, 31))))
     return out
+
+
+def vehicle_scene(fv: FleetVehicle, index: int, seed: int = 3) -> scene_mod.Scene:
+    """The vehicle's own scene: its box count, 720-beam lidar reaching the grid edge, and an ego
+    trajectory with its speed and yaw rate (Scene.ego_pose equals FleetVehicle.pose)."""
+    cfg = scene_mod.SceneConfig(grid_size=fv.side, resolution=RESOLUTION, n_boxes=fv.boxes, n_beams=720,
+                                max_range=0.5 * fv.side * RESOLUTION, ego_speed=fv.speed, ego_yaw_rate=fv.yaw_rate,
+                                seed=seed * 1000 + index)
+    return scene_mod.Scene(cfg)
 
 
 def side_scene(side: int, seed: int = 3) -> scene_mod.Scene:

@@ -68,6 +68,7 @@ def lib():
             "oracle_cells_with_velocity": (i, [i] + [vp] * 7 + [f, f, vp]),
             "oracle_velocity_normalized": (f, [f, f, f, f, f]),
             "oracle_lidar_grid": (None, [i, i, C.c_double, C.c_double, f, f, vp, vp, vp, vp]),
+            "oracle_moment_sums": (None, [vp, i, vp, vp]),
             "oracle_set_ego_mode": (None, [vp, i]),
             "oracle_set_draw_mode": (None, [vp, i]),
             "oracle_two_sum_f32": (None, [f, f, C.POINTER(f), C.POINTER(f)]),
@@ -388,6 +389,17 @@ def lidar_grid(grid_size: int, resolution: float, max_range: float, occ_mass: fl
                                                  ("p_A", "<f4")])
     lib().oracle_lidar_grid(grid_size, cs.size, resolution, max_range, occ_mass, free_mass, cs.ctypes.data,
                             sn.ctypes.data, r.ctypes.data, out.ctypes.data)
+    return out
+
+
+def moment_sums(keys, x) -> np.ndarray:
+    """Per-cell sums of the f32 path's velocity-moment channels in the device's f64 association.
+    keys: sorted cell index per particle; x: (5, n) float32 channel values.  Returns (5, n) float64
+    with the cell's sum at the cell's last particle and NaN elsewhere."""
+    k = np.ascontiguousarray(keys, np.int32)
+    v = np.ascontiguousarray(x, np.float32).reshape(5, k.size)
+    out = np.full((5, k.size), np.nan)
+    lib().oracle_moment_sums(k.ctypes.data, k.size, v.ctypes.data, out.ctypes.data)
     return out
 
 

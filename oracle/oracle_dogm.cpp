@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 using namespace dogm_num;
@@ -207,6 +208,140 @@ struct OracleBase
             spac.data());
     }
 };
+
+// Velocity-moment sums of the f32 path: the device's fixed f64 association (dogm_kernels.hip,
+// k_moments).  A grid's sorted particles are cut into chunks of 256 (chunk k = [256 k, 256 k + 256)),
+// one wave64 per chunk, 4 consecutive particles per lane; every value is the f32 channel product
+// widened to f64.
+//   lane:  sequential fold that restarts where a cell starts: r = x there, r + x otherwise; the
+//          lane's first particle always starts the fold (r = x);
+//   wave:  segmented inclusive scan of the lanes' last folds in six simultaneous steps -- row shifts
+//          1, 2, 4, 8 inside 16-lane rows, lane 15 of rows 0 / 2 into rows 1 / 3, lane 31 into rows
+//          2 / 3 -- each adding the moved value only while no cell starts in the lanes between;
+//   cell:  ending in lane l, its lane-local fold if it starts in lane l, else (scan of lane l - 1) +
+//          fold (lane 0: the fold);
+//   chunks: a cell crossing chunks is the left fold ((piece in its first chunk + whole middle chunk)
+//          + ...) + piece in its last chunk.
+static void wave_segscan64(double* x, const bool* head)
+{
+    bool f[64];
+    std::memcpy(f, head, sizeof(f));
+    auto step = [&](auto src_of) {
+        double nx[64];
+        bool nf[64];
+        for (int l = 0; l < 64; ++l)
+        {
+            nx[l] = x[l];
+            nf[l] = f[l];
+            const int s = src_of(l);
+            if (s >= 0)
+            {
+                if (!f[l])
+                    nx[l] = x[s] + x[l];
+                nf[l] = f[l] || f[s];
+            }
+        }
+        std::memcpy(x, nx, sizeof(nx));
+        std::memcpy(f, nf, sizeof(nf));
+    };
+    for (int sh : {1, 2, 4, 8})
+        step([sh](int l) { return (l & 15) >= sh ? l - sh : -1; });
+    step([](int l) {
+        const int r = l >> 4;
+        return (r == 1 || r == 3) ? 16 * r - 1 : -1;
+    });
+    step([](int l) { return (l >> 4) >= 2 ? 31 : -1; });
+}
+
+constexpr int MOM_CH = 5;
+constexpr int MOM_CHUNK = 256;
+
+// out[k * MOM_CH + c]: channel c's sum of the cell whose particles end at particle k (written at the
+// last particle of every cell).
+static void moment_sums_wave(const int* key, int N, const float* const* ch, double* out)
+{
+    const int chunks = (N + MOM_CHUNK - 1) / MOM_CHUNK;
+    // per chunk: the piece of its last cell (scan of lane 63) and of its first cell when that cell
+    // started in an earlier chunk and ends in this one
+    std::vector<double> tail((size_t)chunks * MOM_CH), headp((size_t)chunks * MOM_CH);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < chunks; ++k)
+    {
+        const int base = k * MOM_CHUNK;
+        const bool from_before = base > 0 && key[base] == key[base - 1];
+        for (int c = 0; c < MOM_CH; ++c)
+        {
+            double r[MOM_CHUNK];
+            double t[64];
+            bool lh[64];
+            for (int l = 0; l < 64; ++l)
+            {
+                t[l] = 0.0;
+                lh[l] = false;
+                for (int i = 0; i < 4; ++i)
+                {
+                    const int p = base + 4 * l + i;
+                    if (p >= N)
+                        break;
+                    const double x = (double)ch[c][p];
+                    const bool h = p == 0 || key[p] != key[p - 1];
+                    lh[l] = lh[l] || h;
+                    r[4 * l + i] = (i == 0 || h) ? x : r[4 * l + i - 1] + x;
+                    t[l] = r[4 * l + i];
+                }
+            }
+            double S[64];
+            std::memcpy(S, t, sizeof(S));
+            wave_segscan64(S, lh);
+            bool seen_end = false;
+            for (int l = 0; l < 64; ++l)
+            {
+                bool started = false;  // a cell starts in lane l at or before particle i
+                for (int i = 0; i < 4; ++i)
+                {
+                    const int p = base + 4 * l + i;
+                    if (p >= N)
+                        break;
+                    started = started || p == 0 || key[p] != key[p - 1];
+                    if (p != N - 1 && key[p + 1] == key[p])
+                        continue;
+                    const double v = (started || l == 0) ? r[4 * l + i] : S[l - 1] + r[4 * l + i];
+                    if (!seen_end && from_before)
+                        headp[(size_t)k * MOM_CH + c] = v;
+                    else
+                        out[(size_t)p * MOM_CH + c] = v;
+                    seen_end = true;
+                }
+            }
+            tail[(size_t)k * MOM_CH + c] = S[63];
+        }
+    }
+    // Cells crossing chunks, folded from the chunk where they start.
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < chunks - 1; ++k)
+    {
+        const int base = k * MOM_CHUNK, last = base + MOM_CHUNK - 1;
+        const int cell = key[last];
+        if (key[last + 1] != cell || (base > 0 && key[base - 1] == cell))
+            continue;  // the chunk's last cell ends here, or started in an earlier chunk
+        double acc[MOM_CH];
+        for (int c = 0; c < MOM_CH; ++c)
+            acc[c] = tail[(size_t)k * MOM_CH + c];
+        int j = k + 1;
+        // middle chunks: every particle of chunk j belongs to the cell and the cell goes on
+        while (j < chunks - 1 && key[j * MOM_CHUNK + MOM_CHUNK - 1] == cell && key[j * MOM_CHUNK + MOM_CHUNK] == cell)
+        {
+            for (int c = 0; c < MOM_CH; ++c)
+                acc[c] = acc[c] + tail[(size_t)j * MOM_CH + c];
+            ++j;
+        }
+        int e = j * MOM_CHUNK;
+        while (e + 1 < N && key[e + 1] == cell)
+            ++e;
+        for (int c = 0; c < MOM_CH; ++c)
+            out[(size_t)e * MOM_CH + c] = acc[c] + headp[(size_t)j * MOM_CH + c];
+    }
+}
 
 // Float CDF modes: restatement of the device association (dogm_resample.hip, k_cdff_*): per tile of
 // 1024 entries, 4-element sequential folds per thread, the 64-lane wave scan of the thread totals as
@@ -825,6 +960,20 @@ struct Oracle final : OracleBase
             m4[i] = w * vy * vy;
             m5[i] = w * vx * vy;
         }
+        // f32 path: the device's f64 association (moment_sums_wave); FP64 twin: exact sums.
+        std::vector<double> wsum;
+        if constexpr (std::is_same<T, float>::value)
+        {
+            wsum.resize((size_t)N * MOM_CH);
+            const float* ch[MOM_CH] = {m1.data(), m2.data(), m3.data(), m4.data(), m5.data()};
+            moment_sums_wave(part.idx.data(), N, ch, wsum.data());
+        }
+        auto sum = [&](int ch, const std::vector<T>& m, int s, int e) -> T {
+            if constexpr (std::is_same<T, float>::value)
+                return (float)wsum[(size_t)e * MOM_CH + ch];
+            else
+                return cell_sum(m, s, e);
+        };
 #pragma omp parallel for schedule(static)
         for (int c = 0; c < C; ++c)
         {
@@ -834,11 +983,11 @@ struct Oracle final : OracleBase
             {
                 const int s = start[c], e = end[c];
                 const T inv = one / rho_p;
-                mx = inv * cell_sum(m1, s, e);
-                my = inv * cell_sum(m2, s, e);
-                vx = inv * cell_sum(m3, s, e) - mx * mx;
-                vy = inv * cell_sum(m4, s, e) - my * my;
-                cv = inv * cell_sum(m5, s, e) - mx * my;
+                mx = inv * sum(0, m1, s, e);
+                my = inv * sum(1, m2, s, e);
+                vx = inv * sum(2, m3, s, e) - mx * mx;
+                vy = inv * sum(3, m4, s, e) - my * my;
+                cv = inv * sum(4, m5, s, e) - mx * my;
             }
             g[GF_MEAN_X][c] = mx;
             g[GF_MEAN_Y][c] = my;
@@ -1306,6 +1455,22 @@ float oracle_velocity_normalized(float mx, float my, float vxx, float vyy, float
 // dogm/demo/simulator/mapping/laser_to_meas_grid.cu:25-70).  Sensor at the grid centre; samples every
 // resolution / 2 metres while the distance is below min(range, max_range) mark free_mass; the cell of
 // a return within max_range gets occ_mass and free 0; likelihood = p_A = 1 (measurement_grid.cu:129-130).
+// Velocity-moment sums of the f32 path in the device's association (moment_sums_wave): for keys[n]
+// sorted, channel values x[c * n + k], out[c * n + k] = the sum of channel c over the cell that ends at
+// particle k (other entries untouched).
+void oracle_moment_sums(const int* keys, int n, const float* x, double* out)
+{
+    const float* ch[MOM_CH];
+    for (int c = 0; c < MOM_CH; ++c)
+        ch[c] = x + (size_t)c * n;
+    std::vector<double> tmp((size_t)n * MOM_CH, 0.0);
+    moment_sums_wave(keys, n, ch, tmp.data());
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < MOM_CH; ++c)
+            if (k == n - 1 || keys[k + 1] != keys[k])
+                out[(size_t)c * n + k] = tmp[(size_t)k * MOM_CH + c];
+}
+
 void oracle_lidar_grid(int gs, int beams, double res, double max_range, float occ_mass, float free_mass,
                        const double* cosv, const double* sinv, const double* ranges, dogm_measurement_cell* out)
 {

@@ -432,3 +432,64 @@ def test_fp32_against_fp64_one_frame():
     for k in ("occ_mass", "free_mass", "pers_occ_mass", "new_born_occ_mass", "pred_occ_mass"):
         err = np.abs(a[k].astype(np.float64) - b[k])
         assert np.all(err <= 1e-5 * np.abs(b[k]) + 1e-7), (k, err.max())
+
+
+def _load_twin(src, p, meas, seed):
+    """An FP64 checker holding the f32 checker's state after the assignment stage."""
+    twin = OracleDOGM(p, seed=seed, fp64=True)
+    twin.set_flags(True, True)
+    twin.update_measurement_grid(meas)
+    parts = src.get_particles()
+    twin.set_particles(state=parts["state"], grid_cell_idx=parts["grid_cell_idx"], weight=parts["weight"],
+                       associated=parts["associated"])
+    twin.set_grid_cells(src.get_grid_cells())
+    twin.set_weight_array(src.get_weight_array())
+    twin.epoch = src.epoch
+    return twin
+
+
+def test_fp32_against_fp64_moments_and_birth_weights():
+    """200 x 200 cells: from identical state after the assignment, the f32 path's occupancy, persistent,
+    birth and moment stages against the same stages in double precision.  Masses within 1e-5 relative;
+    velocity mean / variance / covariance within 1e-5 of the cell's second moment (E[v^2] = var +
+    mean^2, its square root for the mean, sqrt(E[vx^2] E[vy^2]) for the covariance); w_A and w_UA
+    within 1e-5 relative wherever both assign the cell the same number of birth particles (the counts
+    come from prefixes of f32 vs f64 born masses and may differ by one at a rounding boundary)."""
+    gs = 200
+    p = small_params(gs=gs, res=0.25, n=120_000, b=12_000)
+    sc = small_scene(gs=gs, res=0.25, n_boxes=12, n_beams=720, ego_speed=4.0)
+    f32 = OracleDOGM(p, seed=11)
+    for f in range(2):
+        f32.update_grid(sc.measurement(f), *sc.ego_pose(f), 0.1)
+    m = sc.measurement(2)
+    f32.update_measurement_grid(m)
+    f32.update_pose(*sc.ego_pose(2))
+    f32.particle_prediction(0.1)
+    f32.particle_assignment()
+    f64 = _load_twin(f32, p, m, seed=11)
+    for o in (f32, f64):
+        o.grid_cell_occupancy_update(0.1)
+        o.update_persistent_particles()
+        o.initialize_new_particles()
+        o.statistical_moments()
+    a, b = f32.get_grid_cells(), f64.get_grid_cells()
+    occupied = (b["start_idx"] != -1) & (b["pers_occ_mass"] > 0)
+    assert occupied.sum() > 1000
+    for k in ("occ_mass", "free_mass", "pers_occ_mass", "new_born_occ_mass", "pred_occ_mass", "mu_A", "mu_UA"):
+        err = np.abs(a[k].astype(np.float64) - b[k])
+        assert np.all(err <= 1e-5 * np.abs(b[k]) + 1e-12), (k, float((err / np.maximum(np.abs(b[k]), 1e-30)).max()))
+    m2x = b["var_x_vel"].astype(np.float64) + b["mean_x_vel"].astype(np.float64) ** 2
+    m2y = b["var_y_vel"].astype(np.float64) + b["mean_y_vel"].astype(np.float64) ** 2
+    scale = {"mean_x_vel": np.sqrt(m2x), "mean_y_vel": np.sqrt(m2y), "var_x_vel": m2x, "var_y_vel": m2y,
+             "covar_xy_vel": np.sqrt(m2x * m2y)}
+    for k, s in scale.items():
+        err = np.abs(a[k].astype(np.float64) - b[k])[occupied]
+        assert np.all(err <= 1e-5 * s[occupied]), (k, float((err / s[occupied]).max()))
+        assert np.all(a[k][~occupied] == 0) and np.all(b[k][~occupied] == 0), k
+    na = np.bincount(f32.get_birth_particles()["grid_cell_idx"], minlength=gs * gs)
+    nb = np.bincount(f64.get_birth_particles()["grid_cell_idx"], minlength=gs * gs)
+    same = na == nb
+    assert same.mean() > 0.99
+    for k in ("w_A", "w_UA"):
+        err = np.abs(a[k].astype(np.float64) - b[k])[same]
+        assert np.all(err <= 1e-5 * np.abs(b[k][same]) + 1e-30), k
