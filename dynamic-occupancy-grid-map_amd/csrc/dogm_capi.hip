@@ -661,7 +661,11 @@ int dogm_memcpy(void* dst, const void* src, size_t bytes, int kind)
     case 2: k = hipMemcpyDeviceToDevice; break;
     default: return DOGM_ERR_INVALID_ARGUMENT;
     }
-    return hipMemcpy(dst, src, bytes, k) == hipSuccess ? DOGM_OK : DOGM_ERR_HIP;
+    if (hipMemcpy(dst, src, bytes, k) != hipSuccess)
+        return DOGM_ERR_HIP;
+    // The copy runs on the null stream, which the library's non-blocking streams do not wait for:
+    // finish it before a step may read the destination.
+    return hipDeviceSynchronize() == hipSuccess ? DOGM_OK : DOGM_ERR_HIP;
 }
 
 int dogm_device_copy(void* dst, const void* src, size_t bytes)

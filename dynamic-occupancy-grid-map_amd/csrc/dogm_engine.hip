@@ -154,13 +154,16 @@ void free_particles(DevParticles& p)
 int alloc_particles(DevParticles& p, int n, std::string* why, size_t* bytes)
 {
     int st;
-    if ((st = dalloc(&p.pos, 2 * (size_t)n, "particle state", why, bytes)) ||
+    // vel starts on a 16-byte boundary (an odd count pads pos by one entry) so the moments can load
+    // two velocities per 16-byte access.
+    const size_t vel_off = (size_t)n + (size_t)(n & 1);
+    if ((st = dalloc(&p.pos, vel_off + (size_t)n, "particle state", why, bytes)) ||
         (st = dalloc(&p.idx, n, "particle index", why, bytes)) ||
         (st = dalloc(&p.weight, n, "particle weight", why, bytes)) ||
         (st = dalloc(&p.assoc, n, "particle flags", why, bytes)))
         return st;
-    p.vel = p.pos ? p.pos + n : nullptr;
-    if (n > 0 && (hipMemset(p.pos, 0, sizeof(float4) * n) != hipSuccess ||
+    p.vel = p.pos ? p.pos + vel_off : nullptr;
+    if (n > 0 && (hipMemset(p.pos, 0, sizeof(float2) * (vel_off + n)) != hipSuccess ||
                   hipMemset(p.idx, 0, sizeof(int) * n) != hipSuccess ||
                   hipMemset(p.weight, 0, sizeof(float) * n) != hipSuccess || hipMemset(p.assoc, 0, n) != hipSuccess))
         return set_err(why, DOGM_ERR_HIP, "clearing particle buffers");
@@ -176,6 +179,44 @@ uint64_t next_uid()
 }  // namespace
 
 // ---- engine ------------------------------------------------------------------------------------
+
+namespace
+{
+
+// Element counts of the plan-sized step buffers of a step over `veh`.
+struct PlanSizes
+{
+    long long tot[SP_COUNT];  // table units per space
+    long long tab;            // work-table entries (all spaces)
+    long long scan_tiles;     // 1024-element tiles of the longest scanned space
+    long long seg_tiles;      // segmented-reduction carries: one per 256-particle unit (the moments'
+                              // chunks; the other reductions take 512 or more particles per tile)
+    long long sort_tiles;     // 2048-particle sort tiles
+    int passes;               // radix passes for the largest grid
+};
+
+int plan_sizes(Vehicle* const* veh, int m, PlanSizes* ps, std::string* why)
+{
+    *ps = PlanSizes{};
+    ps->passes = 1;
+    for (int s = 0; s < SP_COUNT; ++s)
+        for (int i = 0; i < m; ++i)
+        {
+            ps->tot[s] += units_for(space_len(veh[i], s));
+            if (ps->tot[s] > (long long)INT32_MAX)
+                return set_err(why, DOGM_ERR_INVALID_ARGUMENT, "step too large for the work tables");
+        }
+    for (int i = 0; i < m; ++i)
+        ps->passes = std::max(ps->passes, std::max(1, (bits_for(veh[i]->C - 1) + 7) / 8));
+    for (int s = 0; s < SP_COUNT; ++s)
+        ps->tab += ps->tot[s];
+    ps->scan_tiles = std::max({ps->tot[SP_C], ps->tot[SP_J], ps->tot[SP_E]}) / 4;
+    ps->seg_tiles = ps->tot[SP_P];
+    ps->sort_tiles = ps->tot[SP_P] / 8;
+    return DOGM_OK;
+}
+
+}  // namespace
 
 int engine_init(Engine* E, std::string* why)
 {
@@ -250,8 +291,12 @@ static int engine_reserve(Engine* E, int m, std::string* why)
             (st = dalloc(&E->sort_ghist, (size_t)cap * 4 * 256, "sort histograms", why)) ||
             (st = dalloc(&E->scalars, cap, "step scalars", why)))
             return st;
-        if (hipMemset(E->scalars, 0, sizeof(DevScalars) * cap) != hipSuccess ||
-            hipMemset(E->sort_ghist, 0, sizeof(unsigned) * (size_t)cap * 4 * 256) != hipSuccess)
+        // Cleared on the engine's own stream and waited for: hipMemset runs on the null stream, which
+        // a non-blocking stream does not wait for, so the next step's predict could otherwise count
+        // digits into histograms still holding a freed buffer's contents.
+        if (hipMemsetAsync(E->scalars, 0, sizeof(DevScalars) * cap, E->stream) != hipSuccess ||
+            hipMemsetAsync(E->sort_ghist, 0, sizeof(unsigned) * (size_t)cap * 4 * 256, E->stream) != hipSuccess ||
+            hipStreamSynchronize(E->stream) != hipSuccess)
             return set_err(why, DOGM_ERR_HIP, "clearing step scalars");
         E->ghist_clean = true;
         E->m_cap = cap;
@@ -274,35 +319,27 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         same = E->plan_veh[i] == veh[i]->uid;
     if (!same)
     {
-        long long tot[SP_COUNT] = {};
-        int passes = 1;
+        PlanSizes ps;
+        if ((st = plan_sizes(veh, m, &ps, &why)))
+            return set_err(&E->last_error, st, why);
+        const long long* tot = ps.tot;
         for (int s = 0; s < SP_COUNT; ++s)
         {
             int* pre = E->h_upre + s * (m + 1);
             pre[0] = 0;
             for (int i = 0; i < m; ++i)
-            {
-                const long long u = units_for(space_len(veh[i], s));
-                if (tot[s] + u > (long long)INT32_MAX)
-                    return set_err(&E->last_error, DOGM_ERR_INVALID_ARGUMENT, "step too large for the work tables");
-                tot[s] += u;
-                pre[i + 1] = (int)tot[s];
-            }
+                pre[i + 1] = pre[i] + (int)units_for(space_len(veh[i], s));
         }
-        for (int i = 0; i < m; ++i)
-            passes = std::max(passes, std::max(1, (bits_for(veh[i]->C - 1) + 7) / 8));
-        long long total = 0;
+        const int passes = ps.passes;
+        const long long total = ps.tab;
         for (int s = 0; s < SP_COUNT; ++s)
         {
             E->units[s] = tot[s];
-            E->tab_off[s] = total;
-            total += tot[s];
+            E->tab_off[s] = s == 0 ? 0 : E->tab_off[s - 1] + tot[s - 1];
         }
-        const long long scan_tiles = std::max({tot[SP_C], tot[SP_J], tot[SP_E]}) / 4;
-        // Segmented reductions take 512 or more particles per tile; the moments carry one entry of
-        // five doubles per 256-particle chunk (in the 16-byte partial slots).
-        const long long seg_tiles = tot[SP_P];
-        const long long sort_tiles = tot[SP_P] / 8;
+        const long long scan_tiles = ps.scan_tiles;
+        const long long seg_tiles = ps.seg_tiles;
+        const long long sort_tiles = ps.sort_tiles;
         {
             if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
                 return set_err(&E->last_error, st, why);
@@ -862,8 +899,9 @@ size_t vehicle_footprint(const dogm_params* p)
     const long long part = 16 + 4 + 4 + 1;
     // particles x2 + births, 16 grid fields + free_alt + field_tmp + meas(16) + born + m_raw + pend,
     // weights + ancestors + sort (16 B), CDF (16 B per joint entry), occupancy bits.
-    return (size_t)(2 * N * part + B * part + C * 4 * (GF_COUNT + 2 + 2) + C * 16 + C * 12 + N * 8 + N * 16 +
-                    (N + B) * 16 + (C + 63) / 64 * 8);
+    const long long pad = 8 * (2 * (N & 1) + (B & 1));  // the velocity arrays start 16-byte aligned
+    return (size_t)(2 * N * part + B * part + pad + C * 4 * (GF_COUNT + 2 + 2) + C * 16 + C * 12 + N * 8 +
+                    N * 16 + (N + B) * 16 + (C + 63) / 64 * 8);
 }
 
 int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance, std::string* why)

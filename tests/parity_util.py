@@ -78,3 +78,30 @@ def assert_state_equal(tag, gpu, orc, *, births=True, ancestors=False):
         assert_particles_equal(f"{tag} births", gpu.get_birth_particles(), orc.get_birth_particles())
     if ancestors:
         assert_int_equal(f"{tag} ancestors", gpu.get_resampled_indices(), orc.get_resampled_indices())
+
+
+def state_differences(g_parts: dict, o_parts: dict, g_grid, o_grid) -> list[str]:
+    """Every particle array and grid field that differs, with its count of differing entries (an
+    empty list when the states are bit-identical)."""
+    out = []
+    pairs = [(f"particles {k}", g_parts[k], o_parts[k]) for k in ("grid_cell_idx", "state", "weight", "associated")]
+    pairs += [(f"grid {f}", g_grid[f], o_grid[f]) for f in GRID_FIELDS]
+    for name, a, b in pairs:
+        a, b = np.asarray(a), np.asarray(b)
+        if a.dtype.kind == "f":
+            a, b = _u32(a), _u32(b)
+        if a.shape != b.shape:
+            out.append(f"{name}: shape {a.shape} vs {b.shape}")
+            continue
+        bad = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
+        if bad.size:
+            out.append(f"{name}: {bad.size} of {a.shape[0]} differ (first at {int(bad[0])})")
+    return out
+
+
+def assert_vehicle_equal(tag, g_parts: dict, o_parts: dict, g_grid, o_grid):
+    """Particles and grid bit for bit; a failure lists every array that differs, so the earliest
+    stage whose output differs can be read off."""
+    d = state_differences(g_parts, o_parts, g_grid, o_grid)
+    if d:
+        raise AssertionError(f"{tag}: " + "; ".join(d))

@@ -27,7 +27,7 @@ import oracle_ctypes as oc
 from dogm_ctypes import (EXTRACT_IMAGE, EXTRACT_VELOCITY, MEAS_CELL_DTYPE, DOGM, Batch, DeviceBuffer, Lidar, Server,
                          lib, vehicle_footprint)
 from oracle_ctypes import OracleDOGM
-from parity_util import assert_grid_equal, assert_int_equal, assert_particles_equal
+from parity_util import assert_int_equal, assert_vehicle_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -137,12 +137,11 @@ def test_fleet_batched_eager_and_graph_equal_plain_and_checker(fleet):
                     assert pos != last_pos[i], f"{tag}: the grid must shift on every frame"
                 last_pos[i] = pos
                 pe, pp = eager.get_particles(ids_e[i]), g.get_particles()
-                assert_particles_equal(f"{tag} batch vs plain", pe, pp)
-                assert_particles_equal(f"{tag} graph vs eager", graph.get_particles(ids_g[i]), pe)
-                del pp
                 ge, gp = eager.get_grid_cells(ids_e[i]), g.get_grid_cells()
-                assert_grid_equal(f"{tag} batch vs plain", ge, gp)
-                assert_grid_equal(f"{tag} graph vs eager", graph.get_grid_cells(ids_g[i]), ge)
+                assert_vehicle_equal(f"{tag} batch vs plain", pe, pp, ge, gp)
+                del pp, gp
+                assert_vehicle_equal(f"{tag} graph vs eager", graph.get_particles(ids_g[i]), pe,
+                                     graph.get_grid_cells(ids_g[i]), ge)
                 assert_int_equal(f"{tag} ancestors", eager.get_resampled_indices(ids_e[i]), g.get_resampled_indices())
             for i, orc in checkers.items():
                 tag = f"frame {f} vehicle {i} (side {fleet.vehicles[i].side}) vs checker"
@@ -151,8 +150,8 @@ def test_fleet_batched_eager_and_graph_equal_plain_and_checker(fleet):
                 got = _download_meas(eager.measurement_buffer(ids_e[i]), m.size)
                 assert got.tobytes() == m.tobytes(), f"{tag}: device grid"
                 orc.update_grid(m, *poses[i], DT)
-                assert_particles_equal(tag, eager.get_particles(ids_e[i]), orc.get_particles())
-                assert_grid_equal(tag, eager.get_grid_cells(ids_e[i]), orc.get_grid_cells())
+                assert_vehicle_equal(tag, eager.get_particles(ids_e[i]), orc.get_particles(),
+                                     eager.get_grid_cells(ids_e[i]), orc.get_grid_cells())
                 assert_int_equal(f"{tag} ancestors", eager.get_resampled_indices(ids_e[i]),
                                  orc.get_resampled_indices())
     finally:
@@ -174,9 +173,9 @@ def test_fleet_served_from_scans_equal_plain_and_checker(fleet):
     ci = fleet.checked[-1]
     orc = OracleDOGM(fleet.vehicles[ci].params, seed=SEED, instance=ci)
     with Server(hbm_budget_bytes=200 * 10**9) as s:
+        s.set_extraction(EXTRACT_IMAGE | EXTRACT_VELOCITY, min_occ, min_vel)
         vids = [s.admit(v.params, SEED, i) for i, v in enumerate(fleet.vehicles)]
         assert all(v is not None for v in vids), "the fleet fits the 200 GB budget"
-        s.set_extraction(EXTRACT_IMAGE | EXTRACT_VELOCITY, min_occ, min_vel)
         now = 0.0
         try:
             for f in range(FRAMES):
@@ -199,8 +198,8 @@ def test_fleet_served_from_scans_equal_plain_and_checker(fleet):
                 lib.dogm_device_synchronize()
                 for i, g in enumerate(plains):
                     tag = f"served frame {f} vehicle {i} (side {fleet.vehicles[i].side})"
-                    assert_particles_equal(tag, s.get_particles(vids[i]), g.get_particles())
-                    assert_grid_equal(tag, s.get_grid_cells(vids[i]), g.get_grid_cells())
+                    assert_vehicle_equal(tag, s.get_particles(vids[i]), g.get_particles(), s.get_grid_cells(vids[i]),
+                                         g.get_grid_cells())
                     assert_int_equal(f"{tag} image", s.get_occupancy_image(vids[i]), g.get_occupancy_image())
                     cs, ncs = s.get_cells_with_velocity(vids[i])
                     cp, ncp = g.get_cells_with_velocity(min_occ, min_vel)
@@ -209,8 +208,8 @@ def test_fleet_served_from_scans_equal_plain_and_checker(fleet):
                 _say(tag)
                 m = fleet.checker_grid(ci, f)
                 orc.update_grid(m, *fleet.pose(ci, f), DT)
-                assert_particles_equal(tag, s.get_particles(vids[ci]), orc.get_particles())
-                assert_grid_equal(tag, s.get_grid_cells(vids[ci]), orc.get_grid_cells())
+                assert_vehicle_equal(tag, s.get_particles(vids[ci]), orc.get_particles(), s.get_grid_cells(vids[ci]),
+                                     orc.get_grid_cells())
         finally:
             for b in bufs:
                 b.free()
