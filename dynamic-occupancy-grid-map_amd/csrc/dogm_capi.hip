@@ -361,7 +361,11 @@ int dogm_create_ex(const dogm_params* params, uint64_t seed, uint32_t instance, 
     dogm_handle* h = new (std::nothrow) dogm_handle();
     if (!h)
         return DOGM_ERR_OUT_OF_MEMORY;
-    if ((st = engine_init(&h->eng, &why)) != DOGM_OK || (st = vehicle_alloc(&h->veh, params, seed, instance, &h->eng.last_error)))
+    // The per-vehicle step arrays are reserved here, so a step on this handle allocates only its
+    // plan-sized buffers (and dogm_update_grid_noalloc none at all).
+    if ((st = engine_init(&h->eng, &why)) != DOGM_OK ||
+        (st = vehicle_alloc(&h->veh, params, seed, instance, &h->eng.last_error)) ||
+        (st = engine_reserve(&h->eng, 1, &h->eng.last_error)))
     {
         release(h);
         return st;

@@ -216,7 +216,102 @@ int plan_sizes(Vehicle* const* veh, int m, PlanSizes* ps, std::string* why)
     return DOGM_OK;
 }
 
+// Byte offsets of the step buffers in one region (256-byte aligned pieces).
+struct ScratchLayout
+{
+    size_t tab, scan_sum, scan_pre, cdf_incl, seg_flags, seg_head, seg_tail, thist, total;
+};
+
+ScratchLayout scratch_layout(const PlanSizes& ps)
+{
+    ScratchLayout L{};
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = at;
+        at += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    L.tab = take(sizeof(uint32_t) * (size_t)ps.tab);
+    L.scan_sum = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
+    L.scan_pre = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
+    L.cdf_incl = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
+    L.seg_flags = take((size_t)ps.seg_tiles);
+    L.seg_head = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
+    L.seg_tail = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
+    L.thist = take(sizeof(unsigned) * 256 * (size_t)ps.sort_tiles);
+    L.total = at;
+    return L;
+}
+
 }  // namespace
+
+int engine_step_scratch_bytes(Vehicle* const* veh, int m, size_t* bytes, std::string* why)
+{
+    PlanSizes ps;
+    const int st = plan_sizes(veh, m, &ps, why);
+    if (st != DOGM_OK)
+        return st;
+    *bytes = scratch_layout(ps).total;
+    return DOGM_OK;
+}
+
+int engine_bind_scratch(Engine* E, void* base, size_t bytes, Vehicle* const* veh, int m, std::string* why)
+{
+    if (E->bound)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT, "the engine's step buffers are already bound");
+    PlanSizes ps;
+    int st = plan_sizes(veh, m, &ps, why);
+    if (st != DOGM_OK)
+        return st;
+    const ScratchLayout L = scratch_layout(ps);
+    if (L.total > bytes)
+        return set_err(why, DOGM_ERR_INVALID_ARGUMENT,
+                       "the step needs " + std::to_string(L.total) + " bytes of scratch, " + std::to_string(bytes) +
+                           " are bound");
+    E->own = Engine::OwnStepBuffers{E->d_tab,         E->tab_cap,        E->scan_tile_sum, E->scan_tile_pre,
+                                    E->cdf_tile_incl, E->scan_tiles_cap, E->seg,           E->seg_tiles_cap,
+                                    E->sort_thist,    E->sort_tiles_cap};
+    char* b = static_cast<char*>(base);
+    E->d_tab = reinterpret_cast<uint32_t*>(b + L.tab);
+    E->tab_cap = ps.tab;
+    E->scan_tile_sum = reinterpret_cast<fx_t*>(b + L.scan_sum);
+    E->scan_tile_pre = reinterpret_cast<fx_t*>(b + L.scan_pre);
+    E->cdf_tile_incl = reinterpret_cast<fx_t*>(b + L.cdf_incl);
+    E->scan_tiles_cap = ps.scan_tiles;
+    E->seg.flags = reinterpret_cast<uint8_t*>(b + L.seg_flags);
+    E->seg.head_partial = reinterpret_cast<fx_t*>(b + L.seg_head);
+    E->seg.tail_partial = reinterpret_cast<fx_t*>(b + L.seg_tail);
+    E->seg_tiles_cap = ps.seg_tiles;
+    E->sort_thist = reinterpret_cast<unsigned*>(b + L.thist);
+    E->sort_tiles_cap = ps.sort_tiles * 256;
+    E->bound = true;
+    // The tables of the last plan are in the engine's own buffers: rebuild them in the bound memory
+    // (and recapture a step graph afterwards).
+    E->plan_veh.clear();
+    ++E->plan_epoch;
+    return DOGM_OK;
+}
+
+void engine_unbind_scratch(Engine* E)
+{
+    if (!E->bound)
+        return;
+    const Engine::OwnStepBuffers& o = E->own;
+    E->d_tab = o.d_tab;
+    E->tab_cap = o.tab_cap;
+    E->scan_tile_sum = o.scan_tile_sum;
+    E->scan_tile_pre = o.scan_tile_pre;
+    E->cdf_tile_incl = o.cdf_tile_incl;
+    E->scan_tiles_cap = o.scan_tiles_cap;
+    E->seg = o.seg;
+    E->seg_tiles_cap = o.seg_tiles_cap;
+    E->sort_thist = o.sort_thist;
+    E->sort_tiles_cap = o.sort_tiles_cap;
+    E->own = Engine::OwnStepBuffers{};
+    E->bound = false;
+    E->plan_veh.clear();
+    ++E->plan_epoch;
+}
 
 int engine_init(Engine* E, std::string* why)
 {
@@ -234,6 +329,7 @@ void engine_release(Engine* E)
 {
     if (E->stream)
         (void)hipStreamSynchronize(E->stream);
+    engine_unbind_scratch(E);
     if (E->graph_exec)
         (void)hipGraphExecDestroy(E->graph_exec);
     E->graph_exec = nullptr;
@@ -262,7 +358,7 @@ void engine_release(Engine* E)
     E->stream = nullptr;
 }
 
-static int engine_reserve(Engine* E, int m, std::string* why)
+int engine_reserve(Engine* E, int m, std::string* why)
 {
     if (m > E->m_cap)
     {

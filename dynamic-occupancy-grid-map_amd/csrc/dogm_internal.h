@@ -388,6 +388,21 @@ struct Engine
     long long graph_plan = -1;
     uint64_t graph_sig[3] = {0, 0, 0};
 
+    // Step buffers bound to caller memory (engine_bind_scratch): the engine's own buffers and their
+    // capacities wait here until engine_unbind_scratch puts them back.
+    struct OwnStepBuffers
+    {
+        uint32_t* d_tab;
+        long long tab_cap;
+        fx_t *scan_tile_sum, *scan_tile_pre, *cdf_tile_incl;
+        long long scan_tiles_cap;
+        SegCarry seg;
+        long long seg_tiles_cap;
+        unsigned* sort_thist;
+        long long sort_tiles_cap;
+    } own{};
+    bool bound = false;
+
     std::string last_error;
 
     const uint32_t* tab(int s) const { return d_tab + tab_off[s]; }
@@ -408,6 +423,17 @@ long long units_for(long long len);  // table units of a space of `len` elements
 
 int engine_init(Engine* E, std::string* why);
 void engine_release(Engine* E);
+// Reserves the per-vehicle step arrays (descriptors, unit prefixes, digit histograms, scalars) for
+// steps of up to m vehicles, so that later steps of that size allocate only plan-sized buffers.
+int engine_reserve(Engine* E, int m, std::string* why);
+// Bytes of the plan-sized step buffers (work tables, scan tile totals, segmented-reduction carries,
+// sort tile histograms) of a step over `veh`, each buffer starting on a 256-byte boundary.
+int engine_step_scratch_bytes(Vehicle* const* veh, int m, size_t* bytes, std::string* why);
+// Points those buffers into caller memory [base, base + bytes) for the following steps over `veh`
+// (the engine then allocates none of them), and forces the next step to rebuild its tables there.
+int engine_bind_scratch(Engine* E, void* base, size_t bytes, Vehicle* const* veh, int m, std::string* why);
+// Gives the engine its own buffers back (after the bound steps have completed on its stream).
+void engine_unbind_scratch(Engine* E);
 // Builds descriptors and tables for `veh` (reusing the tables while the vehicle list is
 // unchanged) and uploads the descriptors; the stage set fixes the per-step fields (epochs, shifts).
 int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages);

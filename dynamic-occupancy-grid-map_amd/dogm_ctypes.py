@@ -24,6 +24,7 @@ if not LIB_PATH.exists():
 lib = C.CDLL(str(LIB_PATH))
 
 DOGM_OK = 0
+DOGM_ERR_INVALID_ARGUMENT = 1
 DOGM_DEFAULT_SEED = 123456
 CDF_MODES = {"exact": 0, "fp64": 1, "fp32": 2, "fp32c": 3}
 EGO_MODES = {"reference": 0, "metric": 1}
@@ -87,6 +88,7 @@ _SIGS = {
     "dogm_last_error": (C.c_char_p, [_h]),
     "dogm_status_string": (C.c_char_p, [_i]),
     "dogm_update_grid": (_i, [_h, _vp, _f, _f, _f, _f, _i]),
+    "dogm_update_grid_noalloc": (_i, [_h, _vp, _f, _f, _f, _f, _i]),
     "dogm_update_measurement_grid": (_i, [_h, _vp, _i]),
     "dogm_update_pose": (_i, [_h, _f, _f, _f]),
     "dogm_initialize_particles": (_i, [_h]),

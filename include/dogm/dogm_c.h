@@ -111,6 +111,17 @@ const char* dogm_status_string(int status);
 int dogm_update_grid(dogm_handle* h, const dogm_measurement_cell* measurement_grid, float new_x, float new_y,
                      float new_yaw, float dt, int is_device);
 
+/* No counterpart in the reference: the same update as dogm_update_grid, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The step's plan-sized buffers (work tables, scan tile totals, segmented-reduction
+ * carries, sort tile histograms) come from a static 16 MiB device array per GPU instead of
+ * allocations, and the call allocates nothing (no step graph, no kernel timing).  A grid whose step
+ * needs more than 16 MiB (about 1.3e7 particles) is rejected with DOGM_ERR_INVALID_ARGUMENT.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU (from several host
+ * threads) race and must be serialised by the caller.  Results equal dogm_update_grid's. */
+int dogm_update_grid_noalloc(dogm_handle* h, const dogm_measurement_cell* measurement_grid, float new_x,
+                             float new_y, float new_yaw, float dt, int is_device);
+
 /* Private steps of updateGrid in the reference, exported so tests can drive them. */
 int dogm_update_measurement_grid(dogm_handle* h, const dogm_measurement_cell* measurement_grid,
                                  int is_device);                                    /* dogm.cu:205-215 */
