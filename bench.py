@@ -48,13 +48,14 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
     it hands on); gathers through cell indices count their payload once."""
     table = {
         "predict": 44 * N,                       # state+weight in, state+weight+idx out
-        "sort_upsweep": 4 * N,                   # keys
         "sort_downsweep": 16 * N,                # a pass before the last: keys + values in and out
         # the last pass: keys + values in, the 21 B payload gathered, sorted particles (25 B) and the
         # weight copy out
         "sort_gather": (8 + 21 + 25 + 4) * N,
         "seg_occ": 8 * N,                        # keys + weights
-        "mass_update": 40 * C,                   # start, free, meas occ/free in; 6 fields out
+        # start, free, meas occ/free in; 6 fields out; the step's measurement grid copied from the
+        # caller's device buffer on the way (16 B in, 16 B out per cell)
+        "mass_update": 72 * C,
         "seg_pers": 12 * N,                      # keys + weights in, likelihood-weighted weights out
         "seg_moments": 24 * N,                   # keys, lik-weighted and prior weights, vx, vy in; posterior out
         "rs_merge": 50 * N + 4 * B,              # joint weights, 21 B ancestor payload in; 25 B particles out
@@ -68,9 +69,8 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
 
 # Profiled kernel names (rocprofv3) of the per-kernel timers.
 PROFILED_NAME = {
-    "predict": "k_predict", "sort_upsweep": "k_sort_upsweep", "sort_downsweep": "k_sort_downsweep<false>",
-    "sort_gather": "k_sort_downsweep<true>",
-    "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>", "seg_moments": "k_segreduce<MomOp>",
+    "predict": "k_predict", "sort_downsweep": "k_sort_onesweep<false>", "sort_gather": "k_sort_onesweep<true>",
+    "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>", "seg_moments": "k_moments<4>",
     "mass_update": "k_mass_update", "rs_merge": "k_rs_merge", "birth_cells": "k_scan_apply<SrcBorn,SinkBirth>",
 }
 
@@ -366,7 +366,8 @@ def run_uneven(args) -> dict:
         t_upd_graph += time.perf_counter() - t1
     b.close()
     bank.close()
-    return {"vehicles": len(ids), "sides": {str(sd): len(js) for sd, js in sorted(by_side.items())}, "steps": steps,
+    return {"vehicles": len(ids), "particles": sum(v.params.particle_count for v in vehicles),
+            "sides": {str(sd): len(js) for sd, js in sorted(by_side.items())}, "steps": steps,
             "update_frames_per_s_graph": round(frames / t_upd_graph, 1),
             "frames_per_s": round(frames / (t_gen + t_upd), 1), "update_frames_per_s": round(frames / t_upd, 1),
             "ms_per_step": round((t_gen + t_upd) / steps * 1e3, 3), "update_ms_per_step": round(t_upd / steps * 1e3, 3),
@@ -739,6 +740,16 @@ def _compare_masses(frame, gcells, ocells):
     return out
 
 
+def _tuning_report(N: int, kveh: int, uneven: dict | None) -> dict:
+    from dogm_ctypes import lib, tuning_lookup
+    rep = {"table_entries": int(lib.dogm_tuning_entries()), "plain": tuning_lookup(1, N)}
+    if kveh:
+        rep["batched"] = tuning_lookup(kveh, kveh * N)
+    if uneven:
+        rep["batched_uneven"] = tuning_lookup(uneven["vehicles"], uneven["particles"])
+    return rep
+
+
 def _summary(r):
     fps = r["frames"] / r["elapsed"]
     return {"frames_per_s": round(fps, 3), "ms_per_step": round(1e3 * r["elapsed"] * len(r["lat"]) ** -1, 4),
@@ -911,6 +922,9 @@ def main() -> int:
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),
+        # Launch choices the tuning table gave each form (dynamic-occupancy-grid-map_amd/tuning_table.json,
+        # written by autotune.py on the device; choices never change results).
+        "tuning": _tuning_report(N, kveh if batched else 0, uneven),
         "roofline": {
             "kernel": dominant,
             "bound": "hbm",

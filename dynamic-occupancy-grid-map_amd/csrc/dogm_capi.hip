@@ -18,7 +18,7 @@ struct dogm_batch
     Engine eng;
     Engine aux;             // second lane of steps over two or more vehicles (created on first use)
     bool aux_ready = false;
-    int lanes = 2;
+    int lanes = 0;  // 0: as the tuning of each step gives
     std::map<int, std::unique_ptr<Vehicle>> veh;
     int next_id = 1;
     uint64_t bytes = 0;
@@ -58,10 +58,11 @@ int finish(Engine* E, const char* stage)
     if (st != DOGM_OK)
         return st;
     const hipError_t e = hipStreamSynchronize(E->stream);
+    const int dc = engine_device_check(E);
     if (e != hipSuccess)
         return fail(E, DOGM_ERR_HIP, std::string(stage) + ": " + hipGetErrorString(e));
     engine_collect_timing(E);
-    return DOGM_OK;
+    return dc;
 }
 
 // Runs `stages` on one vehicle and waits.
@@ -864,7 +865,7 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
     for (int i = 0; i < count; ++i)
         in[i] = FrameIn{meas ? meas[i] : nullptr, is_device, x[i], y[i], yaw[i], dt[i]};
     E->ext = b->ext;
-    if (count >= 2 && b->lanes > 1 && !b->aux_ready)
+    if (count >= 2 && b->lanes != 1 && !b->aux_ready)
     {
         std::string why;
         const int st = engine_init(&b->aux, &why);
@@ -880,7 +881,7 @@ int dogm_batch_update(dogm_batch* b, int count, const int* vehicle_ids, const do
         b->aux_ready = true;
     }
     Engine* lanes[2] = {E, &b->aux};
-    return engine_frame_lanes(lanes, b->aux_ready && b->lanes > 1 ? 2 : 1, list.data(), in.data(), count,
+    return engine_frame_lanes(lanes, b->aux_ready ? b->lanes : 1, list.data(), in.data(), count,
                               b->ext.what != 0);
 }
 
@@ -964,8 +965,8 @@ int dogm_batch_set_lanes(dogm_batch* b, int lanes)
 {
     if (!b)
         return DOGM_ERR_INVALID_ARGUMENT;
-    if (lanes < 1 || lanes > 2)
-        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "lanes must be 1 or 2");
+    if (lanes < 0 || lanes > 2)
+        return fail(&b->eng, DOGM_ERR_INVALID_ARGUMENT, "lanes must be 1, 2 or 0 (as the tuning gives)");
     b->lanes = lanes;
     return DOGM_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace dogm_hip
@@ -219,7 +220,7 @@ int plan_sizes(Vehicle* const* veh, int m, PlanSizes* ps, std::string* why)
 // Byte offsets of the step buffers in one region (256-byte aligned pieces).
 struct ScratchLayout
 {
-    size_t tab, scan_sum, scan_pre, cdf_incl, seg_flags, seg_head, seg_tail, thist, total;
+    size_t tab, scan_sum, scan_pre, cdf_incl, seg_flags, seg_head, seg_tail, status, total;
 };
 
 ScratchLayout scratch_layout(const PlanSizes& ps)
@@ -238,7 +239,7 @@ ScratchLayout scratch_layout(const PlanSizes& ps)
     L.seg_flags = take((size_t)ps.seg_tiles);
     L.seg_head = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
     L.seg_tail = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
-    L.thist = take(sizeof(unsigned) * 256 * (size_t)ps.sort_tiles);
+    L.status = take(sizeof(unsigned) * 256 * (size_t)ps.sort_tiles);
     L.total = at;
     return L;
 }
@@ -270,7 +271,7 @@ int engine_bind_scratch(Engine* E, void* base, size_t bytes, Vehicle* const* veh
                            " are bound");
     E->own = Engine::OwnStepBuffers{E->d_tab,         E->tab_cap,        E->scan_tile_sum, E->scan_tile_pre,
                                     E->cdf_tile_incl, E->scan_tiles_cap, E->seg,           E->seg_tiles_cap,
-                                    E->sort_thist,    E->sort_tiles_cap};
+                                    E->sort_status,    E->sort_tiles_cap};
     char* b = static_cast<char*>(base);
     E->d_tab = reinterpret_cast<uint32_t*>(b + L.tab);
     E->tab_cap = ps.tab;
@@ -282,7 +283,7 @@ int engine_bind_scratch(Engine* E, void* base, size_t bytes, Vehicle* const* veh
     E->seg.head_partial = reinterpret_cast<fx_t*>(b + L.seg_head);
     E->seg.tail_partial = reinterpret_cast<fx_t*>(b + L.seg_tail);
     E->seg_tiles_cap = ps.seg_tiles;
-    E->sort_thist = reinterpret_cast<unsigned*>(b + L.thist);
+    E->sort_status = reinterpret_cast<unsigned*>(b + L.status);
     E->sort_tiles_cap = ps.sort_tiles * 256;
     E->bound = true;
     // The tables of the last plan are in the engine's own buffers: rebuild them in the bound memory
@@ -305,7 +306,7 @@ void engine_unbind_scratch(Engine* E)
     E->scan_tiles_cap = o.scan_tiles_cap;
     E->seg = o.seg;
     E->seg_tiles_cap = o.seg_tiles_cap;
-    E->sort_thist = o.sort_thist;
+    E->sort_status = o.sort_status;
     E->sort_tiles_cap = o.sort_tiles_cap;
     E->own = Engine::OwnStepBuffers{};
     E->bound = false;
@@ -322,7 +323,29 @@ int engine_init(Engine* E, std::string* why)
         return set_err(why, DOGM_ERR_NO_DEVICE, "hipGetDevice failed");
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess)
         return set_err(why, DOGM_ERR_HIP, "hipStreamCreateWithFlags failed");
+    if (hipHostMalloc(reinterpret_cast<void**>(&E->h_err), sizeof(unsigned),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    {
+        E->h_err = nullptr;
+        return set_err(why, DOGM_ERR_OUT_OF_MEMORY, "allocating the device-check flag");
+    }
+    *E->h_err = 0u;
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&E->d_err), E->h_err, 0) != hipSuccess)
+        return set_err(why, DOGM_ERR_HIP, "mapping the device-check flag");
+    // Polls before a lookback wait gives up; a negative value gives up at once (a host-forced
+    // failure for tests of that path).
+    if (const char* spin = std::getenv("DOGM_LOOKBACK_SPIN"))
+        E->lookback_spin = std::atoi(spin);
     return DOGM_OK;
+}
+
+int engine_device_check(Engine* E)
+{
+    if (!E->h_err || *reinterpret_cast<volatile unsigned*>(E->h_err) == 0u)
+        return DOGM_OK;
+    *reinterpret_cast<volatile unsigned*>(E->h_err) = 0u;
+    return set_err(&E->last_error, DOGM_ERR_DEVICE_CHECK,
+                   "a radix-sort lookback wait ran out of polls (an earlier tile never published)");
 }
 
 void engine_release(Engine* E)
@@ -339,10 +362,14 @@ void engine_release(Engine* E)
         (void)hipHostFree(E->h_upre);
     E->h_desc = nullptr;
     E->h_upre = nullptr;
+    if (E->h_err)
+        (void)hipHostFree(E->h_err);
+    E->h_err = nullptr;
+    E->d_err = nullptr;
     {
         void* bufs[] = {E->d_desc,        E->d_upre,         E->d_tab,            E->scan_tile_sum,
                         E->scan_tile_pre, E->cdf_tile_incl,  E->seg.flags,        E->seg.head_partial,
-                        E->seg.tail_partial, E->sort_thist,  E->sort_ghist,       E->scalars};
+                        E->seg.tail_partial, E->sort_status,  E->sort_ghist,       E->scalars};
         for (void* b : bufs)
             (void)hipFree(b);
     }
@@ -376,6 +403,7 @@ int engine_reserve(Engine* E, int m, std::string* why)
         E->d_desc = nullptr;
         E->d_upre = nullptr;
         E->sort_ghist = nullptr;
+        E->sort_tickets = nullptr;
         E->scalars = nullptr;
         E->m_cap = 0;
         if (hipHostMalloc(reinterpret_cast<void**>(&E->h_desc), sizeof(VehDesc) * cap) != hipSuccess ||
@@ -384,16 +412,17 @@ int engine_reserve(Engine* E, int m, std::string* why)
         int st;
         if ((st = dalloc(&E->d_desc, cap, "step descriptors", why)) ||
             (st = dalloc(&E->d_upre, (size_t)SP_COUNT * (cap + 1), "step tables", why)) ||
-            (st = dalloc(&E->sort_ghist, (size_t)cap * 4 * 256, "sort histograms", why)) ||
+            (st = dalloc(&E->sort_ghist, (size_t)cap * 4 * 256 + 64, "sort histograms", why)) ||
             (st = dalloc(&E->scalars, cap, "step scalars", why)))
             return st;
         // Cleared on the engine's own stream and waited for: hipMemset runs on the null stream, which
         // a non-blocking stream does not wait for, so the next step's predict could otherwise count
         // digits into histograms still holding a freed buffer's contents.
         if (hipMemsetAsync(E->scalars, 0, sizeof(DevScalars) * cap, E->stream) != hipSuccess ||
-            hipMemsetAsync(E->sort_ghist, 0, sizeof(unsigned) * (size_t)cap * 4 * 256, E->stream) != hipSuccess ||
+            hipMemsetAsync(E->sort_ghist, 0, sizeof(unsigned) * ((size_t)cap * 4 * 256 + 64), E->stream) != hipSuccess ||
             hipStreamSynchronize(E->stream) != hipSuccess)
             return set_err(why, DOGM_ERR_HIP, "clearing step scalars");
+        E->sort_tickets = E->sort_ghist + (size_t)cap * 4 * 256;
         E->ghist_clean = true;
         E->m_cap = cap;
     }
@@ -459,7 +488,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
                     return set_err(&E->last_error, st, why);
                 E->seg_tiles_cap = seg_tiles;
             }
-            if ((st = grow(&E->sort_thist, &E->sort_tiles_cap, sort_tiles * 256, "sort", &why)))
+            if ((st = grow(&E->sort_status, &E->sort_tiles_cap, sort_tiles * 256, "sort", &why)))
                 return set_err(&E->last_error, st, why);
         }
         E->passes = passes;
@@ -595,6 +624,13 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
                 return cs;
         }
     }
+    if (!E->tune_locked)
+    {
+        long long particles = 0;
+        for (int i = 0; i < m; ++i)
+            particles += veh[i]->N;
+        E->tune = tuning_for(m, particles);
+    }
     int st = engine_plan(E, veh, m, stages);
     for (int i = 0; i < m; ++i)
         veh[i]->meas_src = nullptr;  // consumed by the descriptors (or dropped with the failed step)
@@ -617,10 +653,11 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     const uint64_t sig[3] = {(uint64_t)stages | ((uint64_t)any_init << 16) | ((uint64_t)modes << 17) |
                                  ((uint64_t)(stages & ST_EXTRACT ? E->ext.what : 0) << 24) |
                                  ((uint64_t)(stages & ST_EXTRACT ? E->ext.source : 0) << 32),
-                             (stages & ST_EXTRACT) ? ((uint64_t)occ_bits << 32 | vel_bits) : 0, 0};
+                             (stages & ST_EXTRACT) ? ((uint64_t)occ_bits << 32 | vel_bits) : 0,
+                             tuning_signature(E->tune)};
     const bool graph = E->use_graph && !timing;
     bool replay = graph && E->graph_exec && E->graph_plan == E->plan_epoch && sig[0] == E->graph_sig[0] &&
-                  sig[1] == E->graph_sig[1];
+                  sig[1] == E->graph_sig[1] && sig[2] == E->graph_sig[2];
     if (graph && !replay)
     {
         if (E->graph_exec)
@@ -686,6 +723,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
             E->graph_plan = E->plan_epoch;
             E->graph_sig[0] = sig[0];
             E->graph_sig[1] = sig[1];
+            E->graph_sig[2] = sig[2];
         }
         if (hipGraphLaunch(E->graph_exec, E->stream) != hipSuccess)
             return set_err(&E->last_error, DOGM_ERR_HIP, "launching the step graph failed");
@@ -759,10 +797,13 @@ int frame_enqueue(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool
 int frame_wait(Engine* E, int st)
 {
     const hipError_t e = hipStreamSynchronize(E->stream);
+    const int dc = engine_device_check(E);
     if (st != DOGM_OK)
         return st;
     if (e != hipSuccess)
         return set_err(&E->last_error, DOGM_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
+    if (dc != DOGM_OK)
+        return dc;
     engine_collect_timing(E);
     return DOGM_OK;
 }
@@ -774,9 +815,31 @@ int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool 
     return frame_wait(E, frame_enqueue(E, veh, in, m, extract));
 }
 
+static int lanes_run(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m, bool extract);
+
 int engine_frame_lanes(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
 {
-    if (nl < 2 || m < 2)
+    long long particles = 0;
+    for (int i = 0; i < m; ++i)
+        particles += veh[i]->N;
+    const Tuning t = tuning_for(m, particles);
+    if (nl == 0)
+        nl = t.lanes;
+    const int used = nl < 2 || m < 2 ? 1 : 2;
+    for (int l = 0; l < used; ++l)
+    {
+        lanes[l]->tune = t;
+        lanes[l]->tune_locked = true;
+    }
+    const int st = lanes_run(lanes, used, veh, in, m, extract);
+    for (int l = 0; l < used; ++l)
+        lanes[l]->tune_locked = false;
+    return st;
+}
+
+static int lanes_run(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m, bool extract)
+{
+    if (nl < 2)
         return engine_frame(lanes[0], veh, in, m, extract);
     // Each vehicle goes to the lane with less stage-boundary traffic so far (bytes per frame
     // 172 N + 29 B + 128 C); a given vehicle list always splits the same way, so each lane keeps
@@ -1048,7 +1111,13 @@ int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t
     for (int i = 0; i < 4; ++i)
         if (zcnt[i] && hipMemset(zbufs[i], 0, sizeof(float) * zcnt[i]) != hipSuccess)
             return set_err(why, DOGM_ERR_HIP, "clearing buffers");
-    if (hipMemset(v->ancestors, 0, sizeof(int) * N) != hipSuccess)
+    // Sort buffers start as valid cell / particle indices (0), so a key or value slot that a sort
+    // pass failed to write (a lookback that gave up) still indexes inside the vehicle's arrays.
+    if (hipMemset(v->ancestors, 0, sizeof(int) * N) != hipSuccess ||
+        hipMemset(v->sort_keys[0], 0, sizeof(int) * N) != hipSuccess ||
+        hipMemset(v->sort_keys[1], 0, sizeof(int) * N) != hipSuccess ||
+        hipMemset(v->sort_vals[0], 0, sizeof(int) * N) != hipSuccess ||
+        hipMemset(v->sort_vals[1], 0, sizeof(int) * N) != hipSuccess)
         return set_err(why, DOGM_ERR_HIP, "clearing buffers");
     {
         std::vector<dogm_measurement_cell> m0((size_t)C, dogm_measurement_cell{0.0f, 0.0f, 1.0f, 1.0f});

@@ -333,6 +333,37 @@ struct ExtractCfg
     float min_occ = 0.0f, min_vel = 0.0f;
 };
 
+// Radix-sort pass forms (dogm_sort.hip).
+enum SortForm
+{
+    SORT_REDUCE_SCAN = 0,  // upsweep, digit scan, downsweep per pass
+    SORT_ONESWEEP = 1      // one launch per pass, decoupled lookback
+};
+
+// Launch choices of a step that do not change its results (dogm_tuning.hip); the defaults are the
+// built-in choices.
+struct Tuning
+{
+    int sort = SORT_REDUCE_SCAN;  // SortForm of the radix passes
+    int merge_slice = 0;          // exact-CDF merge: CDF tiles staged per workgroup (1, 2; 0: by size)
+    int pred_block = 512;         // predict: 512 threads x 4 particles or 256 x 8 per workgroup
+    int seg_items = 8;            // occupancy and persistent reductions: particles per thread (4, 8)
+    int lanes = 2;                // engines a batched step of two or more vehicles is split over
+};
+
+// Distinct for distinct tunings (part of a step graph's launch signature).
+inline uint64_t tuning_signature(const Tuning& t)
+{
+    return (uint64_t)(unsigned)t.sort | (uint64_t)(unsigned)t.merge_slice << 4 |
+           (uint64_t)(unsigned)t.pred_block << 8 | (uint64_t)(unsigned)t.seg_items << 20 |
+           (uint64_t)(unsigned)t.lanes << 28;
+}
+
+bool tuning_valid(const Tuning& t, std::string* why);
+// The table's choice for a step of `vehicles` vehicles and `particles` particles in all (or the
+// override, or the built-in choices).
+Tuning tuning_for(int vehicles, long long particles);
+
 // Execution context on one device and stream: step descriptors, work tables and the shared
 // scratch of the scans, segmented reductions and the sort.
 struct Engine
@@ -362,7 +393,10 @@ struct Engine
     long long seg_tiles_cap = 0;
     SegCarry seg{};
     long long sort_tiles_cap = 0;
-    unsigned* sort_thist = nullptr;  // [256 * sort tiles]
+    unsigned* sort_status = nullptr;  // [sort tiles * 256] decoupled-lookback words of the radix passes
+    long long status_plan = -1;       // plan whose tiles sort_status holds lookback words for
+    int lookback_spin = 1 << 20;      // polls before a lookback wait gives up (DOGM_ERR_DEVICE_CHECK)
+    unsigned* sort_tickets = nullptr; // [4] tile tickets of the passes (after sort_ghist's histograms)
     // sort_ghist is all zero: every sort clears the histograms it consumed in its last pass, so the
     // next step's predict can count into them without a fill (false after a step that stopped early).
     bool ghist_clean = false;
@@ -370,6 +404,10 @@ struct Engine
     unsigned* sort_ghist = nullptr;  // [m * 4 * 256]
     int sc_cap = 0;
     DevScalars* scalars = nullptr;   // [m]
+    // Device-side check failures (a lookback wait that ran out): set by kernels in host-mapped
+    // memory, read and cleared by the host after every step.
+    unsigned* h_err = nullptr;
+    unsigned* d_err = nullptr;
 
     // kernel timing
     bool timing_on[K_COUNT] = {};
@@ -379,6 +417,8 @@ struct Engine
     std::vector<hipEvent_t> event_pool;
 
     ExtractCfg ext;  // read by steps with ST_EXTRACT
+    Tuning tune;     // launch choices of the current step
+    bool tune_locked = false;  // set by a caller that chose tune for a whole multi-lane step
 
     // Step graphs: with use_graph, a step's kernel sequence is captured once into a HIP graph and
     // replayed while the plan (vehicle list, tables, scratch) and the launch signature are unchanged.
@@ -398,7 +438,7 @@ struct Engine
         long long scan_tiles_cap;
         SegCarry seg;
         long long seg_tiles_cap;
-        unsigned* sort_thist;
+        unsigned* sort_status;
         long long sort_tiles_cap;
     } own{};
     bool bound = false;
@@ -441,6 +481,9 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages);
 // vehicles' host state (epochs, buffer swaps, pending shifts).
 int engine_run(Engine* E, Vehicle* const* veh, int m, int stages);
 void engine_collect_timing(Engine* E);
+// After a step's stream has been waited for: DOGM_ERR_DEVICE_CHECK (and the message) when a kernel
+// of the step flagged a failed device-side check, else DOGM_OK.
+int engine_device_check(Engine* E);
 int vehicle_validate(const dogm_params* p, std::string* why);
 size_t vehicle_footprint(const dogm_params* p);  // device bytes vehicle_alloc takes
 int vehicle_alloc(Vehicle* v, const dogm_params* p, uint64_t seed, uint32_t instance, std::string* why);
@@ -468,7 +511,9 @@ int engine_frame(Engine* E, Vehicle* const* veh, const FrameIn* in, int m, bool 
 // The same step split over two engines ("lanes", each with its own stream, tables and scratch):
 // every vehicle runs all its stages on one lane, and the two lanes' launches overlap on the device
 // (one lane's small, latency-bound launches and tails run beside the other's bulk work).  Results
-// equal those of engine_frame.  Errors are reported on lanes[0].
+// equal those of engine_frame.  nl = 0 takes the lane count from the tuning of the whole step (both
+// lanes then use that step's tuning); lanes[1] must be initialised unless nl == 1.  Errors are
+// reported on lanes[0].
 int engine_frame_lanes(Engine* const* lanes, int nl, Vehicle* const* veh, const FrameIn* in, int m,
                        bool extract = false);
 // Allocates the extraction buffers `what` needs on a vehicle.

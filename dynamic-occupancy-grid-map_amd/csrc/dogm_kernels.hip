@@ -209,27 +209,33 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
     return 0;
 }
 
-// 512-thread workgroups of 4 particles per thread: a single 2e6-particle grid still gets about 8
-// waves per SIMD to hide the Philox / Box-Muller latency.
-constexpr int PRED_ITEMS = 4;  // particles per thread (strided by PRED_BLOCK)
-constexpr int PRED_BLOCK = 512;
+// A workgroup covers one sort tile: PB threads of SORT_TILE / PB particles each (strided by PB).
+// 512 x 4 (the built-in choice) gives a single 2e6-particle grid about 8 waves per SIMD to hide the
+// Philox / Box-Muller latency; 256 x 8 is the other choice of the tuning.
 
 // With passes > 0 the workgroup also counts the radix digits of the new cell indices for every
-// sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram), and
-// stores the first pass's digit counts of its tile as thist[digit * tiles + tile]: a workgroup
-// covers exactly one sort tile (PRED_ITEMS * PRED_BLOCK particles), so the sort skips that upsweep.
+// sort pass into ghist[(vehicle * 4 + pass) * 256 + digit] (the sort's one-time histogram) and
+// clears the sort passes' tile tickets (the passes run after predict on the same stream); with
+// thist set (reduce-then-scan sort) it stores the first pass's digit counts of its tile as
+// thist[digit * tiles + tile]: a workgroup covers exactly one sort tile, so the sort skips that
+// upsweep.
+template <int PRED_BLOCK>
 __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
                                                    int pph, int passes, unsigned* __restrict__ ghist,
-                                                   unsigned* __restrict__ thist, int tiles)
+                                                   unsigned* __restrict__ tickets, unsigned* __restrict__ thist,
+                                                   int tiles)
 {
+    constexpr int PRED_ITEMS = SORT_TILE / PRED_BLOCK;
     static_assert(PRED_ITEMS * PRED_BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
     __shared__ unsigned s_h[4][RADIX];
+    if (passes > 0 && blockIdx.x == 0 && threadIdx.x < 4)
+        tickets[threadIdx.x] = 0u;
     const TileRef tr = tile_ref(tab, SORT_Q, blockIdx.x);
     const VehDesc& d = descs[tr.v];
     const int n = d.N;
     if (tr.base >= n)
     {
-        if (passes > 0)
+        if (passes > 0 && thist)
             for (int i = threadIdx.x; i < RADIX; i += PRED_BLOCK)
                 thist[(size_t)i * tiles + blockIdx.x] = 0u;
         return;
@@ -302,8 +308,9 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             if (v)
                 atomicAdd(&gh[i], v);
         }
-        for (int i = threadIdx.x; i < RADIX; i += PRED_BLOCK)
-            thist[(size_t)i * tiles + blockIdx.x] = s_h[0][i];
+        if (thist)
+            for (int i = threadIdx.x; i < RADIX; i += PRED_BLOCK)
+                thist[(size_t)i * tiles + blockIdx.x] = s_h[0][i];
     }
 }
 
@@ -315,9 +322,15 @@ void run_predict(Engine* E, int pph, bool hist)
     if (hist && !E->ghist_clean)
         run_fill_u32(E, E->sort_ghist, 0u, (long long)E->m * 4 * RADIX);
     E->ghist_clean = false;  // until the sort of this step has cleared them again
+    unsigned* thist = E->tune.sort == SORT_REDUCE_SCAN ? E->sort_status : nullptr;
+    const int passes = hist ? E->passes : 0;
     launch_begin(E, K_PREDICT);
-    hipLaunchKernelGGL(k_predict, dim3(blocks), dim3(PRED_BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                       hist ? E->passes : 0, E->sort_ghist, E->sort_thist, blocks);
+    if (E->tune.pred_block == 256)
+        hipLaunchKernelGGL(k_predict<256>, dim3(blocks), dim3(256), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                           passes, E->sort_ghist, E->sort_tickets, thist, blocks);
+    else
+        hipLaunchKernelGGL(k_predict<512>, dim3(blocks), dim3(512), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                           passes, E->sort_ghist, E->sort_tickets, thist, blocks);
     launch_end(E, K_PREDICT);
 }
 
@@ -399,11 +412,14 @@ void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds)
 // ---------------------------------------------------------------------------------------------
 // Occupancy / mass update (dogm/src/dogm.cu:283-298, mass_update.cu:16-93)
 // ---------------------------------------------------------------------------------------------
+// IT particles per thread (the tuning's seg_items); register budgets of six waves per SIMD for 8
+// (no spills) and eight for 4.
+template <int IT>
 struct OccOp
 {
     static constexpr int NCH = 1;
-    static constexpr int ITEMS = 8;
-    static constexpr int WAVES = 6;  // register budget of six waves per SIMD (no spills)
+    static constexpr int ITEMS = IT;
+    static constexpr int WAVES = IT == 8 ? 6 : 8;
     int pph;
     int bounds;  // record start_idx / end_idx (dogm.cu:262-281) when the assignment deferred them
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
@@ -441,7 +457,7 @@ struct OccOp
 // would, and m_raw is written for them.
 __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
                                                        const uint32_t* __restrict__ tab, int zero_empty,
-                                                       SegCarry carry)
+                                                       SegCarry carry, int occ_tile)
 {
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -465,7 +481,7 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
     }
     if (occupied)
     {
-        constexpr int OT = BLOCK * OccOp::ITEMS;  // particles per reduction tile
+        const int OT = occ_tile;  // particles per reduction tile
         m_occ_pred = d.m_raw[c];
         if (m_occ_pred != m_occ_pred)  // marked: the cell crosses reduction tiles
         {
@@ -553,10 +569,13 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
 
 void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds)
 {
-    launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp{pph, bounds ? 1 : 0}, false);
+    if (E->tune.seg_items == 4)
+        launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<4>{pph, bounds ? 1 : 0}, false);
+    else
+        launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<8>{pph, bounds ? 1 : 0}, false);
     launch_begin(E, K_MASS_UPDATE);
     hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       zero_empty, E->seg);
+                       zero_empty, E->seg, BLOCK * E->tune.seg_items);
     launch_end(E, K_MASS_UPDATE);
     if (defer_normalize)
         return;
@@ -573,11 +592,12 @@ void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, boo
 // norm: normalize_weights (mass_update.cu:51-59) is applied here, when the occupancy update ran in the
 // same step: w <- w / m_occ_pred for cells whose predicted occupancy exceeds 1 (the particle weight is
 // rewritten; weight_array is overwritten by lw below either way).
+template <int IT>
 struct PersOp
 {
     static constexpr int NCH = 1;
-    static constexpr int ITEMS = 8;
-    static constexpr int WAVES = 6;  // register budget of six waves per SIMD (no spills)
+    static constexpr int ITEMS = IT;
+    static constexpr int WAVES = IT == 8 ? 6 : 8;
     int pph;
     int norm;
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
@@ -651,7 +671,10 @@ __global__ __launch_bounds__(BLOCK) void k_post_weights(const VehDesc* __restric
 
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post)
 {
-    launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp{pph, fused_normalize ? 1 : 0});
+    if (E->tune.seg_items == 4)
+        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<4>{pph, fused_normalize ? 1 : 0});
+    else
+        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<8>{pph, fused_normalize ? 1 : 0});
     if (!empty_done)
     {
         launch_begin(E, K_EMPTY_CELLS);

@@ -120,6 +120,7 @@ extern "C" int dogm_update_grid_noalloc(dogm_handle* h, const dogm_measurement_c
     E->use_graph = false;
     st = engine_run(E, list, 1, ST_ALL);
     const hipError_t e = hipStreamSynchronize(E->stream);  // the kernels use the bound buffers
+    const int dc = engine_device_check(E);
     engine_unbind_scratch(E);
     E->use_graph = graph;
     for (int k = 0; k < K_COUNT; ++k)
@@ -128,5 +129,5 @@ extern "C" int dogm_update_grid_noalloc(dogm_handle* h, const dogm_measurement_c
         return st;
     if (e != hipSuccess)
         return noalloc_fail(E, DOGM_ERR_HIP, hipGetErrorString(e));
-    return DOGM_OK;
+    return dc;
 }

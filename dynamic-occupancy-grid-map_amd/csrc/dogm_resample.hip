@@ -587,7 +587,7 @@ void run_resampling(Engine* E, int pph, int modes)
     launch_end(E, K_SCAN_TOP);
 
     const int mt = E->tiles(SP_P, RS_TILE / UNIT);
-    const bool wide = mt <= RS_WIDE_MAX_TILES;
+    const bool wide = E->tune.merge_slice == 0 ? mt <= RS_WIDE_MAX_TILES : E->tune.merge_slice == 2;
     launch_begin(E, K_RS_MERGE);
     if (exact)
     {

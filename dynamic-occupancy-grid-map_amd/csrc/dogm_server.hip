@@ -154,7 +154,7 @@ int run_step(dogm_server* s, const std::vector<dogm_server::Frame>& fr, double* 
     }
     s->eng.ext = s->ext;
     Engine* lanes[2] = {&s->eng, &s->aux};
-    const int st = engine_frame_lanes(lanes, 2, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
+    const int st = engine_frame_lanes(lanes, 0, list.data(), in.data(), (int)fr.size(), s->ext.what != 0);
     *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     if (st != DOGM_OK)
     {

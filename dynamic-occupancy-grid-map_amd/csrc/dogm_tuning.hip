@@ -1,0 +1,388 @@
+// Tuning table: launch choices per step class that change no result (the radix sort's pass form,
+// the exact-CDF merge's staged slice, predict's workgroup shape, the occupancy and persistent
+// reductions' particles per thread, and the lanes of a batched step).
+//
+// The table is JSON written by the autotuner (dynamic-occupancy-grid-map_amd/autotune.py, which
+// times every choice on the device for each step class and keeps the fastest):
+//
+//   {"entries": [{"vehicles": [1, 1], "particles": [1000000, 4000000], "sort": "reduce_scan",
+//                 "merge_slice": 2, "pred_block": 512, "seg_items": 8, "lanes": 1, ...}, ...]}
+//
+// A step of m vehicles and P particles in all takes the first entry whose closed ranges hold m and
+// P; other keys of an entry are informational.  The library loads tuning_table.json next to its
+// own directory (lib/../tuning_table.json) on first use, or the file DOGM_TUNING_TABLE names
+// ("none": no table); without a matching entry the built-in choices apply.  dogm_tuning_override
+// forces one choice for every step (the autotuner's sweep, and tests of each form).
+#include "dogm_internal.h"
+
+#include <dlfcn.h>
+
+#include <cctype>
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+namespace dogm_hip
+{
+namespace
+{
+
+struct Entry
+{
+    long long v_lo = 0, v_hi = 0, p_lo = 0, p_hi = 0;
+    Tuning t;
+};
+
+std::mutex g_mu;
+bool g_loaded = false;
+std::vector<Entry> g_table;
+bool g_override = false;
+Tuning g_forced;
+std::string g_source;  // path of the loaded table ("" when none)
+
+// ---- a small JSON reader for the table's schema (objects, arrays, strings, numbers) ----------
+
+struct JVal
+{
+    enum Kind { NUL, NUM, STR, ARR, OBJ } kind = NUL;
+    double num = 0.0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+
+    const JVal* get(const char* key) const
+    {
+        for (auto& kv : obj)
+            if (kv.first == key)
+                return &kv.second;
+        return nullptr;
+    }
+};
+
+struct Parser
+{
+    const std::string& s;
+    size_t i = 0;
+    bool ok = true;
+
+    void ws()
+    {
+        while (i < s.size() && std::isspace((unsigned char)s[i]))
+            ++i;
+    }
+    bool eat(char c)
+    {
+        ws();
+        if (i < s.size() && s[i] == c)
+        {
+            ++i;
+            return true;
+        }
+        return false;
+    }
+    std::string string()
+    {
+        std::string out;
+        if (!eat('"'))
+        {
+            ok = false;
+            return out;
+        }
+        while (i < s.size() && s[i] != '"')
+        {
+            if (s[i] == '\\' && i + 1 < s.size())
+                ++i;
+            out += s[i++];
+        }
+        if (i >= s.size())
+            ok = false;
+        ++i;
+        return out;
+    }
+    JVal value(int depth = 0)
+    {
+        JVal v;
+        ws();
+        if (!ok || i >= s.size() || depth > 16)
+        {
+            ok = false;
+            return v;
+        }
+        const char c = s[i];
+        if (c == '{')
+        {
+            ++i;
+            v.kind = JVal::OBJ;
+            if (eat('}'))
+                return v;
+            do
+            {
+                std::string k = string();
+                if (!eat(':'))
+                {
+                    ok = false;
+                    return v;
+                }
+                v.obj.emplace_back(k, value(depth + 1));
+            } while (ok && eat(','));
+            if (!eat('}'))
+                ok = false;
+        }
+        else if (c == '[')
+        {
+            ++i;
+            v.kind = JVal::ARR;
+            if (eat(']'))
+                return v;
+            do
+                v.arr.push_back(value(depth + 1));
+            while (ok && eat(','));
+            if (!eat(']'))
+                ok = false;
+        }
+        else if (c == '"')
+        {
+            v.kind = JVal::STR;
+            v.str = string();
+        }
+        else if (s.compare(i, 4, "null") == 0 || s.compare(i, 4, "true") == 0)
+        {
+            i += 4;
+        }
+        else if (s.compare(i, 5, "false") == 0)
+        {
+            i += 5;
+        }
+        else
+        {
+            char* end = nullptr;
+            v.num = std::strtod(s.c_str() + i, &end);
+            if (end == s.c_str() + i)
+                ok = false;
+            i = (size_t)(end - s.c_str());
+            v.kind = JVal::NUM;
+        }
+        return v;
+    }
+};
+
+bool range_of(const JVal* v, long long* lo, long long* hi)
+{
+    if (!v || v->kind != JVal::ARR || v->arr.size() != 2 || v->arr[0].kind != JVal::NUM ||
+        v->arr[1].kind != JVal::NUM)
+        return false;
+    *lo = (long long)v->arr[0].num;
+    *hi = (long long)v->arr[1].num;
+    return *lo <= *hi;
+}
+
+int int_of(const JVal* v, int dflt)
+{
+    return v && v->kind == JVal::NUM ? (int)v->num : dflt;
+}
+
+int parse_table(const std::string& text, std::vector<Entry>* out, std::string* why)
+{
+    Parser p{text};
+    const JVal root = p.value();
+    const JVal* ents = root.kind == JVal::OBJ ? root.get("entries") : nullptr;
+    if (!p.ok || !ents || ents->kind != JVal::ARR)
+    {
+        *why = "not a tuning table (an object with an \"entries\" array)";
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    out->clear();
+    for (const JVal& e : ents->arr)
+    {
+        Entry en;
+        if (e.kind != JVal::OBJ || !range_of(e.get("vehicles"), &en.v_lo, &en.v_hi) ||
+            !range_of(e.get("particles"), &en.p_lo, &en.p_hi))
+        {
+            *why = "entry " + std::to_string(out->size()) + " lacks its vehicles / particles ranges";
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        const JVal* srt = e.get("sort");
+        if (srt && srt->kind == JVal::STR)
+        {
+            if (srt->str == "onesweep")
+                en.t.sort = SORT_ONESWEEP;
+            else if (srt->str == "reduce_scan")
+                en.t.sort = SORT_REDUCE_SCAN;
+            else
+            {
+                *why = "unknown sort form \"" + srt->str + "\"";
+                return DOGM_ERR_INVALID_ARGUMENT;
+            }
+        }
+        en.t.merge_slice = int_of(e.get("merge_slice"), en.t.merge_slice);
+        en.t.pred_block = int_of(e.get("pred_block"), en.t.pred_block);
+        en.t.seg_items = int_of(e.get("seg_items"), en.t.seg_items);
+        en.t.lanes = int_of(e.get("lanes"), en.t.lanes);
+        std::string bad;
+        if (!tuning_valid(en.t, &bad))
+        {
+            *why = "entry " + std::to_string(out->size()) + ": " + bad;
+            return DOGM_ERR_INVALID_ARGUMENT;
+        }
+        out->push_back(en);
+    }
+    return DOGM_OK;
+}
+
+int load_locked(const std::string& path, std::string* why)
+{
+    std::ifstream f(path);
+    if (!f)
+    {
+        *why = "cannot open " + path;
+        return DOGM_ERR_INVALID_ARGUMENT;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::vector<Entry> t;
+    const int st = parse_table(ss.str(), &t, why);
+    if (st != DOGM_OK)
+        return st;
+    g_table.swap(t);
+    g_source = path;
+    return DOGM_OK;
+}
+
+std::string default_path()
+{
+    if (const char* env = std::getenv("DOGM_TUNING_TABLE"))
+        return std::string(env) == "none" ? std::string() : std::string(env);
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&default_path), &info) == 0 || !info.dli_fname)
+        return std::string();
+    std::string lib(info.dli_fname);
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+    return dir + "/../tuning_table.json";
+}
+
+void ensure_loaded_locked()
+{
+    if (g_loaded)
+        return;
+    g_loaded = true;
+    const std::string path = default_path();
+    std::string why;
+    if (!path.empty())
+    {
+        std::ifstream probe(path);
+        if (probe)
+            (void)load_locked(path, &why);  // a broken default table leaves the built-in choices
+    }
+}
+
+}  // namespace
+
+bool tuning_valid(const Tuning& t, std::string* why)
+{
+    if (t.sort != SORT_REDUCE_SCAN && t.sort != SORT_ONESWEEP)
+        return *why = "sort must be reduce_scan or onesweep", false;
+    if (t.merge_slice < 0 || t.merge_slice > 2)
+        return *why = "merge_slice must be 0 (by size), 1 or 2", false;
+    if (t.pred_block != 256 && t.pred_block != 512)
+        return *why = "pred_block must be 256 or 512", false;
+    if (t.seg_items != 4 && t.seg_items != 8)
+        return *why = "seg_items must be 4 or 8", false;
+    if (t.lanes != 1 && t.lanes != 2)
+        return *why = "lanes must be 1 or 2", false;
+    return true;
+}
+
+Tuning tuning_for(int vehicles, long long particles)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_override)
+        return g_forced;
+    ensure_loaded_locked();
+    for (const Entry& e : g_table)
+        if (vehicles >= e.v_lo && vehicles <= e.v_hi && particles >= e.p_lo && particles <= e.p_hi)
+            return e.t;
+    return Tuning{};
+}
+
+}  // namespace dogm_hip
+
+using namespace dogm_hip;
+
+namespace
+{
+
+Tuning from_c(const dogm_tuning& c)
+{
+    Tuning t;
+    t.sort = c.sort;
+    t.merge_slice = c.merge_slice;
+    t.pred_block = c.pred_block;
+    t.seg_items = c.seg_items;
+    t.lanes = c.lanes;
+    return t;
+}
+
+dogm_tuning to_c(const Tuning& t)
+{
+    dogm_tuning c{};
+    c.sort = t.sort;
+    c.merge_slice = t.merge_slice;
+    c.pred_block = t.pred_block;
+    c.seg_items = t.seg_items;
+    c.lanes = t.lanes;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dogm_tuning_load(const char* path)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_loaded = true;
+    std::string why;
+    if (!path)
+    {
+        g_table.clear();
+        g_source.clear();
+        return DOGM_OK;
+    }
+    return load_locked(path, &why);
+}
+
+int dogm_tuning_override(const dogm_tuning* t)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!t)
+    {
+        g_override = false;
+        return DOGM_OK;
+    }
+    const Tuning v = from_c(*t);
+    std::string why;
+    if (!tuning_valid(v, &why))
+        return DOGM_ERR_INVALID_ARGUMENT;
+    g_forced = v;
+    g_override = true;
+    return DOGM_OK;
+}
+
+int dogm_tuning_lookup(int vehicles, long long particles, dogm_tuning* out)
+{
+    if (!out || vehicles < 1 || particles < 0)
+        return DOGM_ERR_INVALID_ARGUMENT;
+    *out = to_c(tuning_for(vehicles, particles));
+    return DOGM_OK;
+}
+
+int dogm_tuning_entries(void)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    ensure_loaded_locked();
+    return (int)g_table.size();
+}
+
+}  // extern "C"

@@ -81,6 +81,12 @@ _h = C.c_void_p
 _i = C.c_int
 _f = C.c_float
 
+
+class dogm_tuning(C.Structure):
+    _fields_ = [("sort", C.c_int), ("merge_slice", C.c_int), ("pred_block", C.c_int), ("seg_items", C.c_int),
+                ("lanes", C.c_int)]
+
+
 _SIGS = {
     "dogm_create": (_i, [C.POINTER(dogm_params), C.POINTER(_h)]),
     "dogm_create_ex": (_i, [C.POINTER(dogm_params), C.c_uint64, C.c_uint32, C.POINTER(_h)]),
@@ -197,6 +203,10 @@ _SIGS = {
     "dogm_server_get_occupancy": (_i, [_h, _i, _vp]),
     "dogm_server_get_occupancy_image": (_i, [_h, _i, _vp]),
     "dogm_server_get_cells_with_velocity": (_i, [_h, _i, _vp, _i, C.POINTER(_i)]),
+    "dogm_tuning_load": (_i, [C.c_char_p]),
+    "dogm_tuning_override": (_i, [C.POINTER(dogm_tuning)]),
+    "dogm_tuning_lookup": (_i, [_i, C.c_longlong, C.POINTER(dogm_tuning)]),
+    "dogm_tuning_entries": (_i, []),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -890,3 +900,50 @@ class Evaluator:
         lib.dogm_evaluator_summary(self.h, _ptr(mae), _ptr(rmse), C.byref(d), C.byref(u))
         as_dict = lambda a: {k: float(a[k][0]) for k in OBJECT_DTYPE.names}  # noqa: E731
         return {"mae": as_dict(mae), "rmse": as_dict(rmse), "detections": d.value, "unassigned": u.value}
+
+
+# ---- tuning (dogm_tuning_*) -------------------------------------------------------------------------
+SORT_FORMS = {"reduce_scan": 0, "onesweep": 1}
+TUNING_KEYS = ("sort", "merge_slice", "pred_block", "seg_items", "lanes")
+
+
+def tuning_dict(t: dogm_tuning) -> dict:
+    d = {k: getattr(t, k) for k in TUNING_KEYS}
+    d["sort"] = {v: k for k, v in SORT_FORMS.items()}[d["sort"]]
+    return d
+
+
+TUNING_BUILTIN = {"sort": "reduce_scan", "merge_slice": 0, "pred_block": 512, "seg_items": 8, "lanes": 2}
+
+
+def _tuning_c(d: dict) -> dogm_tuning:
+    vals = dict(TUNING_BUILTIN)
+    vals.update(d)
+    return dogm_tuning(SORT_FORMS[vals["sort"]], int(vals["merge_slice"]), int(vals["pred_block"]),
+                       int(vals["seg_items"]), int(vals["lanes"]))
+
+
+def tuning_override(d: dict | None) -> None:
+    """Force the launch choices `d` (keys of TUNING_KEYS; missing keys take the built-in choice) for
+    every following step of the process; None returns to the table."""
+    if d is None:
+        st = lib.dogm_tuning_override(None)
+    else:
+        t = _tuning_c(d)
+        st = lib.dogm_tuning_override(C.byref(t))
+    if st != DOGM_OK:
+        raise DogmError(st, f"dogm_tuning_override({d})")
+
+
+def tuning_lookup(vehicles: int, particles: int) -> dict:
+    t = dogm_tuning()
+    st = lib.dogm_tuning_lookup(vehicles, particles, C.byref(t))
+    if st != DOGM_OK:
+        raise DogmError(st, "dogm_tuning_lookup")
+    return tuning_dict(t)
+
+
+def tuning_load(path: str | None) -> None:
+    st = lib.dogm_tuning_load(None if path is None else str(path).encode())
+    if st != DOGM_OK:
+        raise DogmError(st, f"dogm_tuning_load({path})")

@@ -128,7 +128,8 @@ def lookup_traffic(summary_md: str, kernel_prefix: str) -> int | None:
 
 def lookup_valu(summary_md: str, kernel_name: str) -> tuple[int, float] | None:
     """(grid, VALU instructions per wave) of a kernel's row at the largest grid in a counter summary
-    table (kernel | grid | calls | avg us | read MB | write MB | GB/s | valu/wave | ...)."""
+    table (kernel | grid | calls | avg us | read MB | write MB | GB/s | valu/wave | ...); a name
+    without template arguments also matches its template instances."""
     try:
         lines = open(summary_md).read().splitlines()
     except OSError:
@@ -136,7 +137,8 @@ def lookup_valu(summary_md: str, kernel_name: str) -> tuple[int, float] | None:
     best = None
     for ln in lines:
         cells = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if len(cells) < 8 or cells[0].strip("`") != kernel_name:
+        name = cells[0].strip("`") if cells else ""
+        if len(cells) < 8 or not (name == kernel_name or name.startswith(kernel_name + "<")):
             continue
         try:
             grid, valu = int(cells[1]), float(cells[7])

@@ -325,6 +325,43 @@ int dogm_get_kernel_timing(dogm_handle* h, int kernel_id, double* total_ms, long
 int dogm_reset_kernel_timing(dogm_handle* h);
 
 /* ----------------------------------------------------------------------------------------------
+ * Tuning (no counterpart in the reference, whose launch geometry is fixed: dogm/src/dogm.cu:45-47).
+ * Launch choices that change no result, per step class: the radix sort's pass form
+ * (DOGM_SORT_REDUCE_SCAN: upsweep, digit scan and scatter per pass; DOGM_SORT_ONESWEEP: one launch
+ * per pass with a decoupled lookback), the exact-CDF merge's staged CDF tiles per workgroup
+ * (merge_slice 1 or 2; 0 = by grid size), predict's workgroup (pred_block 512 threads x 4 particles
+ * or 256 x 8), the occupancy and persistent reductions' particles per thread (seg_items 4 or 8) and
+ * the lanes of batched steps (1 or 2).  A step of m vehicles and P particles in all takes the first
+ * entry of the process-wide table whose vehicle and particle ranges hold (m, P).  The table is the
+ * JSON file written by the autotuner (dynamic-occupancy-grid-map_amd/autotune.py), loaded on first
+ * use from tuning_table.json beside the library's directory or from the file DOGM_TUNING_TABLE
+ * names ("none": no table); steps without a matching entry use the built-in choices.
+ * ---------------------------------------------------------------------------------------------- */
+enum dogm_sort_form
+{
+    DOGM_SORT_REDUCE_SCAN = 0,
+    DOGM_SORT_ONESWEEP = 1
+};
+
+typedef struct dogm_tuning
+{
+    int sort;        /* dogm_sort_form */
+    int merge_slice; /* 0, 1 or 2 */
+    int pred_block;  /* 256 or 512 */
+    int seg_items;   /* 4 or 8 */
+    int lanes;       /* 1 or 2 */
+} dogm_tuning;
+
+/* Replaces the table with the one in `path` (NULL: no table). */
+int dogm_tuning_load(const char* path);
+/* Forces `t` for every following step in the process (NULL: back to the table). */
+int dogm_tuning_override(const dogm_tuning* t);
+/* The choice a step of `vehicles` vehicles and `particles` particles in all gets. */
+int dogm_tuning_lookup(int vehicles, long long particles, dogm_tuning* out);
+/* Entries of the loaded table. */
+int dogm_tuning_entries(void);
+
+/* ----------------------------------------------------------------------------------------------
  * Precision of the resampling CDF (the cumulative-weight scan behind resampling.cu:34-47 and the
  * new weight total / N, dogm.cu:386-423).  Particle state and weights stay f32 in every mode.
  *  DOGM_CDF_EXACT (default): exact 128-bit fixed-point prefix sums; the draws and the ancestors
@@ -431,10 +468,10 @@ int dogm_batch_get_resampled_indices(dogm_batch* b, int vehicle_id, int* out);
 int dogm_batch_set_cdf_mode(dogm_batch* b, int vehicle_id, int mode);
 int dogm_batch_set_ego_mode(dogm_batch* b, int vehicle_id, int mode);
 int dogm_batch_set_resampling_mode(dogm_batch* b, int vehicle_id, int mode);
-/* Lanes of a step (1 or 2, default 2).  With 2, a step over two or more vehicles splits them over
- * two engines with their own HIP streams, balanced by per-frame traffic; both lanes' launches run
- * concurrently on the device, which fills one lane's small launches and tails with the other's
- * work.  Results do not depend on the lane count. */
+/* Lanes of a step (1, 2, or 0 = as the tuning of each step gives: default).  With 2, a step over
+ * two or more vehicles splits them over two engines with their own HIP streams, balanced by
+ * per-frame traffic; both lanes' launches run concurrently on the device, which fills one lane's
+ * small launches and tails with the other's work.  Results do not depend on the lane count. */
 int dogm_batch_set_lanes(dogm_batch* b, int lanes);
 /* Step graphs for the batch (both lanes), as dogm_set_graph. */
 int dogm_batch_set_graph(dogm_batch* b, int enable);
