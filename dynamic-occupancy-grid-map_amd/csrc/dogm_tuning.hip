@@ -14,6 +14,7 @@
 // ("none": no table); without a matching entry the built-in choices apply.  dogm_tuning_override
 // forces one choice for every step (the autotuner's sweep, and tests of each form).
 #include "dogm_internal.h"
+#include "dogm_json.h"
 
 #include <dlfcn.h>
 
@@ -41,145 +42,22 @@ bool g_override = false;
 Tuning g_forced;
 std::string g_source;  // path of the loaded table ("" when none)
 
-// ---- a small JSON reader for the table's schema (objects, arrays, strings, numbers) ----------
-
-struct JVal
-{
-    enum Kind { NUL, NUM, STR, ARR, OBJ } kind = NUL;
-    double num = 0.0;
-    std::string str;
-    std::vector<JVal> arr;
-    std::vector<std::pair<std::string, JVal>> obj;
-
-    const JVal* get(const char* key) const
-    {
-        for (auto& kv : obj)
-            if (kv.first == key)
-                return &kv.second;
-        return nullptr;
-    }
-};
-
-struct Parser
-{
-    const std::string& s;
-    size_t i = 0;
-    bool ok = true;
-
-    void ws()
-    {
-        while (i < s.size() && std::isspace((unsigned char)s[i]))
-            ++i;
-    }
-    bool eat(char c)
-    {
-        ws();
-        if (i < s.size() && s[i] == c)
-        {
-            ++i;
-            return true;
-        }
-        return false;
-    }
-    std::string string()
-    {
-        std::string out;
-        if (!eat('"'))
-        {
-            ok = false;
-            return out;
-        }
-        while (i < s.size() && s[i] != '"')
-        {
-            if (s[i] == '\\' && i + 1 < s.size())
-                ++i;
-            out += s[i++];
-        }
-        if (i >= s.size())
-            ok = false;
-        ++i;
-        return out;
-    }
-    JVal value(int depth = 0)
-    {
-        JVal v;
-        ws();
-        if (!ok || i >= s.size() || depth > 16)
-        {
-            ok = false;
-            return v;
-        }
-        const char c = s[i];
-        if (c == '{')
-        {
-            ++i;
-            v.kind = JVal::OBJ;
-            if (eat('}'))
-                return v;
-            do
-            {
-                std::string k = string();
-                if (!eat(':'))
-                {
-                    ok = false;
-                    return v;
-                }
-                v.obj.emplace_back(k, value(depth + 1));
-            } while (ok && eat(','));
-            if (!eat('}'))
-                ok = false;
-        }
-        else if (c == '[')
-        {
-            ++i;
-            v.kind = JVal::ARR;
-            if (eat(']'))
-                return v;
-            do
-                v.arr.push_back(value(depth + 1));
-            while (ok && eat(','));
-            if (!eat(']'))
-                ok = false;
-        }
-        else if (c == '"')
-        {
-            v.kind = JVal::STR;
-            v.str = string();
-        }
-        else if (s.compare(i, 4, "null") == 0 || s.compare(i, 4, "true") == 0)
-        {
-            i += 4;
-        }
-        else if (s.compare(i, 5, "false") == 0)
-        {
-            i += 5;
-        }
-        else
-        {
-            char* end = nullptr;
-            v.num = std::strtod(s.c_str() + i, &end);
-            if (end == s.c_str() + i)
-                ok = false;
-            i = (size_t)(end - s.c_str());
-            v.kind = JVal::NUM;
-        }
-        return v;
-    }
-};
+using dogm_json::JVal;
+using dogm_json::Parser;
 
 bool range_of(const JVal* v, long long* lo, long long* hi)
 {
     if (!v || v->kind != JVal::ARR || v->arr.size() != 2 || v->arr[0].kind != JVal::NUM ||
         v->arr[1].kind != JVal::NUM)
         return false;
-    *lo = (long long)v->arr[0].num;
-    *hi = (long long)v->arr[1].num;
+    *lo = dogm_json::clamp_ll(v->arr[0].num);
+    *hi = dogm_json::clamp_ll(v->arr[1].num);
     return *lo <= *hi;
 }
 
 int int_of(const JVal* v, int dflt)
 {
-    return v && v->kind == JVal::NUM ? (int)v->num : dflt;
+    return v && v->kind == JVal::NUM ? (int)dogm_json::clamp_ll(v->num, -1000000, 1000000) : dflt;
 }
 
 int parse_table(const std::string& text, std::vector<Entry>* out, std::string* why)
