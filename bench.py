@@ -196,17 +196,18 @@ def run_plain(args, p, meas_bufs, poses, dt):
 
 
 def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
-    """k vehicles of the same workload (Philox instances 0..k-1, the same frame sequence) per
-    dogm_batch_update step."""
+    """k vehicles of the same workload (Philox instances 0..k-1) per dogm_batch_update step; in step f
+    vehicle i consumes resident grid (f + i) mod G at that frame's pose, so the vehicles of a step
+    read k distinct grids (each vehicle follows the scene's frame sequence from its own offset)."""
     from dogm_ctypes import Batch, DOGM, lib
     b = Batch()
     ids = [b.add_vehicle(p, 123456, i) for i in range(k)]
     names = DOGM.kernel_names()
     total = args.warmup + args.steps
+    ng = len(meas_bufs)
 
     def step(f):
-        x, y, yaw = poses[f]
-        b.update(ids, [meas_bufs[f]] * k, [(x, y, yaw)] * k, dt)
+        b.update(ids, [meas_bufs[(f + i) % ng] for i in range(k)], [poses[(f + i) % ng] for i in range(k)], dt)
 
     for f in range(args.warmup):
         if f == 1:
@@ -245,7 +246,7 @@ def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
     lib.dogm_device_synchronize()
     roof_ms, roof_n = b.kernel_timing(dominant)
     b.set_kernel_timing(dominant, False)
-    b.set_lanes(2)
+    b.set_lanes(0)  # back to the tuning's lane count
     extraction = run_extraction(args, b, ids, p, step, total) if args.extract_steps > 0 else None
     generation = run_generation(args, b, ids, sc, total) if (args.generation_steps > 0 and sc is not None) else None
     b.close()
@@ -255,7 +256,7 @@ def run_batched(args, k, p, meas_bufs, poses, dt, sc=None):
             "launches_per_step": dom_n / args.steps,
             "roofline_leg": {"dom_avg_s": roof_ms / max(1, roof_n) / 1e3, "launches_per_step": roof_n / nroof,
                              "steps": nroof, "lanes": 1},
-            "lanes": 2}
+            "lanes": "tuning"}
 
 
 EXTRACT_KERNELS = ("extract_mark", "extract_scan", "extract_compact")
@@ -270,8 +271,8 @@ def _percentiles(lat) -> dict:
 def run_latency(args, p, sc, dt, meas_bufs) -> dict:
     """Latency percentiles over `args.latency_frames` frames of the same workload: the plain path (one
     blocking updateGrid per frame) and the batched path (one blocking step of `args.batch` vehicles
-    per frame).  The ego pose advances every frame (a grid shift every frame); the measurement grids
-    cycle through the timed region's resident grids (the scene's boxes leave the sensor range after
+    per frame, vehicle i at frame f + i).  The ego pose advances every frame (a grid shift every frame);
+    the measurement grids cycle through the timed region's resident grids (the scene's boxes leave the sensor range after
     about a minute of driving, so later frames of the scene would be empty).  The first
     `args.warmup` frames of each path are not counted."""
     from dogm_ctypes import Batch, DOGM
@@ -296,9 +297,8 @@ def run_latency(args, p, sc, dt, meas_bufs) -> dict:
         ids = [b.add_vehicle(p, 123456, i) for i in range(k)]
         lat = []
         for f in range(nf):
-            x, y, yaw = poses[f]
             t1 = time.perf_counter()
-            b.update(ids, [bufs[f]] * k, [(x, y, yaw)] * k, dt)
+            b.update(ids, [bufs[(f + i) % nf] for i in range(k)], [poses[(f + i) % nf] for i in range(k)], dt)
             lat.append(time.perf_counter() - t1)
         b.close()
         out["batched"] = dict(_percentiles(lat[args.warmup:]), vehicles=k)
@@ -727,6 +727,162 @@ def _fp64_twin(g, meas, oracle_ctypes, p):
 
 
 MASS_FIELDS = ("occ_mass", "free_mass", "pers_occ_mass", "new_born_occ_mass", "pred_occ_mass", "mu_A", "mu_UA")
+MOMENT_FIELDS = ("mean_x_vel", "mean_y_vel", "var_x_vel", "var_y_vel", "covar_xy_vel")
+BIRTH_WEIGHT_FIELDS = ("w_A", "w_UA")
+
+
+def _compare_all(frame, gcells, ocells, g_birth_idx, o_birth_idx) -> dict:
+    """Error of every per-cell output of the f32 path against the FP64 checker run from the same state:
+    masses (relative), velocity moments (relative to the cell's second moment E[v^2] = var + mean^2:
+    its square root for the means, sqrt(E[vx^2] E[vy^2]) for the covariance, over cells holding
+    persistent mass) and w_A / w_UA (relative, over cells given the same number of birth particles;
+    the counts come from f32 vs f64 prefixes of the born masses and may differ by one)."""
+    out = _compare_masses(frame, gcells, ocells)
+    b = ocells
+    occ = (np.asarray(b["start_idx"]) != -1) & (np.asarray(b["pers_occ_mass"]) > 0)
+    m2x = np.asarray(b["var_x_vel"], np.float64) + np.asarray(b["mean_x_vel"], np.float64) ** 2
+    m2y = np.asarray(b["var_y_vel"], np.float64) + np.asarray(b["mean_y_vel"], np.float64) ** 2
+    scale = {"mean_x_vel": np.sqrt(m2x), "mean_y_vel": np.sqrt(m2y), "var_x_vel": m2x, "var_y_vel": m2y,
+             "covar_xy_vel": np.sqrt(m2x * m2y)}
+    for k in MOMENT_FIELDS:
+        err = np.abs(gcells[k].astype(np.float64) - np.asarray(b[k], np.float64))[occ]
+        sc = np.maximum(scale[k][occ], 1e-30)
+        rel = err / sc
+        out[k] = {"max_rel_to_second_moment": float(rel.max()) if rel.size else 0.0,
+                  "within_1e-5": float(np.mean(rel <= 1e-5)) if rel.size else 1.0, "cells": int(occ.sum())}
+    ncells = gcells.size
+    same = np.bincount(g_birth_idx, minlength=ncells) == np.bincount(o_birth_idx, minlength=ncells)
+    out["birth_counts_equal"] = float(np.mean(same))
+    for k in BIRTH_WEIGHT_FIELDS:
+        ov = np.asarray(b[k], np.float64)[same]
+        rel = np.abs(gcells[k].astype(np.float64)[same] - ov) / np.maximum(np.abs(ov), 1e-30)
+        out[k] = {"max_rel": float(rel.max()) if rel.size else 0.0, "within_1e-5": float(np.mean(rel <= 1e-5))}
+    return out
+
+
+def run_fp64_errors(p, meas, poses, dt, frames=(2, 3)) -> dict:
+    """Config-2 frames of the plain f32 GPU path against the FP64 checker: after the assignment stage
+    the checker takes the GPU handle's state (particles, grid, weights, RNG epoch) and both run the
+    occupancy, persistent, birth and moment stages (statistical_moments.cu:58-106 for the moments)."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle_ctypes  # noqa: E402  (checker: FP64 reference only)
+    from dogm_ctypes import DOGM
+    g = DOGM(p, seed=123456)
+    rows = []
+    for f in range(max(frames) + 1):
+        m = meas[f]
+        x, y, yaw = poses[f]
+        if f not in frames:
+            g.update_grid(m, x, y, yaw, dt)
+            continue
+        g.update_measurement_grid(m)
+        g.update_pose(x, y, yaw)
+        g.particle_prediction(dt)
+        g.particle_assignment()
+        twin = _fp64_twin(g, m, oracle_ctypes, p)
+        twin.epoch = _handle_epoch(g)
+        for o in (g, twin):
+            o.grid_cell_occupancy_update(dt)
+            o.update_persistent_particles()
+            o.initialize_new_particles()
+            o.statistical_moments()
+        rows.append(_compare_all(f, g.get_grid_cells(), twin.get_grid_cells(),
+                                 g.get_birth_particles()["grid_cell_idx"], twin.get_birth_particles()["grid_cell_idx"]))
+        twin.close()
+        g.resampling()
+    g.close()
+    worst = {}
+    for k in MASS_FIELDS + BIRTH_WEIGHT_FIELDS:
+        worst[k] = max(r[k]["max_rel"] for r in rows)
+    for k in MOMENT_FIELDS:
+        worst[k] = max(r[k]["max_rel_to_second_moment"] for r in rows)
+    return {"frames": list(frames), "tolerance": "1e-5: relative for masses, w_A and w_UA; relative to the cell's "
+                                                 "second moment for the velocity moments",
+            "max_error": worst, "all_within_tolerance": bool(all(v <= 1e-5 for v in worst.values())),
+            "per_frame": rows}
+
+
+def run_plain_small(args) -> dict:
+    """Config 1 (250 x 250 cells at 0.2 m, 3e5 + 3e4 particles, stationary ego): plain per-frame
+    updateGrid with resident grids, frames/s, p50/p99 and the frame's HBM fraction."""
+    import scene
+    from dogm_ctypes import DOGM, DeviceBuffer, Params
+    w = scene.WORKLOADS["plain_small"]
+    p = Params(**w["params"])
+    sc = scene.make_scene("plain_small", seed=args.seed)
+    nf = w["frames"]
+    bufs = []
+    for f in range(nf):
+        d = DeviceBuffer(sc.measurement(f).nbytes)
+        d.upload(sc.measurement(f))
+        bufs.append(d)
+    poses = [sc.ego_pose(f) for f in range(nf)]
+    g = DOGM(p, seed=123456)
+    lat = []
+    frames = max(200, nf)
+    for f in range(frames + args.warmup):
+        k = f % nf
+        t1 = time.perf_counter()
+        g.update_grid(bufs[k], *poses[k], float(w["dt"]))
+        lat.append(time.perf_counter() - t1)
+    g.close()
+    for d in bufs:
+        d.free()
+    lat = lat[args.warmup:]
+    gs = 250
+    fps = len(lat) / float(np.sum(lat))
+    fb = frame_bytes(p.particle_count, p.new_born_particle_count, gs * gs, False)
+    return {"workload": WORKLOAD_TEXT["plain_small"], "frames": len(lat), "frames_per_s": round(fps, 1),
+            **{k: v for k, v in _percentiles(lat).items() if k != "samples"},
+            "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 4)}
+
+
+def run_precision_leg(args) -> dict:
+    """Config 3, short: run_precision on 5 frames (ancestor agreement per CDF mode)."""
+    import contextlib
+    import io
+    a = argparse.Namespace(**vars(args))
+    a.precision_frames = 5
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        run_precision(a)
+    r = json.loads(buf.getvalue().strip().splitlines()[-1])
+    return {"workload": r["config"]["workload"], "frames": 5, "modes": r["modes"]}
+
+
+def run_serving_leg(args) -> dict:
+    """Config 5, short: 256 vehicles of mixed size (scans in, grids generated inside each step), 10 Hz
+    per vehicle with +-20 ms jitter, 200 GB admission budget.  A wall-clock run (the worker thread
+    steps while this thread submits every frame at its arrival time; latency = result time - arrival
+    time on the server's clock) and a virtual-time run of the same stream."""
+    import fleet as fl
+    from dogm_ctypes import Server, lib
+    vehicles = fl.make_fleet(256, seed=args.seed)
+    bank = fl.ScanBank([v.side for v in vehicles])
+    srv = Server(hbm_budget_bytes=200 * 10**9)
+    vids = [srv.admit(v.params, 123456, i) for i, v in enumerate(vehicles)]
+    assert all(v is not None for v in vids), "the config-5 fleet fits the 200 GB budget"
+    for k in range(2):
+        for j, v in enumerate(vehicles):
+            bank.submit(srv, vids[j], v.side, k, v.pose(k), 0.1, 0.0)
+        while srv.queued():
+            srv.step(0.0)
+        srv.poll(1 << 16)
+    lib.dogm_device_synchronize()
+    secs = args.serve_wall_seconds
+    ev = [(t, v, k + 2) for (t, v, k) in fl.arrivals(vehicles, secs, rate_multiplier=1.0, seed=args.seed)]
+    wall = fl.run_wallclock(srv, vehicles, vids, bank, ev)
+    ev2 = [(t, v, k + 2 + int(10 * secs) + 2) for (t, v, k) in fl.arrivals(vehicles, secs, rate_multiplier=1.0,
+                                                                             seed=args.seed + 1)]
+    virt = fl.simulate(srv, vehicles, vids, bank, ev2)
+    srv.close()
+    bank.close()
+    return {"workload": "256 vehicles, grid sides from {128, 256, 512, 768, 1024} at 0.1 m, N = 1.5 x cells, B = N/10, "
+                        "lidar scans in (grids generated on the device inside each step), 200 GB budget",
+            "arrivals": "10 Hz per vehicle, +-20 ms jitter, random phase",
+            "wall_clock": wall, "virtual_time": virt}
+
+
 
 
 def _compare_masses(frame, gcells, ocells):
@@ -789,6 +945,9 @@ def main() -> int:
                     help="batched steps with result extraction after the timed frames (0: skip)")
     ap.add_argument("--latency-frames", type=int, default=1000,
                     help="distinct frames of the latency-percentile leg after the timed region (0: skip)")
+    ap.add_argument("--config-legs", type=int, default=1,
+                    help="short legs of configs 1, 3 and 5 and the FP64 error report in the default line (0: skip)")
+    ap.add_argument("--serve-wall-seconds", type=float, default=10.0, help="wall-clock seconds of the config-5 leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the frames (scales HBM counters)")
     args = ap.parse_args()
@@ -852,6 +1011,26 @@ def main() -> int:
     if args.batch > 0 and args.uneven > 0:
         uneven = run_uneven(args)
         print(f"[bench] uneven x{args.uneven}: {json.dumps(uneven)}", file=sys.stderr, flush=True)
+
+    legs = {}
+    if args.config_legs and args.form == "both":
+        t0 = time.perf_counter()
+        legs["fp64_errors"] = run_fp64_errors(p, meas, poses, dt) if args.workload == "plain_full" else None
+        print(f"[bench] fp64 errors ({time.perf_counter() - t0:.1f} s): "
+              f"{json.dumps(legs['fp64_errors']['max_error']) if legs['fp64_errors'] else None}", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        legs["plain_small"] = run_plain_small(args)
+        print(f"[bench] config 1 ({time.perf_counter() - t0:.1f} s): {json.dumps(legs['plain_small'])}",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        legs["precision"] = run_precision_leg(args)
+        print(f"[bench] config 3 ({time.perf_counter() - t0:.1f} s): {json.dumps(legs['precision'])}",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        legs["serving"] = run_serving_leg(args)
+        print(f"[bench] config 5 ({time.perf_counter() - t0:.1f} s): {json.dumps(legs['serving'])}",
+              file=sys.stderr, flush=True)
 
     latency = None
     if args.latency_frames > args.warmup:
@@ -919,6 +1098,11 @@ def main() -> int:
         "extraction": batched.get("extraction") if batched else None,
         "measurement_generation": batched.get("generation") if batched else None,
         "batched_uneven": uneven,
+        # Short legs of the other configurations and the f32-vs-FP64 error report (config 2 frames).
+        "fp64_errors": legs.get("fp64_errors"),
+        "config_plain_small": legs.get("plain_small"),
+        "config_precision": legs.get("precision"),
+        "config_serving": legs.get("serving"),
         "particle_updates_per_s": round((N + B) * fps, 1),
         "frame_algorithmic_bytes": fb,
         "frame_hbm_fraction": round(fb * fps / (HBM_PEAK_GBS * 1e9), 5),
