@@ -678,7 +678,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (any_init)
             run_init_particles(E);
         if (stages & ST_PREDICT)
-            run_predict(E, pph, hist);
+            run_predict(E, pph, hist, hist && assign_occ);
     }
     if (stages & ST_ASSIGN)
     {

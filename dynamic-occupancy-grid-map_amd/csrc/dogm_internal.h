@@ -544,7 +544,9 @@ void run_build_tables(Engine* E);
 // the weight normalisation moves into the persistent update's load and the posterior weights into
 // the moments' load.  Each stage run alone keeps the reference's separate passes.
 void run_init_particles(Engine* E);                // pph 0
-void run_predict(Engine* E, int pph, bool hist);
+// aos: the state goes to the CDF buffer as 16-byte records for the gathering sort pass of the same
+// step (run_sort with hist_done and gather), which reads it there.
+void run_predict(Engine* E, int pph, bool hist, bool aos);
 // defer_bounds: the last sort pass writes the particles directly and the occupancy reduction records
 // the cell bounds (bounds = true there).
 void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds);  // reads pph, writes pph ^ 1

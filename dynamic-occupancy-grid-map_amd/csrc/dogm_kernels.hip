@@ -218,12 +218,15 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
 // clears the sort passes' tile tickets (the passes run after predict on the same stream); with
 // thist set (reduce-then-scan sort) it stores the first pass's digit counts of its tile as
 // thist[digit * tiles + tile]: a workgroup covers exactly one sort tile, so the sort skips that
-// upsweep.
+// upsweep.  With aos (the sort's gathering pass follows in the same step) the new state goes to the
+// vehicle's CDF buffer, free until the resampling, as 16-byte (x, y, vx, vy) records, so that the
+// gather reads one 16-byte record per particle at random instead of two 8-byte ones from two arrays;
+// nothing else reads the positions or velocities in between.
 template <int PRED_BLOCK>
 __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
                                                    int pph, int passes, unsigned* __restrict__ ghist,
                                                    unsigned* __restrict__ tickets, unsigned* __restrict__ thist,
-                                                   int tiles)
+                                                   int tiles, int aos)
 {
     constexpr int PRED_ITEMS = SORT_TILE / PRED_BLOCK;
     static_assert(PRED_ITEMS * PRED_BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
@@ -290,8 +293,15 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
                 w = zero;
             cell = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
-            P.pos[i] = make_float2(o.x, o.y);
-            P.vel[i] = make_float2(o.z, o.w);
+            if (aos)
+            {
+                reinterpret_cast<float4*>(d.cdf)[i] = o;
+            }
+            else
+            {
+                P.pos[i] = make_float2(o.x, o.y);
+                P.vel[i] = make_float2(o.z, o.w);
+            }
             P.weight[i] = w;
             P.idx[i] = cell;
         }
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
     }
 }
 
-void run_predict(Engine* E, int pph, bool hist)
+void run_predict(Engine* E, int pph, bool hist, bool aos)
 {
     const int blocks = E->tiles(SP_P, SORT_Q);
     if (blocks <= 0)
@@ -327,10 +337,10 @@ void run_predict(Engine* E, int pph, bool hist)
     launch_begin(E, K_PREDICT);
     if (E->tune.pred_block == 256)
         hipLaunchKernelGGL(k_predict<256>, dim3(blocks), dim3(256), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           passes, E->sort_ghist, E->sort_tickets, thist, blocks);
+                           passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0);
     else
         hipLaunchKernelGGL(k_predict<512>, dim3(blocks), dim3(512), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           passes, E->sort_ghist, E->sort_tickets, thist, blocks);
+                           passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0);
     launch_end(E, K_PREDICT);
 }
 

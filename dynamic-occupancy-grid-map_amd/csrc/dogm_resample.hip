@@ -299,6 +299,36 @@ struct CdfFloatTr
     }
 };
 
+// A chunk of CDF entries staged in LDS.  Exact entries (128-bit) are kept as separate high and low
+// 64-bit words: the lower-bound search compares the high words (8-byte LDS reads, half the bytes
+// and bank traffic of 16-byte ones) and reads a low word only where the high words tie.
+template <class V, int LEN>
+struct CdfStage
+{
+    V v[LEN];
+    __device__ void put(int i, const V& x) { v[i] = x; }
+    __device__ V get(int i) const { return v[i]; }
+    __device__ bool less(int i, const V& r) const { return v[i] < r; }
+};
+
+template <int LEN>
+struct CdfStage<fx_t, LEN>
+{
+    int64_t hi[LEN];
+    uint64_t lo[LEN];
+    __device__ void put(int i, const fx_t& x)
+    {
+        hi[i] = (int64_t)(x >> 64);
+        lo[i] = (uint64_t)x;
+    }
+    __device__ fx_t get(int i) const { return (fx_t)(((ufx_t)(uint64_t)hi[i] << 64) | (ufx_t)lo[i]); }
+    __device__ bool less(int i, const fx_t& r) const
+    {
+        const int64_t h = hi[i], rh = (int64_t)(r >> 64);
+        return h != rh ? h < rh : lo[i] < (uint64_t)r;
+    }
+};
+
 // Index of the first CDF tile whose inclusive end is >= r (T - 1 if none): two-level k-ary search.
 template <class Tr>
 __device__ int find_cdf_tile(const void* __restrict__ ci, int t0, int T, typename Tr::V r, int* s_cnt)
@@ -336,7 +366,7 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
 {
     using V = typename Tr::V;
     constexpr int RS_SLICE = RS_SLICE_TILES * SCAN_TILE;
-    __shared__ V s_cdf[RS_SLICE];
+    __shared__ CdfStage<V, RS_SLICE> s_cdf;
     __shared__ V s_rb[2];
     __shared__ int s_cnt[1];
     __shared__ int s_kmin;
@@ -419,23 +449,23 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
                 }
                 fx_t ttot;
                 fx_t run = block_exclusive_scan_fx(sum, &ttot) + (tt > 0 ? Tr::incl(cdf_incl, t0, tt - 1) : (fx_t)0);
-                fx_t* dst = &s_cdf[(tt - tc) * SCAN_TILE + threadIdx.x * SCAN_ITEMS];
+                const int at = (tt - tc) * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
 #pragma unroll
                 for (int i = 0; i < SCAN_ITEMS; ++i)
                 {
                     run += v[i];
                     if (i < cnt)
-                        dst[i] = run;
+                        s_cdf.put(at + i, run);
                 }
             }
         }
         else
         {
             for (int i = threadIdx.x; i < clen; i += BLOCK)
-                s_cdf[i] = Tr::value(d, cdf_incl, t0, c0 + i);
+                s_cdf.put(i, Tr::value(d, cdf_incl, t0, c0 + i));
         }
         __syncthreads();
-        const V cmax = s_cdf[clen - 1];
+        const V cmax = s_cdf.get(clen - 1);
         int pos = -1;  // lower bound of the previous draw inside this chunk
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
@@ -449,7 +479,7 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
                 {
                     lo = pos;
                     hi = pos;
-                    for (int step = 1; hi < clen - 1 && s_cdf[hi] < r[i]; step <<= 1)
+                    for (int step = 1; hi < clen - 1 && s_cdf.less(hi, r[i]); step <<= 1)
                     {
                         lo = hi + 1;
                         hi = min(clen - 1, hi + step);
@@ -459,7 +489,7 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
                 while (count > 0)
                 {
                     const int step = count >> 1;
-                    if (s_cdf[lo + step] < r[i])
+                    if (s_cdf.less(lo + step, r[i]))
                     {
                         lo += step + 1;
                         count -= step + 1;
@@ -505,7 +535,7 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
     }
     // Transpose through LDS so the gather and the stores run wave-contiguous.
     __syncthreads();
-    int* s_anc = reinterpret_cast<int*>(s_cdf);
+    int* s_anc = reinterpret_cast<int*>(&s_cdf);
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
         s_anc[threadIdx.x * RS_ITEMS + i] = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);

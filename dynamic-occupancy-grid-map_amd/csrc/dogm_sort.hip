@@ -195,12 +195,13 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
 // gather needs no random byte load); later passes read the previous pass's output.
 // GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
 // sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
-// weight_array; the cell bounds are then recorded by the next segmented reduction.
+// weight_array; the cell bounds are then recorded by the next segmented reduction.  With aos the
+// state is read from the 16-byte records predict left in the CDF buffer.
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
                                                           const unsigned* __restrict__ goff, int passes,
-                                                          unsigned* __restrict__ ghist)
+                                                          unsigned* __restrict__ ghist, int aos)
 {
     __shared__ uint16_t s_wcnt[SORT_WAVES][RADIX];  // counts <= SORT_TILE; 16 bits keep 8 workgroups per CU
     __shared__ unsigned s_digoff[RADIX];
@@ -331,8 +332,17 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
             const DevParticles dst = dv.part[pph ^ 1];
             const int o = sort_value_index(vv[b]);
             const float w = src.weight[o];
-            dst.pos[g] = src.pos[o];
-            dst.vel[g] = src.vel[o];
+            if (aos)
+            {
+                const float4 st = reinterpret_cast<const float4*>(dv.cdf)[o];
+                dst.pos[g] = make_float2(st.x, st.y);
+                dst.vel[g] = make_float2(st.z, st.w);
+            }
+            else
+            {
+                dst.pos[g] = src.pos[o];
+                dst.vel[g] = src.vel[o];
+            }
             dst.weight[g] = w;
             dst.assoc[g] = sort_value_assoc(vv[b]);
             dst.idx[g] = kk[b];
@@ -407,13 +417,14 @@ __device__ __forceinline__ unsigned lb_read(const unsigned* w)
 // passes read the previous pass's output.
 // GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
 // sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
-// weight_array; the cell bounds are then recorded by the next segmented reduction.
+// weight_array; the cell bounds are then recorded by the next segmented reduction.  With aos the
+// state is read from the 16-byte records predict left in the CDF buffer.
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restrict__ descs,
                                                          const uint32_t* __restrict__ tab, int pph, int ps, int passes,
                                                          unsigned* __restrict__ ghist, unsigned* __restrict__ status,
                                                          unsigned* __restrict__ tickets, unsigned epoch,
-                                                         int spin_limit, unsigned* __restrict__ err)
+                                                         int spin_limit, unsigned* __restrict__ err, int aos)
 {
     __shared__ uint16_t s_wcnt[SORT_WAVES][RADIX];  // counts <= SORT_TILE; 16 bits keep 8 workgroups per CU
     __shared__ unsigned s_digoff[RADIX];
@@ -628,8 +639,17 @@ __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restri
             const DevParticles dst = dv.part[pph ^ 1];
             const int o = sort_value_index(vv[b]);
             const float w = src.weight[o];
-            dst.pos[g] = src.pos[o];
-            dst.vel[g] = src.vel[o];
+            if (aos)
+            {
+                const float4 st = reinterpret_cast<const float4*>(dv.cdf)[o];
+                dst.pos[g] = make_float2(st.x, st.y);
+                dst.vel[g] = make_float2(st.z, st.w);
+            }
+            else
+            {
+                dst.pos[g] = src.pos[o];
+                dst.vel[g] = src.vel[o];
+            }
             dst.weight[g] = w;
             dst.assoc[g] = sort_value_assoc(vv[b]);
             dst.idx[g] = kk[b];
@@ -652,6 +672,8 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
     if (tiles <= 0)
         return;
     const bool onesweep = E->tune.sort == SORT_ONESWEEP;
+    // predict of the same step left the state as 16-byte records in the CDF buffer (run_predict)
+    const bool aos = hist_done && gather;
     if (onesweep && (E->status_plan != E->plan_epoch || passes == 1))
     {
         // New tiles (or a new buffer, or words of the other form): no word may look published.
@@ -682,11 +704,11 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
             if (g)
                 hipLaunchKernelGGL(k_sort_onesweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                    E->tab(SP_P), pph, ps, passes, E->sort_ghist, E->sort_status, E->sort_tickets,
-                                   epoch, E->lookback_spin, E->d_err);
+                                   epoch, E->lookback_spin, E->d_err, aos ? 1 : 0);
             else
                 hipLaunchKernelGGL(k_sort_onesweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                    E->tab(SP_P), pph, ps, passes, E->sort_ghist, E->sort_status, E->sort_tickets,
-                                   epoch, E->lookback_spin, E->d_err);
+                                   epoch, E->lookback_spin, E->d_err, 0);
             launch_end(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
             continue;
         }
@@ -705,10 +727,10 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
         launch_begin(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
         if (g)
             hipLaunchKernelGGL(k_sort_downsweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
-                               pph, ps, tiles, E->sort_status, passes, E->sort_ghist);
+                               pph, ps, tiles, E->sort_status, passes, E->sort_ghist, aos ? 1 : 0);
         else
             hipLaunchKernelGGL(k_sort_downsweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
-                               E->tab(SP_P), pph, ps, tiles, E->sort_status, passes, E->sort_ghist);
+                               E->tab(SP_P), pph, ps, tiles, E->sort_status, passes, E->sort_ghist, 0);
         launch_end(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
     }
     E->ghist_clean = true;  // the last pass cleared the histograms
