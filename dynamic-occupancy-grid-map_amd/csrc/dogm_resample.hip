@@ -301,31 +301,44 @@ struct CdfFloatTr
 
 // A chunk of CDF entries staged in LDS.  Exact entries (128-bit) are kept as separate high and low
 // 64-bit words: the lower-bound search compares the high words (8-byte LDS reads, half the bytes
-// and bank traffic of 16-byte ones) and reads a low word only where the high words tie.
+// and bank traffic of 16-byte ones) and reads a low word only where the high words tie.  Entry i
+// sits at slot i + i / 32: the rebuild writes four consecutive entries per thread, and without the
+// skew the lanes of a wave would hit the same banks eight at a time (entries 32 words apart).
+__device__ __forceinline__ int cdf_slot(int i)
+{
+    return i + (i >> 5);
+}
+
 template <class V, int LEN>
 struct CdfStage
 {
-    V v[LEN];
-    __device__ void put(int i, const V& x) { v[i] = x; }
-    __device__ V get(int i) const { return v[i]; }
-    __device__ bool less(int i, const V& r) const { return v[i] < r; }
+    V v[LEN + LEN / 32];
+    __device__ void put(int i, const V& x) { v[cdf_slot(i)] = x; }
+    __device__ V get(int i) const { return v[cdf_slot(i)]; }
+    __device__ bool less(int i, const V& r) const { return v[cdf_slot(i)] < r; }
 };
 
 template <int LEN>
 struct CdfStage<fx_t, LEN>
 {
-    int64_t hi[LEN];
-    uint64_t lo[LEN];
+    int64_t hi[LEN + LEN / 32];
+    uint64_t lo[LEN + LEN / 32];
     __device__ void put(int i, const fx_t& x)
     {
-        hi[i] = (int64_t)(x >> 64);
-        lo[i] = (uint64_t)x;
+        const int s = cdf_slot(i);
+        hi[s] = (int64_t)(x >> 64);
+        lo[s] = (uint64_t)x;
     }
-    __device__ fx_t get(int i) const { return (fx_t)(((ufx_t)(uint64_t)hi[i] << 64) | (ufx_t)lo[i]); }
+    __device__ fx_t get(int i) const
+    {
+        const int s = cdf_slot(i);
+        return (fx_t)(((ufx_t)(uint64_t)hi[s] << 64) | (ufx_t)lo[s]);
+    }
     __device__ bool less(int i, const fx_t& r) const
     {
-        const int64_t h = hi[i], rh = (int64_t)(r >> 64);
-        return h != rh ? h < rh : lo[i] < (uint64_t)r;
+        const int s = cdf_slot(i);
+        const int64_t h = hi[s], rh = (int64_t)(r >> 64);
+        return h != rh ? h < rh : lo[s] < (uint64_t)r;
     }
 };
 
@@ -379,20 +392,27 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         return;
     const DevScalars* sc = scs + tr.v;
     const int J = n + d.B;
-    const int p0 = base + threadIdx.x * RS_ITEMS;
     const Spacing sp = sc->spacing;
+    const int last = min(base + RS_TILE, n) - 1;
+    // Item i of thread t is draw base + t + i * BLOCK (strided), so at every item a wave searches 64
+    // consecutive draws, whose lower bounds are neighbouring CDF entries: the probes of a wave read
+    // neighbouring LDS words instead of words a fixed stride apart (bank conflicts), and the
+    // ancestors come out wave-contiguous for the gather.
     V r[RS_ITEMS];
     if (d.draw_mode == DRAW_SYSTEMATIC)
     {
         const uint32_t u = systematic_u32(d.seed, d.inst, d.ep_rs);
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
-            r[i] = Tr::draw(systematic_s(p0 + i, u), sp, sc, d.cdf_mode);
+            r[i] = Tr::draw(systematic_s(base + (int)threadIdx.x + i * BLOCK, u), sp, sc, d.cdf_mode);
     }
     else
     {
+        // The spacings' prefix runs over four consecutive draws per thread (one Philox call each);
+        // the draws are then moved to their strided owners through LDS.
+        const int q0 = base + threadIdx.x * RS_ITEMS;
         fx_t e[4];
-        exp4(d, p0, n, e);
+        exp4(d, q0, n, e);
         fx_t tot;
         fx_t run = block_exclusive_scan_fx(e[0] + e[1] + e[2] + e[3], &tot) +
                    exp_tile_pre[d.u0[SP_E] / SCAN_Q + base / RS_TILE];
@@ -400,16 +420,22 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         for (int i = 0; i < RS_ITEMS; ++i)
         {
             run += e[i];
-            r[i] = Tr::draw(run, sp, sc, d.cdf_mode);
+            s_cdf.put(threadIdx.x * RS_ITEMS + i, Tr::draw(run, sp, sc, d.cdf_mode));
         }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+            r[i] = s_cdf.get(threadIdx.x + i * BLOCK);
     }
-    const int last = min(base + RS_TILE, n) - 1;
-    if (p0 == base)
-        s_rb[0] = r[0];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
-        if (p0 + i == last)
+    {
+        const int k = base + (int)threadIdx.x + i * BLOCK;
+        if (k == base)
+            s_rb[0] = r[i];
+        if (k == last)
             s_rb[1] = r[i];
+    }
     __syncthreads();
     const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
     const int t0 = d.u0[SP_J] / SCAN_Q;
@@ -417,9 +443,8 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
     const int t_hi = find_cdf_tile<Tr>(cdf_incl, t0, T, s_rb[1], s_cnt);
     // Walk the CDF tiles [t_lo, t_hi] in LDS-sized chunks; a draw is resolved in the first chunk
     // whose last entry is >= its value (the CDF is non-decreasing, so its lower bound is there).
-    // A thread's four draws are consecutive, so after one binary search the next lower bounds are
-    // found by galloping forward from the previous one (bounded by 2 log2(RS_SLICE) probes even
-    // where consecutive draws are thousands of entries apart).
+    // A thread's later items are later draws, so each search starts at the previous item's lower
+    // bound.
     int anc[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
@@ -466,25 +491,15 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         }
         __syncthreads();
         const V cmax = s_cdf.get(clen - 1);
-        int pos = -1;  // lower bound of the previous draw inside this chunk
+        int pos = 0;  // lower bound of the thread's previous draw inside this chunk
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
         {
-            if (anc[i] < 0 && p0 + i < n && !(cmax < r[i]))
+            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n && !(cmax < r[i]))
             {
                 // Lower bound of r[i] in [lo, hi]; s_cdf[hi] >= r[i] holds throughout (cmax >= r[i]).
-                int lo = 0;
+                int lo = pos;
                 int hi = clen - 1;
-                if (pos >= 0)
-                {
-                    lo = pos;
-                    hi = pos;
-                    for (int step = 1; hi < clen - 1 && s_cdf.less(hi, r[i]); step <<= 1)
-                    {
-                        lo = hi + 1;
-                        hi = min(clen - 1, hi + step);
-                    }
-                }
                 int count = hi - lo;
                 while (count > 0)
                 {
@@ -513,8 +528,8 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         int mine = n;
 #pragma unroll
         for (int i = RS_ITEMS - 1; i >= 0; --i)
-            if (anc[i] < 0 && p0 + i < n)
-                mine = p0 + i;
+            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n)
+                mine = base + (int)threadIdx.x + i * BLOCK;
         if (mine < n)
             atomicMin(&s_kmin, mine);
         __syncthreads();
@@ -523,7 +538,7 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
             break;
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
-            if (p0 + i == kmin)
+            if (base + (int)threadIdx.x + i * BLOCK == kmin)
                 s_rb[0] = r[i];
         __syncthreads();
         const V rmin = s_rb[0];
@@ -533,13 +548,6 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         else
             tc = max(tc_end + 1, find_cdf_tile<Tr>(cdf_incl, t0, T, rmin, s_cnt));
     }
-    // Transpose through LDS so the gather and the stores run wave-contiguous.
-    __syncthreads();
-    int* s_anc = reinterpret_cast<int*>(&s_cdf);
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i)
-        s_anc[threadIdx.x * RS_ITEMS + i] = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);
-    __syncthreads();
     const float nw = sc->new_weight;
     const DevParticles P = d.part[pph];
     const DevParticles Bp = d.birth;
@@ -547,11 +555,10 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
     {
-        const int j = i * BLOCK + threadIdx.x;
-        const int k = base + j;
+        const int k = base + i * BLOCK + (int)threadIdx.x;
         if (k >= n)
             break;
-        const int a = s_anc[j];
+        const int a = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);
         d.ancestors[k] = a;
         const bool from_p = a < n;
         const int o = from_p ? a : a - n;

@@ -288,6 +288,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
                 wc += s_scan[w];
         s_digoff[d] = wc + incl - run;
     }
+    // The digit's destination base for this tile: one load per digit instead of one per key.
+    const unsigned gbase = goff[(size_t)threadIdx.x * tiles + tile];
     __syncthreads();
 
 #pragma unroll
@@ -302,6 +304,9 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
             s_vals[s] = val[j];
         }
     }
+    __syncthreads();
+    // From here on s_digoff holds destination minus tile-local slot per digit.
+    s_digoff[threadIdx.x] = gbase - s_digoff[threadIdx.x];
     __syncthreads();
 
     const int cnt = min(SORT_TILE, n - tbase);
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         kk[b] = s_keys[s];
         vv[b] = s_vals[s];
         const unsigned d = ((unsigned)kk[b] >> shift) & (RADIX - 1);
-        gg[b] = goff[(size_t)d * tiles + tile] + ((unsigned)s - s_digoff[d]);
+        gg[b] = s_digoff[d] + (unsigned)s;
     }
 #pragma unroll
     for (int b = 0; b < GB; ++b)
