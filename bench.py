@@ -562,6 +562,15 @@ def run_serving(args) -> int:
         "sweep": sweep,
         "max_frames_per_s_with_p99_under_10ms": best["frames_per_s"] if best else None,
     }
+    if args.serve_wall_seconds > 0:
+        # The nominal stream once more on the wall clock: the worker thread steps while arrivals are
+        # submitted in real time.
+        k0 = 2 + int(10 * max(args.serve_seconds, args.sweep_seconds)) + 2
+        ev = [(t, v, k + k0) for (t, v, k) in fl.arrivals(fleet_adm, args.serve_wall_seconds, rate_multiplier=1.0,
+                                                           seed=args.seed + 1000)]
+        say(f"wall clock: {len(ev)} frames over {args.serve_wall_seconds} s")
+        out["wall_clock"] = fl.run_wallclock(srv, fleet_adm, vids_adm, bank, ev)
+        say(json.dumps(out["wall_clock"]))
     print(json.dumps(out))
     srv.close()
     if isinstance(bank, fl.ScanBank):

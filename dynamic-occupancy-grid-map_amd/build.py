@@ -64,7 +64,8 @@ def build_lib(jobs: int = 8) -> Path:
         list(ex.map(compile_one, zip(sources, objs)))
     out = LIB / SONAME
     tmp = LIB / (SONAME + ".tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
     return out
 

@@ -3,6 +3,8 @@
 // the optional per-kernel event timing.
 #include "dogm_internal.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -183,6 +185,16 @@ uint64_t next_uid()
 
 namespace
 {
+
+// A roctx range around the host-side enqueue of one stage (rocprofv3 --marker-trace shows them
+// on the host timeline beside the stage's kernels); a no-op when no tool listens.
+struct StageRange
+{
+    explicit StageRange(const char* name) { roctxRangePush(name); }
+    ~StageRange() { roctxRangePop(); }
+    StageRange(const StageRange&) = delete;
+    StageRange& operator=(const StageRange&) = delete;
+};
 
 // Element counts of the plan-sized step buffers of a step over `veh`.
 struct PlanSizes
@@ -672,39 +684,67 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     const bool assign_occ = (stages & ST_ASSIGN) && (stages & ST_OCC);
     const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
     const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
+    StageRange step_range(replay ? "dogm step (graph replay)" : "dogm step");
     if (!replay)
     {
         // pph below follows the same sequence in capture and replay
         if (any_init)
+        {
+            StageRange r("initializeParticles");
             run_init_particles(E);
+        }
         if (stages & ST_PREDICT)
+        {
+            StageRange r("particlePrediction");
             run_predict(E, pph, hist, hist && assign_occ);
+        }
     }
     if (stages & ST_ASSIGN)
     {
         if (!replay)
+        {
+            StageRange r("particleAssignment");
             run_assignment(E, pph, hist, assign_occ);
+        }
         pph ^= 1;
     }
     if (!replay)
     {
         if (stages & ST_OCC)
+        {
+            StageRange r("gridCellOccupancyUpdate");
             run_occupancy(E, pph, zero_empty, occ_pers, assign_occ);
+        }
         if (stages & ST_PERS)
+        {
+            StageRange r("updatePersistentParticles");
             run_persistent(E, pph, occ_pers, (zero_empty & 1) != 0, pers_mom);
+        }
         if (stages & ST_BIRTH)
+        {
+            StageRange r("initializeNewParticles");
             run_births(E);
+        }
         if (stages & ST_MOMENTS)
+        {
+            StageRange r("statisticalMoments");
             run_moments(E, pph, pers_mom, (zero_empty & 2) != 0);
+        }
     }
     if (stages & ST_RESAMPLE)
     {
         if (!replay)
+        {
+            StageRange r("resampling");
             run_resampling(E, pph, modes);
+        }
         pph ^= 1;
     }
     if ((stages & ST_EXTRACT) && !replay)
+    {
+        StageRange r("extraction");
         run_extract(E, (stages & ST_OCC) ? 1 : 0);
+    }
     if (graph)
     {
         if (!replay)

@@ -149,9 +149,26 @@ def lookup_valu(summary_md: str, kernel_name: str) -> tuple[int, float] | None:
     return best
 
 
+def marker_summary(csv_path: str, title: str, command: str) -> str:
+    """roctx ranges of a run (rocprofv3 --marker-trace --stats --output-format csv: the marker statistics
+    CSV with Name, Calls, TotalDurationNs, AverageNs): host time spent inside each stage's range, i.e.
+    enqueueing its kernels (the kernels themselves run asynchronously on the device)."""
+    rows = [(r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]))
+            for r in csv.DictReader(open(csv_path))]
+    out = [f"# {title}", "", "Command (from the repository root, after `export TMPDIR=/tmp`):", "", f"    {command}", "",
+           "Host time inside each roctx range of the library (a stage's range covers the enqueue of its kernels; the "
+           "kernels run asynchronously, so device time is in the kernel trace).", "",
+           "| range | calls | total us | avg us |", "|---|---:|---:|---:|"]
+    for name, calls, t, avg in sorted(rows, key=lambda r: -r[2]):
+        out.append(f"| `{name}` | {calls} | {t / 1e3:.1f} | {avg / 1e3:.2f} |")
+    return "\n".join(out) + "\n"
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 5 and sys.argv[1] == "trace":
         print(trace_summary(sys.argv[2], sys.argv[3], sys.argv[4]), end="")
+    elif len(sys.argv) >= 5 and sys.argv[1] == "markers":
+        print(marker_summary(sys.argv[2], sys.argv[3], sys.argv[4]), end="")
     elif len(sys.argv) >= 6 and sys.argv[1] == "pmc":
         print(pmc_summary(sys.argv[2], int(sys.argv[3]), sys.argv[4], sys.argv[5]), end="")
     else:
