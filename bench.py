@@ -69,18 +69,32 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
 
 # Profiled kernel names (rocprofv3) of the per-kernel timers.
 PROFILED_NAME = {
-    "predict": "k_predict", "sort_downsweep": "k_sort_onesweep<false>", "sort_gather": "k_sort_onesweep<true>",
-    "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>", "seg_moments": "k_moments<4>",
-    "mass_update": "k_mass_update", "rs_merge": "k_rs_merge", "birth_cells": "k_scan_apply<SrcBorn,SinkBirth>",
+    "predict": "k_predict", "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>",
+    "seg_moments": "k_moments<4>", "mass_update": "k_mass_update", "rs_merge": "k_rs_merge",
+    "birth_cells": "k_scan_apply<SrcBorn,SinkBirth>",
+}
+# The scatter and gathering passes of the radix sort, by the pass form the tuning gives the step.
+SORT_PROFILED_NAME = {
+    "reduce_scan": {"sort_downsweep": "k_sort_downsweep<false>", "sort_gather": "k_sort_downsweep<true>"},
+    "onesweep": {"sort_downsweep": "k_sort_onesweep<false>", "sort_gather": "k_sort_onesweep<true>"},
 }
 
 
-def measured_traffic(kernel: str, workload: str, vehicles: int) -> tuple[int | None, str | None]:
+def profiled_name(kernel: str, vehicles: int, particles: int) -> str | None:
+    """Kernel-name prefix of a bench stage in the rocprofv3 summaries; the sort stages resolve to the
+    pass form the tuning table selects for a step of `vehicles` vehicles and `particles` particles."""
+    if kernel in ("sort_downsweep", "sort_gather"):
+        from dogm_ctypes import tuning_lookup
+        return SORT_PROFILED_NAME[tuning_lookup(vehicles, particles)["sort"]][kernel]
+    return PROFILED_NAME.get(kernel)
+
+
+def measured_traffic(kernel: str, workload: str, vehicles: int, particles: int) -> tuple[int | None, str | None]:
     """HBM bytes per launch of `kernel` from the committed counter summary of this workload and
     batch size (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), if one exists."""
     import profiling
     path = REPO / "profiles" / f"hbm_traffic_{workload}_batch{vehicles}.md"
-    name = PROFILED_NAME.get(kernel)
+    name = profiled_name(kernel, vehicles, particles)
     if name is None or not path.exists():
         return None, None
     return profiling.lookup_traffic(str(path), name), str(path.relative_to(REPO))
@@ -92,13 +106,13 @@ SIMDS = 1024
 ENGINE_HZ = 2.4e9
 
 
-def valu_issue(kernel: str, workload: str, vehicles: int, avg_launch_s: float) -> dict | None:
+def valu_issue(kernel: str, workload: str, vehicles: int, particles: int, avg_launch_s: float) -> dict | None:
     """Fraction of the VALU issue rate a launch of `kernel` uses: (waves x VALU instructions per wave
     x 4 cycles) / (SIMDs x clock x launch time), with the instruction count from the committed counter
     summary (SQ_INSTS_VALU / SQ_WAVES at the largest grid recorded, the batched launch)."""
     import profiling
     path = REPO / "profiles" / f"hbm_traffic_{workload}_batch{vehicles}.md"
-    name = PROFILED_NAME.get(kernel)
+    name = profiled_name(kernel, vehicles, particles)
     if name is None or not path.exists() or not avg_launch_s:
         return None
     row = profiling.lookup_valu(str(path), name)
@@ -1068,7 +1082,7 @@ def main() -> int:
 
     shifted = args.workload == "plain_full"
     fb = frame_bytes(N, B, C, shifted)
-    traffic, traffic_src = measured_traffic(dominant, args.workload, kveh)
+    traffic, traffic_src = measured_traffic(dominant, args.workload, kveh, kveh * N)
     cpu = None
     if args.cpu_frames > 0:
         cpu = cpu_baseline(args.workload, min(args.cpu_frames, total), meas, poses, dt)
@@ -1127,7 +1141,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "valu_issue": valu_issue(dominant, args.workload, kveh, dom_avg_s),
+            "valu_issue": valu_issue(dominant, args.workload, kveh, kveh * N, dom_avg_s),
             "avg_launch_us": round(dom_avg_s * 1e6, 3),
             "algorithmic_bytes_per_launch": dbytes,
             "measured_on": (f"{roof['steps']} steps of the same {kveh}-vehicle batch on one lane after the timed "
