@@ -70,7 +70,7 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
 # Profiled kernel names (rocprofv3) of the per-kernel timers.
 PROFILED_NAME = {
     "predict": "k_predict", "seg_occ": "k_segreduce<OccOp>", "seg_pers": "k_segreduce<PersOp>",
-    "seg_moments": "k_moments<4>", "mass_update": "k_mass_update", "rs_merge": "k_rs_merge",
+    "seg_moments": "k_moments<4>", "mass_update": "k_mass_update", "rs_merge": "k_rs_merge_exact",
     "birth_cells": "k_scan_apply<SrcBorn,SinkBirth>",
 }
 # The scatter and gathering passes of the radix sort, by the pass form the tuning gives the step.

@@ -665,9 +665,15 @@ constexpr int SCAN_Q = SCAN_TILE / UNIT;  // units per scan tile
 //   __device__ bool active(const VehDesc& d) const;                   // vehicle takes part
 //   __device__ int len(const VehDesc& d) const;                       // elements of the vehicle
 //   __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const;
+// A CDF segment is one wave's share of a scan tile.  With seg_end set, the tile-local inclusive end
+// of every segment is written at seg_end[CDF_SEGS_PER_TILE * tile + segment] (k_scan_top turns
+// these into the vehicle's inclusive segment ends: the resampling CDF index).
+static_assert(CDF_SEG == WAVE * SCAN_ITEMS && CDF_SEGS_PER_TILE * CDF_SEG == SCAN_TILE, "CDF segments are waves");
+
 template <class Src>
 __global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __restrict__ descs,
-                                                      const uint32_t* __restrict__ tab, fx_t* __restrict__ tile_sum)
+                                                      const uint32_t* __restrict__ tab, fx_t* __restrict__ tile_sum,
+                                                      fx_t* __restrict__ seg_end)
 {
     const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -685,9 +691,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __
         if (i < cnt)
             s += fx_from_f32(v[i]);
     fx_t total;
-    block_exclusive_scan_fx(s, &total);
+    const fx_t ex = block_exclusive_scan_fx(s, &total);
     if (threadIdx.x == 0)
         tile_sum[blockIdx.x] = total;
+    if (seg_end && (threadIdx.x & (WAVE - 1)) == WAVE - 1)
+        seg_end[(size_t)CDF_SEGS_PER_TILE * blockIdx.x + threadIdx.x / WAVE] = ex + s;
 }
 
 enum ScanMode
@@ -712,8 +720,9 @@ __device__ __forceinline__ bool scan_takes(const VehDesc& d, int only)
 }
 
 // One workgroup per vehicle: exclusive prefix of the vehicle's tile totals (each thread scans a
-// contiguous chunk), optional inclusive tile ends, and the scalars derived from the vehicle's
-// total.  Vehicles the filter `only` excludes are skipped.
+// contiguous chunk), optionally the inclusive segment ends (tile_incl holds the tile-local ends
+// k_scan_tiles wrote; the tile's prefix is added in place), and the scalars derived from the
+// vehicle's total.  Vehicles the filter `only` excludes are skipped.
 template <int Unused = 0>
 __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ descs, int space,
                                                     const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
@@ -756,7 +765,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
             {
                 tile_pre[tc + k] = run;
                 if (tile_incl)
-                    tile_incl[tc + k] = run + x[k];
+                {
+                    fx_t* se = tile_incl + (size_t)CDF_SEGS_PER_TILE * (tc + k);
+#pragma unroll
+                    for (int q = 0; q < CDF_SEGS_PER_TILE; ++q)
+                        se[q] += run;
+                }
             }
             run += x[k];
         }
@@ -837,7 +851,7 @@ inline void run_exact_scan(Engine* E, int space, const Src& src, const Sink& sin
         return;
     launch_begin(E, K_SCAN_TILES);
     hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, E->stream, src, E->d_desc, E->tab(space),
-                       E->scan_tile_sum);
+                       E->scan_tile_sum, tile_incl);
     launch_end(E, K_SCAN_TILES);
     launch_begin(E, K_SCAN_TOP);
     hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, space, E->scan_tile_sum,

@@ -247,7 +247,7 @@ ScratchLayout scratch_layout(const PlanSizes& ps)
     L.tab = take(sizeof(uint32_t) * (size_t)ps.tab);
     L.scan_sum = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
     L.scan_pre = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
-    L.cdf_incl = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
+    L.cdf_incl = take(sizeof(fx_t) * CDF_SEGS_PER_TILE * (size_t)ps.scan_tiles);
     L.seg_flags = take((size_t)ps.seg_tiles);
     L.seg_head = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
     L.seg_tail = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
@@ -480,12 +480,12 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         {
             if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
                 return set_err(&E->last_error, st, why);
-            long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = E->scan_tiles_cap;
+            long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = CDF_SEGS_PER_TILE * E->scan_tiles_cap;
             if ((st = grow(&E->scan_tile_sum, &c1, scan_tiles, "scan", &why)) ||
                 (st = grow(&E->scan_tile_pre, &c2, scan_tiles, "scan", &why)) ||
-                (st = grow(&E->cdf_tile_incl, &c3, scan_tiles, "cdf index", &why)))
+                (st = grow(&E->cdf_tile_incl, &c3, CDF_SEGS_PER_TILE * scan_tiles, "cdf index", &why)))
                 return set_err(&E->last_error, st, why);
-            E->scan_tiles_cap = std::min({c1, c2, c3});
+            E->scan_tiles_cap = std::min({c1, c2, c3 / CDF_SEGS_PER_TILE});
             if (seg_tiles > E->seg_tiles_cap)
             {
                 long long a = 0, b = 0, c = 0;

@@ -80,6 +80,11 @@ enum Space
     SP_COUNT
 };
 
+// The resampling CDF index has one entry per CDF segment: 256 consecutive joint weights, one
+// wave's share (64 lanes x 4) of a 1024-element scan tile.
+constexpr int CDF_SEG = 256;
+constexpr int CDF_SEGS_PER_TILE = 4;
+
 constexpr int UNIT = 256;          // elements per table entry
 constexpr int UNIT_ALIGN = 8;      // units per vehicle are padded to a multiple of this
 constexpr int TAB_VEH_SHIFT = 20;  // entry = vehicle << 20 | local unit
@@ -389,6 +394,8 @@ struct Engine
     long long scan_tiles_cap = 0;
     fx_t* scan_tile_sum = nullptr;
     fx_t* scan_tile_pre = nullptr;
+    // Resampling CDF index: exact mode, the inclusive end of every CDF segment (CDF_SEGS_PER_TILE
+    // per scan tile); float modes, two doubles per scan tile (envelope at the end and before it).
     fx_t* cdf_tile_incl = nullptr;
     long long seg_tiles_cap = 0;
     SegCarry seg{};

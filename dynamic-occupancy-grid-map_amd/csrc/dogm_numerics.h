@@ -291,6 +291,25 @@ DOGM_HD fx_t fx_from_f32(float v)
     return (fx_t)(((((ufx_t)hi << 64) | (ufx_t)lo) ^ sgn) - sgn);
 }
 
+// fx_from_f32 for v with a clear sign bit (+0, positive values, +inf; what cdf_weight returns):
+// the same value without the two's-complement step.
+DOGM_HD fx_t fx_from_f32_nonneg(float v)
+{
+    const uint32_t b = f32_bits(v);
+    const int sh = (int)((b >> 23) & 0xffu) - 150 + FX_FRAC;
+    const uint64_t m = (uint64_t)((b & 0x007fffffu) | 0x00800000u);
+    const uint64_t hi_a = m << ((sh - 64) & 63);
+    const uint64_t lo_b = m << (sh & 63);
+    const uint64_t hi_b = m >> ((64 - sh) & 63);
+    const uint64_t lo_c = m >> ((-sh) & 63);
+    uint64_t lo = sh >= 64 ? 0 : (sh >= 0 ? lo_b : lo_c);
+    uint64_t hi = sh >= 64 ? hi_a : (sh > 40 ? hi_b : 0);
+    const bool sat = sh > 103;
+    lo = sat ? 0 : lo;
+    hi = sat ? (1ull << 62) : hi;
+    return (fx_t)(((ufx_t)hi << 64) | (ufx_t)lo);
+}
+
 DOGM_HD fx_t fx_from_f64(double v)
 {
     const uint64_t b = f64_bits(v);

@@ -1,17 +1,18 @@
 // Multinomial resampling on gfx950 (dogm/src/dogm.cu:386-423, resampling.cu:17-68).
 //
 // 1. Exact CDF over [posterior weights | birth weights]: tile totals and their scan give the
-//    inclusive CDF value at the end of every 1024-entry tile (the coarse index); entries are not
-//    stored but recomputed per staged tile in the merge.
+//    inclusive CDF value at the end of every 256-entry segment (the coarse index, four per
+//    1024-entry scan tile); entries are not stored but recomputed per staged segment in the merge.
 // 2. Sorted uniforms by exponential spacings: E_k for k = 0..N, exact prefix S_k, draw k is the
 //    fraction S_k / S_N of the CDF total (spacing_draw in dogm_numerics.h).  Only the tile totals of E are
 //    stored; the merge kernel regenerates E from the counter-based RNG (four E per Philox call).
 // 3. Merge: a workgroup owns 1024 consecutive (hence sorted) draws.  Its ancestors lie between the
-//    CDF tiles holding its first and last draw; those are found by a two-level k-ary search over
+//    CDF segments holding its first and last draw; those are found by a two-level k-ary search over
 //    the coarse index (two dependent loads instead of a 22-level binary search), the CDF entries of
-//    those tiles are rebuilt in LDS from the weights (exact block scan + tile prefix), every draw
-//    finds its ancestor there, and the next particle is written directly (the reference's
-//    lower_bound and resamplingKernel gather in one pass).
+//    those segments are rebuilt in LDS from the weights (one wave per segment: exact wave scan +
+//    the index entry before the segment), every draw finds its ancestor there, and the next
+//    particle is written directly (the reference's lower_bound and resamplingKernel gather in one
+//    pass).
 // No atomics on global memory and no global sort: the result is a pure function of the inputs.
 // Every vehicle of a step resamples from its own CDF with its own draws.
 #include "dogm_device.h"
@@ -21,12 +22,18 @@ namespace dogm_hip
 
 constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = BLOCK * RS_ITEMS;  // draws per workgroup
-// CDF tiles staged in LDS per chunk: 2 for launches of up to RS_WIDE_MAX_TILES workgroups (fewer chunks
-// per workgroup shortens the critical path), 1 for larger launches (half the LDS, twice the
-// workgroups per CU).  Measured on 2e6-particle grids: two tiles win for one grid per launch (1954
-// workgroups), one tile for eight (15.6k workgroups).
-constexpr int RS_SLICE_MAX_TILES = 2;
+// CDF entries staged in LDS per window: 8 segments (2048 entries) for launches of up to
+// RS_WIDE_MAX_TILES workgroups (fewer windows per workgroup shortens the critical path), 4 segments
+// for larger launches (half the LDS, more workgroups per CU).  The float modes stage whole tiles
+// (RS_FLOAT_SLICE_TILES per window).
+constexpr int RS_WINDOW_NARROW = 4;
+constexpr int RS_WINDOW_WIDE = 8;
 constexpr int RS_WIDE_MAX_TILES = 4096;
+constexpr int RS_FLOAT_SLICE_TILES = 2;
+// Doubles per scan tile in the float modes' index (the envelope at the tile end and before it),
+// spaced like the exact mode's segment ends so that vehicles of both kinds in one step keep
+// disjoint index ranges.
+constexpr int ENV_STRIDE = 2 * CDF_SEGS_PER_TILE;
 
 static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-scan tile size");
 
@@ -179,7 +186,7 @@ __global__ void k_cdff_top(const VehDesc* __restrict__ descs, int m, const doubl
     }
 }
 
-// Values, the envelope inside the tile, and the tile maximum (into env[2 t]).
+// Values, the envelope inside the tile, and the tile maximum (into env[ENV_STRIDE t]).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_cdff_apply(const VehDesc* __restrict__ descs,
                                                       const uint32_t* __restrict__ tab,
@@ -234,7 +241,7 @@ __global__ __launch_bounds__(BLOCK) void k_cdff_apply(const VehDesc* __restrict_
         }
     }
     if (threadIdx.x == 0)
-        env[2 * (size_t)blockIdx.x] = (double)tmax;
+        env[ENV_STRIDE * (size_t)blockIdx.x] = (double)tmax;
 }
 
 // One thread per vehicle: the envelope across tiles, the total and the new particle weight.
@@ -252,10 +259,10 @@ __global__ void k_cdff_env(const VehDesc* __restrict__ descs, int m, double* __r
     double run = 0.0;
     for (int t = 0; t < nt; ++t)
     {
-        const double tm = env[2 * (size_t)(t0 + t)];
-        env[2 * (size_t)(t0 + t) + 1] = run;
+        const double tm = env[ENV_STRIDE * (size_t)(t0 + t)];
+        env[ENV_STRIDE * (size_t)(t0 + t) + 1] = run;
         run = tm > run ? tm : run;
-        env[2 * (size_t)(t0 + t)] = run;
+        env[ENV_STRIDE * (size_t)(t0 + t)] = run;
     }
     DevScalars* sc = scs + v;
     sc->ftotal = run;
@@ -264,24 +271,11 @@ __global__ void k_cdff_env(const VehDesc* __restrict__ descs, int m, double* __r
     sc->new_weight = jm / (float)d.N;
 }
 
-// Ancestor-search traits: exact 128-bit CDF or float-mode envelope (doubles).
-// The exact CDF is not stored: a chunk's entries are recomputed from the 4-byte joint weights (exact
-// block scan plus the tile's prefix from the coarse index), which is cheaper than writing and
-// re-reading 16-byte entries.
-struct CdfExactTr
-{
-    using V = fx_t;
-    static constexpr bool kRecompute = true;
-    __device__ static bool takes(const VehDesc& d) { return d.cdf_mode == CDF_EXACT; }
-    __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars*, int) { return spacing_draw(s_k, sp); }
-    __device__ static V incl(const void* ci, int t0, int t) { return static_cast<const fx_t*>(ci)[t0 + t]; }
-    __device__ static V value(const VehDesc& d, const void*, int, int j) { return d.cdf[j]; }
-};
-
+// Ancestor search over the float modes' envelope (doubles): the draw, the coarse index entry of a
+// tile and a staged CDF value.
 struct CdfFloatTr
 {
     using V = double;
-    static constexpr bool kRecompute = false;
     __device__ static bool takes(const VehDesc& d) { return d.cdf_mode != CDF_EXACT; }
     __device__ static V draw(fx_t s_k, const Spacing& sp, const DevScalars* sc, int mode)
     {
@@ -289,12 +283,12 @@ struct CdfFloatTr
     }
     __device__ static V incl(const void* ci, int t0, int t)
     {
-        return static_cast<const double*>(ci)[2 * (size_t)(t0 + t)];
+        return static_cast<const double*>(ci)[ENV_STRIDE * (size_t)(t0 + t)];
     }
     __device__ static V value(const VehDesc& d, const void* ci, int t0, int j)
     {
         const double c = reinterpret_cast<const double*>(d.cdf)[j];
-        const double b = static_cast<const double*>(ci)[2 * (size_t)(t0 + j / SCAN_TILE) + 1];
+        const double b = static_cast<const double*>(ci)[ENV_STRIDE * (size_t)(t0 + j / SCAN_TILE) + 1];
         return c > b ? c : b;
     }
 };
@@ -342,74 +336,55 @@ struct CdfStage<fx_t, LEN>
     }
 };
 
-// Index of the first CDF tile whose inclusive end is >= r (T - 1 if none): two-level k-ary search.
-template <class Tr>
-__device__ int find_cdf_tile(const void* __restrict__ ci, int t0, int T, typename Tr::V r, int* s_cnt)
+// Index of the first of n non-decreasing values value(0 .. n-1) that is >= r (n - 1 if none): a
+// two-level k-ary search by the workgroup (the last value of each of up to BLOCK groups, then the
+// values of one group).
+template <class V, class Get>
+__device__ int find_first_not_below(const Get& value, int n, V r, int* s_cnt)
 {
-    const int G = (T + BLOCK - 1) / BLOCK;  // tiles per group; at most BLOCK groups
-    const int ng = (T + G - 1) / G;
+    const int G = (n + BLOCK - 1) / BLOCK;  // values per group; at most BLOCK groups
+    const int ng = (n + G - 1) / G;
     if (threadIdx.x == 0)
         s_cnt[0] = 0;
     __syncthreads();
     const int i = threadIdx.x;
-    const bool below = i < ng && Tr::incl(ci, t0, min(T - 1, (i + 1) * G - 1)) < r;
+    const bool below = i < ng && value(min(n - 1, (i + 1) * G - 1)) < r;
     const int groups_below = __syncthreads_count(below);
     const int g = min(groups_below, ng - 1);
     for (int j = threadIdx.x; j < G; j += BLOCK)
     {
         const int t = g * G + j;
-        if (t < T && Tr::incl(ci, t0, t) < r)
+        if (t < n && value(t) < r)
             atomicAdd(&s_cnt[0], 1);
     }
     __syncthreads();
-    const int t = min(g * G + s_cnt[0], T - 1);
+    const int t = min(g * G + s_cnt[0], n - 1);
     __syncthreads();
     return t;
 }
 
-// Seven waves per SIMD for the one-tile exact form (72 registers; the few values spilled cost less
-// than the latency the extra wave hides); the two-tile form is bounded by its LDS.
-template <class Tr, int RS_SLICE_TILES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RS_SLICE_TILES == 1 ? 7 : 4)))
-void k_rs_merge(const VehDesc* __restrict__ descs,
-                                                    const uint32_t* __restrict__ tab, int pph,
-                                                    const fx_t* __restrict__ exp_tile_pre,
-                                                    const void* __restrict__ cdf_incl,
-                                                    const DevScalars* __restrict__ scs)
+// The workgroup's draws r_k, k = base + threadIdx.x + i * BLOCK (item i of a thread is strided, so
+// at every item a wave searches 64 consecutive draws, whose lower bounds are neighbouring CDF
+// entries: the probes of a wave read neighbouring LDS words, and the ancestors come out
+// wave-contiguous for the gather), and s_rb = {first draw, last draw}.  Multinomial draws: the
+// spacings' prefix runs over four consecutive draws per thread (one Philox call each); the draws
+// are then moved to their strided owners through the staging buffer.
+template <class V, class Draw, class Stage>
+__device__ __forceinline__ void merge_draws(const VehDesc& d, const DevScalars* sc, int base, int n,
+                                            const fx_t* __restrict__ exp_tile_pre, const Draw& draw, Stage& stage,
+                                            V (&r)[RS_ITEMS], V* s_rb)
 {
-    using V = typename Tr::V;
-    constexpr int RS_SLICE = RS_SLICE_TILES * SCAN_TILE;
-    __shared__ CdfStage<V, RS_SLICE> s_cdf;
-    __shared__ V s_rb[2];
-    __shared__ int s_cnt[1];
-    __shared__ int s_kmin;
-
-    const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
-    const VehDesc& d = descs[tr.v];
-    const int n = d.N;
-    const int base = tr.base;
-    if (base >= n || !Tr::takes(d))
-        return;
-    const DevScalars* sc = scs + tr.v;
-    const int J = n + d.B;
-    const Spacing sp = sc->spacing;
     const int last = min(base + RS_TILE, n) - 1;
-    // Item i of thread t is draw base + t + i * BLOCK (strided), so at every item a wave searches 64
-    // consecutive draws, whose lower bounds are neighbouring CDF entries: the probes of a wave read
-    // neighbouring LDS words instead of words a fixed stride apart (bank conflicts), and the
-    // ancestors come out wave-contiguous for the gather.
-    V r[RS_ITEMS];
+    const Spacing sp = sc->spacing;
     if (d.draw_mode == DRAW_SYSTEMATIC)
     {
         const uint32_t u = systematic_u32(d.seed, d.inst, d.ep_rs);
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
-            r[i] = Tr::draw(systematic_s(base + (int)threadIdx.x + i * BLOCK, u), sp, sc, d.cdf_mode);
+            r[i] = draw(systematic_s(base + (int)threadIdx.x + i * BLOCK, u), sp);
     }
     else
     {
-        // The spacings' prefix runs over four consecutive draws per thread (one Philox call each);
-        // the draws are then moved to their strided owners through LDS.
         const int q0 = base + threadIdx.x * RS_ITEMS;
         fx_t e[4];
         exp4(d, q0, n, e);
@@ -420,12 +395,12 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
         for (int i = 0; i < RS_ITEMS; ++i)
         {
             run += e[i];
-            s_cdf.put(threadIdx.x * RS_ITEMS + i, Tr::draw(run, sp, sc, d.cdf_mode));
+            stage.put(threadIdx.x * RS_ITEMS + i, draw(run, sp));
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
-            r[i] = s_cdf.get(threadIdx.x + i * BLOCK);
+            r[i] = stage.get(threadIdx.x + i * BLOCK);
     }
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i)
@@ -437,117 +412,42 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
             s_rb[1] = r[i];
     }
     __syncthreads();
-    const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
-    const int t0 = d.u0[SP_J] / SCAN_Q;
-    const int t_lo = find_cdf_tile<Tr>(cdf_incl, t0, T, s_rb[0], s_cnt);
-    const int t_hi = find_cdf_tile<Tr>(cdf_incl, t0, T, s_rb[1], s_cnt);
-    // Walk the CDF tiles [t_lo, t_hi] in LDS-sized chunks; a draw is resolved in the first chunk
-    // whose last entry is >= its value (the CDF is non-decreasing, so its lower bound is there).
-    // A thread's later items are later draws, so each search starts at the previous item's lower
-    // bound.
-    int anc[RS_ITEMS];
+}
+
+// The smallest draw of the workgroup that has no ancestor yet (n if none), and its value in s_rb[0].
+template <class V>
+__device__ __forceinline__ int merge_next_draw(int base, int n, const int (&anc)[RS_ITEMS], const V (&r)[RS_ITEMS],
+                                               int* s_kmin, V* s_rb)
+{
+    if (threadIdx.x == 0)
+        *s_kmin = n;
+    __syncthreads();
+    int mine = n;
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i)
-        anc[i] = -1;
-    for (int tc = t_lo; tc <= t_hi;)
+    for (int i = RS_ITEMS - 1; i >= 0; --i)
+        if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n)
+            mine = base + (int)threadIdx.x + i * BLOCK;
+    if (mine < n)
+        atomicMin(s_kmin, mine);
+    __syncthreads();
+    const int kmin = *s_kmin;
+    if (kmin < n)
     {
-        const int tc_end = min(tc + RS_SLICE_TILES, t_hi + 1) - 1;
-        const int c0 = tc * SCAN_TILE;
-        const int clen = min(J, (tc_end + 1) * SCAN_TILE) - c0;
-        __syncthreads();
-        if constexpr (Tr::kRecompute)
-        {
-            for (int tt = tc; tt <= tc_end; ++tt)
-            {
-                const int q0 = tt * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-                const int cnt = max(0, min(SCAN_ITEMS, J - q0));
-                float f[SCAN_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (cnt > 0)
-                    SrcJointPos{}.load4(d, q0, cnt, f);
-                fx_t v[SCAN_ITEMS];
-                fx_t sum = 0;
-#pragma unroll
-                for (int i = 0; i < SCAN_ITEMS; ++i)
-                {
-                    v[i] = i < cnt ? fx_from_f32(f[i]) : (fx_t)0;
-                    sum += v[i];
-                }
-                fx_t ttot;
-                fx_t run = block_exclusive_scan_fx(sum, &ttot) + (tt > 0 ? Tr::incl(cdf_incl, t0, tt - 1) : (fx_t)0);
-                const int at = (tt - tc) * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-#pragma unroll
-                for (int i = 0; i < SCAN_ITEMS; ++i)
-                {
-                    run += v[i];
-                    if (i < cnt)
-                        s_cdf.put(at + i, run);
-                }
-            }
-        }
-        else
-        {
-            for (int i = threadIdx.x; i < clen; i += BLOCK)
-                s_cdf.put(i, Tr::value(d, cdf_incl, t0, c0 + i));
-        }
-        __syncthreads();
-        const V cmax = s_cdf.get(clen - 1);
-        int pos = 0;  // lower bound of the thread's previous draw inside this chunk
-#pragma unroll
-        for (int i = 0; i < RS_ITEMS; ++i)
-        {
-            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n && !(cmax < r[i]))
-            {
-                // Lower bound of r[i] in [lo, hi]; s_cdf[hi] >= r[i] holds throughout (cmax >= r[i]).
-                int lo = pos;
-                int hi = clen - 1;
-                int count = hi - lo;
-                while (count > 0)
-                {
-                    const int step = count >> 1;
-                    if (s_cdf.less(lo + step, r[i]))
-                    {
-                        lo += step + 1;
-                        count -= step + 1;
-                    }
-                    else
-                    {
-                        count = step;
-                    }
-                }
-                pos = lo;
-                anc[i] = c0 + lo;
-            }
-        }
-        // Next chunk: the one holding the smallest unresolved draw.  Tiles no draw lands in (long
-        // runs of zero weights) are skipped with one coarse-index search instead of being staged.
-        if (tc_end >= t_hi)
-            break;
-        if (threadIdx.x == 0)
-            s_kmin = n;
-        __syncthreads();
-        int mine = n;
-#pragma unroll
-        for (int i = RS_ITEMS - 1; i >= 0; --i)
-            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n)
-                mine = base + (int)threadIdx.x + i * BLOCK;
-        if (mine < n)
-            atomicMin(&s_kmin, mine);
-        __syncthreads();
-        const int kmin = s_kmin;
-        if (kmin >= n)
-            break;
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; ++i)
             if (base + (int)threadIdx.x + i * BLOCK == kmin)
                 s_rb[0] = r[i];
-        __syncthreads();
-        const V rmin = s_rb[0];
-        const int next_end = min(tc_end + RS_SLICE_TILES, t_hi);
-        if (!(Tr::incl(cdf_incl, t0, next_end) < rmin))
-            tc = tc_end + 1;
-        else
-            tc = max(tc_end + 1, find_cdf_tile<Tr>(cdf_incl, t0, T, rmin, s_cnt));
     }
+    __syncthreads();
+    return kmin;
+}
+
+// The resampled particles k = base + threadIdx.x + i * BLOCK: ancestor index, state copied from the
+// ancestor (posterior particle or new-born one) and the uniform new weight.  A draw without an
+// ancestor (above the CDF total by rounding) takes the last joint entry.
+__device__ __forceinline__ void merge_write(const VehDesc& d, const DevScalars* sc, int pph, int base, int n, int J,
+                                            const int (&anc)[RS_ITEMS])
+{
     const float nw = sc->new_weight;
     const DevParticles P = d.part[pph];
     const DevParticles Bp = d.birth;
@@ -574,6 +474,192 @@ void k_rs_merge(const VehDesc* __restrict__ descs,
     }
 }
 
+// Exact-CDF merge.  The workgroup's ancestors lie in the CDF segments [s_lo, s_hi] holding its first
+// and last draw; they are staged WSEG segments at a time.  Each wave rebuilds whole segments: the 64
+// lanes' four joint weights in fixed point, a 64-lane exact scan, and the index entry before the
+// segment as its start (integer adds, so the entries equal those of any other association).  A
+// draw is resolved in the first window whose last entry is >= its value (the CDF is non-decreasing,
+// so its lower bound is there), by a branch-free binary search over the window padded to a power
+// of two; windows no remaining draw lands in are skipped with one coarse-index search.
+// Seven waves per SIMD for the 4-segment window; the 8-segment form is bounded by its LDS.
+template <int WSEG>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WSEG <= RS_WINDOW_NARROW ? 7 : 4)))
+void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab, int pph,
+                      const fx_t* __restrict__ exp_tile_pre, const fx_t* __restrict__ seg_incl,
+                      const DevScalars* __restrict__ scs)
+{
+    constexpr int WLEN = WSEG * CDF_SEG;
+    static_assert(WLEN >= RS_TILE, "the window also stages the workgroup's draws");
+    __shared__ CdfStage<fx_t, WLEN> s_cdf;
+    __shared__ fx_t s_rb[2];
+    __shared__ int s_cnt[1];
+    __shared__ int s_kmin;
+
+    const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int n = d.N;
+    const int base = tr.base;
+    if (base >= n || d.cdf_mode != CDF_EXACT)
+        return;
+    const DevScalars* sc = scs + tr.v;
+    const int J = n + d.B;
+    fx_t r[RS_ITEMS];
+    merge_draws<fx_t>(d, sc, base, n, exp_tile_pre, [](fx_t s_k, const Spacing& sp) { return spacing_draw(s_k, sp); },
+                      s_cdf, r, s_rb);
+    const int S = (J + CDF_SEG - 1) / CDF_SEG;  // segments holding entries
+    const fx_t* si = seg_incl + (size_t)(d.u0[SP_J] / SCAN_Q) * CDF_SEGS_PER_TILE;
+    const auto seg_end = [si](int s) { return si[s]; };
+    const int s_lo = find_first_not_below<fx_t>(seg_end, S, s_rb[0], s_cnt);
+    const int s_hi = find_first_not_below<fx_t>(seg_end, S, s_rb[1], s_cnt);
+    const int wid = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int anc[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+        anc[i] = -1;
+    for (int s0 = s_lo; s0 <= s_hi;)
+    {
+        const int s1 = min(s0 + WSEG, s_hi + 1);  // window: segments [s0, s1)
+        const int c0 = s0 * CDF_SEG;
+        const int clen = min(J, s1 * CDF_SEG) - c0;
+        __syncthreads();
+        for (int s = s0 + wid; s < s1; s += BLOCK / WAVE)
+        {
+            const int q0 = s * CDF_SEG + lane * SCAN_ITEMS;
+            const int cnt = max(0, min(SCAN_ITEMS, J - q0));
+            float f[SCAN_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (cnt > 0)
+                SrcJointPos{}.load4(d, q0, cnt, f);
+            fx_t v[SCAN_ITEMS];
+            fx_t sum = 0;
+#pragma unroll
+            for (int i = 0; i < SCAN_ITEMS; ++i)
+            {
+                v[i] = i < cnt ? fx_from_f32_nonneg(f[i]) : (fx_t)0;  // cdf_weight: sign bit clear
+                sum += v[i];
+            }
+            fx_t run = wave_inclusive_scan_fx(sum) - sum + (s > 0 ? si[s - 1] : (fx_t)0);
+            const int at = (s - s0) * CDF_SEG + lane * SCAN_ITEMS;
+#pragma unroll
+            for (int i = 0; i < SCAN_ITEMS; ++i)
+            {
+                run += v[i];
+                if (i < cnt)
+                    s_cdf.put(at + i, run);
+            }
+        }
+        __syncthreads();
+        const fx_t cmax = s_cdf.get(clen - 1);
+        const int span = clen > 1 ? 1 << (32 - __clz(clen - 1)) : 1;  // power of two >= clen, <= WLEN
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+        {
+            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n && !(cmax < r[i]))
+            {
+                // lo counts the entries < r[i]: after the probe at lo + step - 1 the count lies in
+                // [lo, lo + step); entries at clen and beyond count as +infinity.
+                int lo = 0;
+                for (int step = span >> 1; step > 0; step >>= 1)
+                {
+                    const int j = lo + step - 1;
+                    if (j < clen && s_cdf.less(j, r[i]))
+                        lo += step;
+                }
+                anc[i] = c0 + lo;
+            }
+        }
+        if (s1 > s_hi)
+            break;
+        const int kmin = merge_next_draw<fx_t>(base, n, anc, r, &s_kmin, s_rb);
+        if (kmin >= n)
+            break;
+        const fx_t rmin = s_rb[0];
+        const int next_last = min(s1 - 1 + WSEG, s_hi);
+        if (!(si[next_last] < rmin))
+            s0 = s1;
+        else
+            s0 = max(s1, find_first_not_below<fx_t>(seg_end, S, rmin, s_cnt));
+    }
+    merge_write(d, sc, pph, base, n, J, anc);
+}
+
+// Float-mode merge (precision study): the envelope is stored, so a window stages
+// RS_FLOAT_SLICE_TILES tiles of it; the search and the window walk are those of the exact merge
+// at tile granularity.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_rs_merge_float(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab, int pph,
+                      const fx_t* __restrict__ exp_tile_pre, const void* __restrict__ cdf_incl,
+                      const DevScalars* __restrict__ scs)
+{
+    using Tr = CdfFloatTr;
+    constexpr int SLICE = RS_FLOAT_SLICE_TILES * SCAN_TILE;
+    __shared__ CdfStage<double, SLICE> s_cdf;
+    __shared__ double s_rb[2];
+    __shared__ int s_cnt[1];
+    __shared__ int s_kmin;
+
+    const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int n = d.N;
+    const int base = tr.base;
+    if (base >= n || !Tr::takes(d))
+        return;
+    const DevScalars* sc = scs + tr.v;
+    const int J = n + d.B;
+    double r[RS_ITEMS];
+    merge_draws<double>(
+        d, sc, base, n, exp_tile_pre,
+        [sc, &d](fx_t s_k, const Spacing& sp) { return Tr::draw(s_k, sp, sc, d.cdf_mode); }, s_cdf, r, s_rb);
+    const int T = (J + SCAN_TILE - 1) / SCAN_TILE;
+    const int t0 = d.u0[SP_J] / SCAN_Q;
+    const auto tile_end = [cdf_incl, t0](int t) { return Tr::incl(cdf_incl, t0, t); };
+    const int t_lo = find_first_not_below<double>(tile_end, T, s_rb[0], s_cnt);
+    const int t_hi = find_first_not_below<double>(tile_end, T, s_rb[1], s_cnt);
+    int anc[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i)
+        anc[i] = -1;
+    for (int tc = t_lo; tc <= t_hi;)
+    {
+        const int tc_end = min(tc + RS_FLOAT_SLICE_TILES, t_hi + 1) - 1;
+        const int c0 = tc * SCAN_TILE;
+        const int clen = min(J, (tc_end + 1) * SCAN_TILE) - c0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < clen; i += BLOCK)
+            s_cdf.put(i, Tr::value(d, cdf_incl, t0, c0 + i));
+        __syncthreads();
+        const double cmax = s_cdf.get(clen - 1);
+        const int span = clen > 1 ? 1 << (32 - __clz(clen - 1)) : 1;
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; ++i)
+        {
+            if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n && !(cmax < r[i]))
+            {
+                int lo = 0;
+                for (int step = span >> 1; step > 0; step >>= 1)
+                {
+                    const int j = lo + step - 1;
+                    if (j < clen && s_cdf.less(j, r[i]))
+                        lo += step;
+                }
+                anc[i] = c0 + lo;
+            }
+        }
+        if (tc_end >= t_hi)
+            break;
+        const int kmin = merge_next_draw<double>(base, n, anc, r, &s_kmin, s_rb);
+        if (kmin >= n)
+            break;
+        const double rmin = s_rb[0];
+        const int next_end = min(tc_end + RS_FLOAT_SLICE_TILES, t_hi);
+        if (!(tile_end(next_end) < rmin))
+            tc = tc_end + 1;
+        else
+            tc = max(tc_end + 1, find_first_not_below<double>(tile_end, T, rmin, s_cnt));
+    }
+    merge_write(d, sc, pph, base, n, J, anc);
+}
+
 void run_resampling(Engine* E, int pph, int modes)
 {
     const bool exact = modes & (1 << CDF_EXACT);
@@ -583,7 +669,7 @@ void run_resampling(Engine* E, int pph, int modes)
         // Tile totals and the coarse index (inclusive tile ends) only; the merge recomputes entries.
         launch_begin(E, K_SCAN_TILES);
         hipLaunchKernelGGL(k_scan_tiles<SrcJointPos>, dim3(jtiles), dim3(BLOCK), 0, E->stream, SrcJointPos{},
-                           E->d_desc, E->tab(SP_J), E->scan_tile_sum);
+                           E->d_desc, E->tab(SP_J), E->scan_tile_sum, E->cdf_tile_incl);
         launch_end(E, K_SCAN_TILES);
         launch_begin(E, K_SCAN_TOP);
         hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_J,
@@ -616,7 +702,7 @@ void run_resampling(Engine* E, int pph, int modes)
     const int etiles = E->tiles(SP_E, SCAN_Q);
     launch_begin(E, K_EXP_TILES);
     hipLaunchKernelGGL(k_scan_tiles<SrcExp>, dim3(etiles), dim3(BLOCK), 0, E->stream, SrcExp{}, E->d_desc,
-                       E->tab(SP_E), E->scan_tile_sum);
+                       E->tab(SP_E), E->scan_tile_sum, (fx_t*)nullptr);
     launch_end(E, K_EXP_TILES);
     launch_begin(E, K_SCAN_TOP);
     hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_E, E->scan_tile_sum,
@@ -629,16 +715,15 @@ void run_resampling(Engine* E, int pph, int modes)
     if (exact)
     {
         if (wide)
-            hipLaunchKernelGGL((k_rs_merge<CdfExactTr, RS_SLICE_MAX_TILES>), dim3(mt), dim3(BLOCK), 0, E->stream,
-                               E->d_desc, E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl,
-                               E->scalars);
+            hipLaunchKernelGGL(k_rs_merge_exact<RS_WINDOW_WIDE>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc,
+                               E->tab(SP_P), pph, E->scan_tile_pre, E->cdf_tile_incl, E->scalars);
         else
-            hipLaunchKernelGGL((k_rs_merge<CdfExactTr, 1>), dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc,
-                               E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+            hipLaunchKernelGGL(k_rs_merge_exact<RS_WINDOW_NARROW>, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc,
+                               E->tab(SP_P), pph, E->scan_tile_pre, E->cdf_tile_incl, E->scalars);
     }
     if (modes & ~(1 << CDF_EXACT))
-        hipLaunchKernelGGL((k_rs_merge<CdfFloatTr, RS_SLICE_MAX_TILES>), dim3(mt), dim3(BLOCK), 0, E->stream,
-                           E->d_desc, E->tab(SP_P), pph, E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
+        hipLaunchKernelGGL(k_rs_merge_float, dim3(mt), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
+                           E->scan_tile_pre, (const void*)E->cdf_tile_incl, E->scalars);
     launch_end(E, K_RS_MERGE);
 }
 

@@ -58,6 +58,7 @@ def lib():
             "oracle_uniform01": (f, [C.c_uint32]),
             "oracle_uniform01_oc": (f, [C.c_uint32]),
             "oracle_fx_from_f32": (None, [f, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+            "oracle_fx_from_f32_nonneg_mismatches": (C.c_int, [C.c_void_p, C.c_int]),
             "oracle_fx_from_f64": (None, [C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
             "oracle_fx_to_f32": (f, [C.c_int64, C.c_uint64]),
             "oracle_fx_to_f64": (C.c_double, [C.c_int64, C.c_uint64]),
@@ -320,6 +321,14 @@ def fx_from_f32(v: float) -> int:
     hi, lo = C.c_int64(), C.c_uint64()
     lib().oracle_fx_from_f32(v, C.byref(hi), C.byref(lo))
     return _to_int(hi, lo)
+
+
+def fx_from_f32_nonneg_mismatches(bits) -> int:
+    """Count of the float bit patterns in `bits` (uint32 array) where the kernels' sign-free
+    conversion differs from fx_from_f32."""
+    import numpy as np
+    b = np.ascontiguousarray(bits, dtype=np.uint32)
+    return lib().oracle_fx_from_f32_nonneg_mismatches(b.ctypes.data, len(b))
 
 
 def fx_from_f64(v: float) -> int:

@@ -1383,6 +1383,19 @@ void oracle_fx_from_f32(float v, int64_t* hi, uint64_t* lo)
     *hi = (int64_t)(r >> 64);
     *lo = (uint64_t)r;
 }
+// All n floats of bit patterns bits[0..n) through fx_from_f32_nonneg (the kernels' form for
+// weights with a clear sign bit); returns how many differ from fx_from_f32.
+int oracle_fx_from_f32_nonneg_mismatches(const uint32_t* bits, int n)
+{
+    int bad = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        float v;
+        std::memcpy(&v, &bits[i], sizeof v);
+        bad += fx_from_f32_nonneg(v) != fx_from_f32(v);
+    }
+    return bad;
+}
 void oracle_fx_from_f64(double v, int64_t* hi, uint64_t* lo)
 {
     const fx_t r = fx_from_f64(v);

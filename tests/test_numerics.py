@@ -143,6 +143,15 @@ def _fx_exact(v: float) -> int:
     return -q if fr < 0 else q
 
 
+def test_fx_from_f32_nonneg_equals_general_form():
+    """Every float with a clear sign bit (all 2^31 patterns: zero, subnormals, normals, +inf, NaNs
+    with a clear sign) converts identically through the sign-free form the merge kernel uses."""
+    step = 1 << 24
+    for start in range(0, 1 << 31, step):
+        bits = np.arange(start, start + step, dtype=np.uint32)
+        assert oc.fx_from_f32_nonneg_mismatches(bits) == 0, hex(start)
+
+
 def test_fx_from_f32_exact():
     rng = np.random.default_rng(4)
     vals = [0.0, -0.0, 1.0, -1.0, 0.5, 1e-30, 2.0**-88, 2.0**-89, 3 * 2.0**-90, 2.0**38, -(2.0**38) * 1.5,
