@@ -474,13 +474,40 @@ __device__ __forceinline__ void merge_write(const VehDesc& d, const DevScalars* 
     }
 }
 
+// Index of the first of n non-decreasing values value(0 .. n-1) that is >= r (n - 1 if none), for
+// two values at once, found by one wave without a barrier: the lanes probe every s-th value (s =
+// ceil(n / 64)); the probes below r form a prefix of k lanes, so the answer lies in [k s, (k + 1) s),
+// whose values below r are then counted (ceil(s / 64) probes per lane).
+template <class V, class Get>
+__device__ __forceinline__ void wave_find2(const Get& value, int n, V r0, V r1, int* c0, int* c1)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int s = (n + WAVE - 1) / WAVE;
+    const V p = value(min(n - 1, (lane + 1) * s - 1));
+    const int b0 = __popcll(__ballot(p < r0)) * s;
+    const int b1 = __popcll(__ballot(p < r1)) * s;
+    int n0 = 0, n1 = 0;
+    for (int q0 = 0; q0 < s; q0 += WAVE)
+    {
+        const int q = q0 + lane;
+        const bool in0 = q < s && b0 + q < n, in1 = q < s && b1 + q < n;
+        const V v0 = in0 ? value(b0 + q) : V(0), v1 = in1 ? value(b1 + q) : V(0);
+        n0 += __popcll(__ballot(in0 && v0 < r0));
+        n1 += __popcll(__ballot(in1 && v1 < r1));
+    }
+    *c0 = min(b0 + n0, n - 1);
+    *c1 = min(b1 + n1, n - 1);
+}
+
 // Exact-CDF merge.  The workgroup's ancestors lie in the CDF segments [s_lo, s_hi] holding its first
 // and last draw; they are staged WSEG segments at a time.  Each wave rebuilds whole segments: the 64
 // lanes' four joint weights in fixed point, a 64-lane exact scan, and the index entry before the
 // segment as its start (integer adds, so the entries equal those of any other association).  A
 // draw is resolved in the first window whose last entry is >= its value (the CDF is non-decreasing,
 // so its lower bound is there), by a branch-free binary search over the window padded to a power
-// of two; windows no remaining draw lands in are skipped with one coarse-index search.
+// of two; windows no remaining draw lands in are skipped with one coarse-index search.  Every wave
+// runs the coarse-index searches itself (the same answer in each), so the only barriers are those of
+// the draws' prefix and two per window plus one between windows.
 // Seven waves per SIMD for the 4-segment window; the 8-segment form is bounded by its LDS.
 template <int WSEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WSEG <= RS_WINDOW_NARROW ? 7 : 4)))
@@ -489,11 +516,12 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
                       const DevScalars* __restrict__ scs)
 {
     constexpr int WLEN = WSEG * CDF_SEG;
+    constexpr int NWAVE = BLOCK / WAVE;
     static_assert(WLEN >= RS_TILE, "the window also stages the workgroup's draws");
     __shared__ CdfStage<fx_t, WLEN> s_cdf;
     __shared__ fx_t s_rb[2];
-    __shared__ int s_cnt[1];
-    __shared__ int s_kmin;
+    __shared__ int s_wmin[NWAVE];
+    __shared__ fx_t s_wr[NWAVE];
 
     const TileRef tr = tile_ref(tab, RS_TILE / UNIT, blockIdx.x);
     const VehDesc& d = descs[tr.v];
@@ -509,8 +537,8 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
     const int S = (J + CDF_SEG - 1) / CDF_SEG;  // segments holding entries
     const fx_t* si = seg_incl + (size_t)(d.u0[SP_J] / SCAN_Q) * CDF_SEGS_PER_TILE;
     const auto seg_end = [si](int s) { return si[s]; };
-    const int s_lo = find_first_not_below<fx_t>(seg_end, S, s_rb[0], s_cnt);
-    const int s_hi = find_first_not_below<fx_t>(seg_end, S, s_rb[1], s_cnt);
+    int s_lo, s_hi;
+    wave_find2<fx_t>(seg_end, S, s_rb[0], s_rb[1], &s_lo, &s_hi);
     const int wid = threadIdx.x / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
     int anc[RS_ITEMS];
@@ -522,8 +550,8 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
         const int s1 = min(s0 + WSEG, s_hi + 1);  // window: segments [s0, s1)
         const int c0 = s0 * CDF_SEG;
         const int clen = min(J, s1 * CDF_SEG) - c0;
-        __syncthreads();
-        for (int s = s0 + wid; s < s1; s += BLOCK / WAVE)
+        __syncthreads();  // the previous window's searches (and the draws' staging) are done
+        for (int s = s0 + wid; s < s1; s += NWAVE)
         {
             const int q0 = s * CDF_SEG + lane * SCAN_ITEMS;
             const int cnt = max(0, min(SCAN_ITEMS, J - q0));
@@ -570,15 +598,57 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
         }
         if (s1 > s_hi)
             break;
-        const int kmin = merge_next_draw<fx_t>(base, n, anc, r, &s_kmin, s_rb);
+        // The smallest unresolved draw: each wave's minimum (draw indices are distinct, so one lane
+        // holds it) with its value, then the minimum over the waves.
+        int mine = n;
+        fx_t rmine = 0;
+#pragma unroll
+        for (int i = RS_ITEMS - 1; i >= 0; --i)
+        {
+            const int k = base + (int)threadIdx.x + i * BLOCK;
+            if (anc[i] < 0 && k < n)
+            {
+                mine = k;
+                rmine = r[i];
+            }
+        }
+        int wmin = mine;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1)
+            wmin = min(wmin, __shfl_xor(wmin, o, WAVE));
+        if (wmin < n && mine == wmin)
+        {
+            s_wmin[wid] = wmin;
+            s_wr[wid] = rmine;
+        }
+        if (wmin >= n && lane == 0)
+            s_wmin[wid] = n;
+        __syncthreads();
+        int kmin = n, kw = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w)
+        {
+            const int x = s_wmin[w];
+            if (x < kmin)
+            {
+                kmin = x;
+                kw = w;
+            }
+        }
         if (kmin >= n)
             break;
-        const fx_t rmin = s_rb[0];
+        const fx_t rmin = s_wr[kw];
         const int next_last = min(s1 - 1 + WSEG, s_hi);
         if (!(si[next_last] < rmin))
+        {
             s0 = s1;
+        }
         else
-            s0 = max(s1, find_first_not_below<fx_t>(seg_end, S, rmin, s_cnt));
+        {
+            int c, unused;
+            wave_find2<fx_t>(seg_end, S, rmin, rmin, &c, &unused);
+            s0 = max(s1, c);
+        }
     }
     merge_write(d, sc, pph, base, n, J, anc);
 }
