@@ -188,6 +188,8 @@ int dogm_hip::write_particles(Engine* E, Vehicle* v, const DevParticles& p, int 
         if ((st = check_launch(E, "unpacking particle state")))
             return st;
     }
+    if (weight && p.weight == v->part[0].weight)
+        v->w_uniform = false;  // the next prediction reads every weight
     if ((idx && (st = copy_in(E, p.idx, idx, sizeof(int) * (size_t)n))) ||
         (weight && (st = copy_in(E, p.weight, weight, sizeof(float) * (size_t)n))) ||
         (assoc && (st = copy_in(E, p.assoc, assoc, (size_t)n))))

@@ -140,6 +140,19 @@ __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
     return carry + incl - v;
 }
 
+// Predicted weight of a particle (dogm/src/kernel/predict.cu:34-43): persistence probability times
+// the prior weight, 0 once the predicted position o leaves the grid.  predict and the gathering sort
+// pass (uniform weights) both form it here, so the two give the same bits.
+__device__ __forceinline__ float predicted_weight(const VehDesc& d, float w_in, const float4& o)
+{
+    const float zero = 0.0f;
+    const float top = (float)(d.gs - 1);
+    float w = d.ps * w_in;
+    if ((o.x > top || o.x < zero) || (o.y > top || o.y < zero))
+        w = zero;
+    return w;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Vector access to a thread's ITEMS consecutive elements of 32-bit values: 16-byte (ITEMS a
 // multiple of 4) or 8-byte (ITEMS even) accesses when the run is complete; runs start at multiples
@@ -662,6 +675,7 @@ constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 constexpr int SCAN_Q = SCAN_TILE / UNIT;  // units per scan tile
 
 // Src interface:
+//   static constexpr bool kNonNeg;                                    // values have a clear sign bit
 //   __device__ bool active(const VehDesc& d) const;                   // vehicle takes part
 //   __device__ int len(const VehDesc& d) const;                       // elements of the vehicle
 //   __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const;
@@ -689,7 +703,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
         if (i < cnt)
-            s += fx_from_f32(v[i]);
+            s += Src::kNonNeg ? fx_from_f32_nonneg(v[i]) : fx_from_f32(v[i]);
     fx_t total;
     const fx_t ex = block_exclusive_scan_fx(s, &total);
     if (threadIdx.x == 0)
@@ -826,7 +840,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, const 
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
     {
-        v[i] = i < cnt ? fx_from_f32(f[i]) : (fx_t)0;
+        v[i] = i < cnt ? (Src::kNonNeg ? fx_from_f32_nonneg(f[i]) : fx_from_f32(f[i])) : (fx_t)0;
         s += v[i];
     }
     fx_t total;

@@ -529,6 +529,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.inst = v->instance;
         uint32_t e = v->epoch;
         d.init = (stages & ST_INIT) && v->init_pending ? 1 : 0;
+        d.wuni = v->w_uniform || d.init ? 1 : 0;  // initialisation sets every weight to 1 / N
         d.ep_init = e;
         e += d.init ? 1u : 0u;
         d.ep_pred = e;
@@ -704,7 +705,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (!replay)
         {
             StageRange r("particleAssignment");
-            run_assignment(E, pph, hist, assign_occ);
+            run_assignment(E, pph, hist, assign_occ, !occ_pers);
         }
         pph ^= 1;
     }
@@ -780,6 +781,14 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         v->epoch = e;
         if (d.init)
             v->init_pending = false;
+        // Resampling leaves every weight at the new weight; prediction, the sort and the persistent
+        // update leave them differing.
+        if (stages & ST_RESAMPLE)
+            v->w_uniform = true;
+        else if (stages & (ST_PREDICT | ST_ASSIGN | ST_PERS))
+            v->w_uniform = false;
+        else
+            v->w_uniform = d.wuni != 0;
         if (pph)
             std::swap(v->part[0], v->part[1]);
         if (stages & ST_OCC)

@@ -113,6 +113,7 @@ struct VehDesc
     float alpha;                                 // freespace_discount^dt (host std::pow)
     float dt;
     int init;                // first measurement: particles are initialised in this step
+    int wuni;                // every weight predict reads equals weight[0] (resampled or initialised)
     int move;                // pending particle shift (fused into predict)
     float mx, my;
     int shifted;             // pending grid shift (fused into the occupancy update)
@@ -248,6 +249,7 @@ struct Vehicle
     uint32_t epoch = 0;  // Philox epoch: incremented by every random-consuming stage
     uint64_t uid = 0;    // unique per allocation (work tables are reused while the list is unchanged)
     bool init_pending = false;  // initialise particles in the next step that includes ST_INIT
+    bool w_uniform = false;     // every weight of part[0] equals part[0].weight[0]
     int cdf_mode = 0;           // resampling CDF arithmetic (dogm_num::CdfMode)
     int draw_mode = 0;          // resampling draws (dogm_num::DrawMode)
     float dt = 0.0f;            // time step of the next step
@@ -552,18 +554,26 @@ void run_build_tables(Engine* E);
 // the moments' load.  Each stage run alone keeps the reference's separate passes.
 void run_init_particles(Engine* E);                // pph 0
 // aos: the state goes to the CDF buffer as 16-byte records for the gathering sort pass of the same
-// step (run_sort with hist_done and gather), which reads it there.
+// step (run_sort with hist_done and gather), which reads it there; for vehicles whose weights are
+// uniform (VehDesc::wuni) predict then leaves the weights in place and the gathering pass forms
+// each predicted weight from weight[0] and the predicted position, with predict's arithmetic.
 void run_predict(Engine* E, int pph, bool hist, bool aos);
 // defer_bounds: the last sort pass writes the particles directly and the occupancy reduction records
-// the cell bounds (bounds = true there).
-void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds);  // reads pph, writes pph ^ 1
+// the cell bounds (bounds = true there; the reduction then reads the sorted particles' weights).
+// write_wa: that pass also writes the weight copy into weight_array (not needed when the persistent
+// update of the same step overwrites weight_array).
+void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds, bool write_wa);  // reads pph, writes pph ^ 1
 void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds);  // free[0] -> free[1]
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
 void run_births(Engine* E);
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
 // modes: bit m set when a vehicle of the step resamples with CDF mode m (dogm_num::CdfMode).
 void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 1
-void run_sort(Engine* E, int pph, bool hist_done, bool gather);  // gather: last pass writes the particles
+// Flags of the gathering sort pass.
+constexpr int GATHER_AOS = 1;  // the state is in the CDF buffer as 16-byte records (predict, same step)
+constexpr int GATHER_WA = 2;   // also write the weight copy into weight_array
+// gather: the last pass writes the particles (and, with write_wa, the weight copy into weight_array).
+void run_sort(Engine* E, int pph, bool hist_done, bool gather, bool write_wa);
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
 void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t);

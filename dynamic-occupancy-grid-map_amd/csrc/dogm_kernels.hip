@@ -262,7 +262,8 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
         {
             const float2 ps = P.pos[i], vs = P.vel[i];
             float4 s = make_float4(ps.x, ps.y, vs.x, vs.y);
-            const float w_in = P.weight[i];
+            const bool keep_w = aos && d.wuni;  // the gathering pass forms the weight
+            const float w_in = keep_w ? 0.0f : P.weight[i];
             if (d.move)
             {
                 s.x = s.x - d.mx;
@@ -289,9 +290,7 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             o.y = sy + npy;
             o.z = svx + nvx;
             o.w = svy + nvy;
-            float w = d.ps * w_in;
-            if ((o.x > (float)(gs - 1) || o.x < zero) || (o.y > (float)(gs - 1) || o.y < zero))
-                w = zero;
+            const float w = predicted_weight(d, w_in, o);
             cell = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
             if (aos)
             {
@@ -302,7 +301,8 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
                 P.pos[i] = make_float2(o.x, o.y);
                 P.vel[i] = make_float2(o.z, o.w);
             }
-            P.weight[i] = w;
+            if (!keep_w)
+                P.weight[i] = w;
             P.idx[i] = cell;
         }
         for (int ps = 0; ps < passes; ++ps)
@@ -405,12 +405,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
         d.end[key] = k;
 }
 
-void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds)
+void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds, bool write_wa)
 {
     launch_begin(E, K_RESET_BOUNDS);
     hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 4)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
     launch_end(E, K_RESET_BOUNDS);
-    run_sort(E, pph, hist_done, defer_bounds);
+    run_sort(E, pph, hist_done, defer_bounds, write_wa || !defer_bounds);
     if (defer_bounds)
         return;  // the last sort pass wrote the particles; the occupancy reduction records the bounds
     launch_begin(E, K_SORT_GATHER);
@@ -445,8 +445,10 @@ struct OccOp
     }
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
+        // After the gathering sort pass of the same step the particles' weights equal weight_array
+        // (which that pass may leave unwritten); otherwise weight_array is what the step starts from.
         float w[ITEMS];
-        load_run<ITEMS>(d.weight_array, p0, cnt, w);
+        load_run<ITEMS>(bounds ? d.part[pph].weight : d.weight_array, p0, cnt, w);
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
             v[i][0] = w[i];
@@ -703,6 +705,7 @@ void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, b
 // Born masses per cell (first-frame placement when init_only, births otherwise).
 struct SrcBorn
 {
+    static constexpr bool kNonNeg = false;
     int init_only;
     __device__ bool active(const VehDesc& d) const { return !init_only || d.init; }
     __device__ int len(const VehDesc& d) const { return d.C; }
@@ -935,9 +938,9 @@ static_assert(MOM_CHUNK == UNIT, "a moments chunk is one work-table unit");
 constexpr uint8_t MC_CONT = 1;    // the chunk's last cell continues into the next chunk
 constexpr uint8_t MC_NOHEAD = 2;  // no cell starts in the chunk
 
-__device__ __forceinline__ void mom_finalize(const VehDesc& d, int key, const float (&s)[MOM_CH])
+// rho_p = the cell's persistent occupancy mass (d.f[GF_PERS][key]).
+__device__ __forceinline__ void mom_finalize(const VehDesc& d, int key, float rho_p, const float (&s)[MOM_CH])
 {
-    const float rho_p = d.f[GF_PERS][key];
     float mx = 0.0f, my = 0.0f, vx = 0.0f, vy = 0.0f, cv = 0.0f;
     if (rho_p > 0.0f)
     {
@@ -964,7 +967,7 @@ __device__ __forceinline__ int dpp_i(int x)
 // carry.flags[unit], and per unit MOM_CH doubles in carry.head_partial (piece of the chunk's first
 // cell when that cell started in an earlier chunk) and carry.tail_partial (piece of its last cell).
 template <int WAVES_PER_WG>
-__global__ __launch_bounds__(WAVES_PER_WG* WAVE) void k_moments(const VehDesc* __restrict__ descs,
+__global__ __launch_bounds__(WAVES_PER_WG* WAVE) __attribute__((amdgpu_waves_per_eu(7))) void k_moments(const VehDesc* __restrict__ descs,
                                                                 const uint32_t* __restrict__ tab, int pph, int post,
                                                                 SegCarry carry)
 {
@@ -1004,6 +1007,12 @@ __global__ __launch_bounds__(WAVES_PER_WG* WAVE) void k_moments(const VehDesc* _
         started[i] = lh;
     }
 
+    // The persistent masses of the cells ending here, loaded before any store (so that they travel
+    // with the other loads instead of waiting behind the weight_array stores).
+    float rho[MOM_ITEMS];
+#pragma unroll
+    for (int i = 0; i < MOM_ITEMS; ++i)
+        rho[i] = end[i] ? d.f[GF_PERS][key[i]] : 0.0f;
     float w[MOM_ITEMS];
     load_run<MOM_ITEMS>(d.weight_array, p0, cnt, w);
     if (post)
@@ -1120,7 +1129,7 @@ __global__ __launch_bounds__(WAVES_PER_WG* WAVE) void k_moments(const VehDesc* _
 #pragma unroll
         for (int c = 0; c < MOM_CH; ++c)
             s[c] = s32[i][c];
-        mom_finalize(d, key[i], s);
+        mom_finalize(d, key[i], rho[i], s);
     }
 }
 
@@ -1155,7 +1164,8 @@ __global__ __launch_bounds__(BLOCK) void k_moments_fix(const VehDesc* __restrict
         s[c] = (float)(acc[c] + headp[(size_t)j * MOM_CH + c]);
     const TileRef tr = tile_ref(tab, 1, u);
     const VehDesc& d = descs[tr.v];
-    mom_finalize(d, d.part[pph].idx[tr.base + MOM_CHUNK - 1], s);
+    const int key = d.part[pph].idx[tr.base + MOM_CHUNK - 1];
+    mom_finalize(d, key, d.f[GF_PERS][key], s);
 }
 
 constexpr int MOM_WAVES_PER_WG = 4;

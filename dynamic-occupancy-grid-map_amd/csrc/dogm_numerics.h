@@ -336,6 +336,28 @@ DOGM_HD fx_t fx_from_f64(double v)
     return neg ? -(fx_t)mag : (fx_t)mag;
 }
 
+// fx_from_f64 for v with a clear sign bit (+0, positive values, +inf; the resampling draws): the
+// same value, written on 64-bit halves without branches (the 53-bit significand shifted by
+// sh = E - 987 lands in at most two adjacent words) and without the two's-complement step.
+DOGM_HD fx_t fx_from_f64_nonneg(double v)
+{
+    const uint64_t b = f64_bits(v);
+    const int E = (int)((b >> 52) & 0x7ffu);
+    const int sh = E - 1075 + FX_FRAC;
+    const uint64_t m = (b & 0x000fffffffffffffull) | 0x0010000000000000ull;
+    const uint64_t hi_a = m << ((sh - 64) & 63);  // 64 <= sh <= 74
+    const uint64_t lo_b = m << (sh & 63);         // 0 <= sh < 64
+    const uint64_t hi_b = m >> ((64 - sh) & 63);  // 12 <= sh < 64
+    const uint64_t lo_c = m >> ((-sh) & 63);      // -64 < sh < 0
+    // E = 0 (+0, subnormals) gives sh = -987: both words 0.
+    uint64_t lo = sh >= 64 ? 0 : (sh >= 0 ? lo_b : (sh > -64 ? lo_c : 0));
+    uint64_t hi = sh >= 64 ? hi_a : (sh > 11 ? hi_b : 0);
+    const bool sat = E == 0x7ff || sh > 74;  // v >= 2^39, +inf, NaN
+    lo = sat ? 0 : lo;
+    hi = sat ? (1ull << 62) : hi;
+    return (fx_t)(((ufx_t)hi << 64) | (ufx_t)lo);
+}
+
 // Round-to-nearest-even of raw * 2^-88 to a float with `mbits` significand bits
 // (24 for f32, 53 for f64).  Returns the significand (with the implicit bit) and the
 // unbiased exponent of its leading bit; mag must be non-zero.
@@ -500,7 +522,8 @@ DOGM_HD Spacing spacing_setup(fx_t s_n, fx_t cdf_total)
         sp.shift = msb > 52 ? msb - 52 : 0;
         sp.inv = 1.0 / (double)(uint64_t)(s_n >> sp.shift);
     }
-    sp.total = fx_to_f64(cdf_total);
+    // A total wrapped below zero (sums of saturated weights) counts as 0, so draws are never negative.
+    sp.total = cdf_total > 0 ? fx_to_f64(cdf_total) : 0.0;
     return sp;
 }
 
@@ -516,6 +539,13 @@ DOGM_HD double spacing_fraction(fx_t s_k, const Spacing& sp)
 DOGM_HD fx_t spacing_draw(fx_t s_k, const Spacing& sp)
 {
     return fx_from_f64(sp.total * spacing_fraction(s_k, sp));
+}
+
+// spacing_draw through the sign-free conversion: the product of the total (never below 0) and
+// u_k (in [0, 1)) has a clear sign bit, so the value is the same.
+DOGM_HD fx_t spacing_draw_nonneg(fx_t s_k, const Spacing& sp)
+{
+    return fx_from_f64_nonneg(sp.total * spacing_fraction(s_k, sp));
 }
 
 // Systematic resampling (an opt-in draw mode; the reference draws multinomially,

@@ -39,6 +39,7 @@ static_assert(RS_TILE == SCAN_TILE, "draw tiles and CDF tiles share the exact-sc
 
 struct SrcJointPos
 {
+    static constexpr bool kNonNeg = true;  // cdf_weight: +0 or positive
     __device__ bool active(const VehDesc& d) const { return d.cdf_mode == CDF_EXACT; }
     __device__ int len(const VehDesc& d) const { return d.N + d.B; }
     __device__ void load4(const VehDesc& d, int p0, int cnt, float (&v)[4]) const
@@ -67,12 +68,13 @@ __device__ __forceinline__ void exp4(const VehDesc& d, int p0, int n, fx_t* e)
     resample_exponentials4(d.seed, d.inst, d.ep_rs, (uint32_t)(p0 >> 2), f);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        e[i] = p0 + i < n ? fx_from_f32(f[i]) : (fx_t)0;
+        e[i] = p0 + i < n ? fx_from_f32_nonneg(f[i]) : (fx_t)0;  // -log(u) >= 0 (or -0)
 }
 
 // The N + 1 exponentials as a scan source (tile totals only); systematic draws need none.
 struct SrcExp
 {
+    static constexpr bool kNonNeg = true;  // -log(u) >= 0 (or -0)
     __device__ bool active(const VehDesc& d) const { return d.draw_mode == DRAW_MULTINOMIAL; }
     __device__ int len(const VehDesc& d) const { return d.N + 1; }
     __device__ void load4(const VehDesc& d, int p0, int, float (&v)[4]) const
@@ -532,8 +534,8 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
     const DevScalars* sc = scs + tr.v;
     const int J = n + d.B;
     fx_t r[RS_ITEMS];
-    merge_draws<fx_t>(d, sc, base, n, exp_tile_pre, [](fx_t s_k, const Spacing& sp) { return spacing_draw(s_k, sp); },
-                      s_cdf, r, s_rb);
+    merge_draws<fx_t>(d, sc, base, n, exp_tile_pre,
+                      [](fx_t s_k, const Spacing& sp) { return spacing_draw_nonneg(s_k, sp); }, s_cdf, r, s_rb);
     const int S = (J + CDF_SEG - 1) / CDF_SEG;  // segments holding entries
     const fx_t* si = seg_incl + (size_t)(d.u0[SP_J] / SCAN_Q) * CDF_SEGS_PER_TILE;
     const auto seg_end = [si](int s) { return si[s]; };

@@ -194,14 +194,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(const VehDesc* __restrict__
 // values carrying the particle's association flag in bit 31 (read here in particle order, so the
 // gather needs no random byte load); later passes read the previous pass's output.
 // GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
-// sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
-// weight_array; the cell bounds are then recorded by the next segmented reduction.  With aos the
-// state is read from the 16-byte records predict left in the CDF buffer.
+// sorted order into part[pph ^ 1] (cell index, state, weight, association) and, with GATHER_WA, the
+// weight copy into weight_array; the cell bounds are then recorded by the next segmented reduction.
+// With GATHER_AOS the state is read from the 16-byte records predict left in the CDF buffer, and
+// for vehicles with uniform weights the predicted weight is formed here (predicted_weight).
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ tab, int pph, int ps, int tiles,
                                                           const unsigned* __restrict__ goff, int passes,
-                                                          unsigned* __restrict__ ghist, int aos)
+                                                          unsigned* __restrict__ ghist, int gflags)
 {
     __shared__ uint16_t s_wcnt[SORT_WAVES][RADIX];  // counts <= SORT_TILE; 16 bits keep 8 workgroups per CU
     __shared__ unsigned s_digoff[RADIX];
@@ -325,6 +326,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
         const unsigned d = ((unsigned)kk[b] >> shift) & (RADIX - 1);
         gg[b] = s_digoff[d] + (unsigned)s;
     }
+    // Uniform weights: every particle's prior weight is weight[0], which predict left in place.
+    const float w_prior = GATHER && (gflags & GATHER_AOS) && dv.wuni ? dv.part[pph].weight[0] : 0.0f;
 #pragma unroll
     for (int b = 0; b < GB; ++b)
     {
@@ -336,22 +339,25 @@ __global__ __launch_bounds__(BLOCK) void k_sort_downsweep(const VehDesc* __restr
             const DevParticles src = dv.part[pph];
             const DevParticles dst = dv.part[pph ^ 1];
             const int o = sort_value_index(vv[b]);
-            const float w = src.weight[o];
-            if (aos)
+            float w;
+            if (gflags & GATHER_AOS)
             {
                 const float4 st = reinterpret_cast<const float4*>(dv.cdf)[o];
                 dst.pos[g] = make_float2(st.x, st.y);
                 dst.vel[g] = make_float2(st.z, st.w);
+                w = dv.wuni ? predicted_weight(dv, w_prior, st) : src.weight[o];
             }
             else
             {
+                w = src.weight[o];
                 dst.pos[g] = src.pos[o];
                 dst.vel[g] = src.vel[o];
             }
             dst.weight[g] = w;
             dst.assoc[g] = sort_value_assoc(vv[b]);
             dst.idx[g] = kk[b];
-            dv.weight_array[g] = w;
+            if (gflags & GATHER_WA)
+                dv.weight_array[g] = w;
         }
         else
         {
@@ -421,15 +427,16 @@ __device__ __forceinline__ unsigned lb_read(const unsigned* w)
 // flag in bit 31 (read here in particle order, so the gather needs no random byte load); later
 // passes read the previous pass's output.
 // GATHER (last pass only): instead of the sorted keys and the permutation, write the particles in
-// sorted order into part[pph ^ 1] (cell index, state, weight, association) and the weight copy into
-// weight_array; the cell bounds are then recorded by the next segmented reduction.  With aos the
-// state is read from the 16-byte records predict left in the CDF buffer.
+// sorted order into part[pph ^ 1] (cell index, state, weight, association) and, with GATHER_WA, the
+// weight copy into weight_array; the cell bounds are then recorded by the next segmented reduction.
+// With GATHER_AOS the state is read from the 16-byte records predict left in the CDF buffer, and
+// for vehicles with uniform weights the predicted weight is formed here (predicted_weight).
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restrict__ descs,
                                                          const uint32_t* __restrict__ tab, int pph, int ps, int passes,
                                                          unsigned* __restrict__ ghist, unsigned* __restrict__ status,
                                                          unsigned* __restrict__ tickets, unsigned epoch,
-                                                         int spin_limit, unsigned* __restrict__ err, int aos)
+                                                         int spin_limit, unsigned* __restrict__ err, int gflags)
 {
     __shared__ uint16_t s_wcnt[SORT_WAVES][RADIX];  // counts <= SORT_TILE; 16 bits keep 8 workgroups per CU
     __shared__ unsigned s_digoff[RADIX];
@@ -632,6 +639,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restri
         const unsigned db = ((unsigned)kk[b] >> shift) & (RADIX - 1);
         gg[b] = s_gbase[db] + ((unsigned)s - s_digoff[db]);
     }
+    // Uniform weights: every particle's prior weight is weight[0], which predict left in place.
+    const float w_prior = GATHER && (gflags & GATHER_AOS) && dv.wuni ? dv.part[pph].weight[0] : 0.0f;
 #pragma unroll
     for (int b = 0; b < GB; ++b)
     {
@@ -643,22 +652,25 @@ __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restri
             const DevParticles src = dv.part[pph];
             const DevParticles dst = dv.part[pph ^ 1];
             const int o = sort_value_index(vv[b]);
-            const float w = src.weight[o];
-            if (aos)
+            float w;
+            if (gflags & GATHER_AOS)
             {
                 const float4 st = reinterpret_cast<const float4*>(dv.cdf)[o];
                 dst.pos[g] = make_float2(st.x, st.y);
                 dst.vel[g] = make_float2(st.z, st.w);
+                w = dv.wuni ? predicted_weight(dv, w_prior, st) : src.weight[o];
             }
             else
             {
+                w = src.weight[o];
                 dst.pos[g] = src.pos[o];
                 dst.vel[g] = src.vel[o];
             }
             dst.weight[g] = w;
             dst.assoc[g] = sort_value_assoc(vv[b]);
             dst.idx[g] = kk[b];
-            dv.weight_array[g] = w;
+            if (gflags & GATHER_WA)
+                dv.weight_array[g] = w;
         }
         else
         {
@@ -670,7 +682,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_onesweep(const VehDesc* __restri
 
 static_assert(BLOCK == RADIX, "the sort kernels assign one digit per thread");
 
-void run_sort(Engine* E, int pph, bool hist_done, bool gather)
+void run_sort(Engine* E, int pph, bool hist_done, bool gather, bool write_wa)
 {
     const int tiles = E->tiles(SP_P, SORT_Q);
     const int passes = E->passes;
@@ -679,6 +691,7 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
     const bool onesweep = E->tune.sort == SORT_ONESWEEP;
     // predict of the same step left the state as 16-byte records in the CDF buffer (run_predict)
     const bool aos = hist_done && gather;
+    const int gflags = (aos ? GATHER_AOS : 0) | (write_wa ? GATHER_WA : 0);
     if (onesweep && (E->status_plan != E->plan_epoch || passes == 1))
     {
         // New tiles (or a new buffer, or words of the other form): no word may look published.
@@ -709,7 +722,7 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
             if (g)
                 hipLaunchKernelGGL(k_sort_onesweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                    E->tab(SP_P), pph, ps, passes, E->sort_ghist, E->sort_status, E->sort_tickets,
-                                   epoch, E->lookback_spin, E->d_err, aos ? 1 : 0);
+                                   epoch, E->lookback_spin, E->d_err, gflags);
             else
                 hipLaunchKernelGGL(k_sort_onesweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                    E->tab(SP_P), pph, ps, passes, E->sort_ghist, E->sort_status, E->sort_tickets,
@@ -732,7 +745,7 @@ void run_sort(Engine* E, int pph, bool hist_done, bool gather)
         launch_begin(E, g ? K_SORT_GATHER : K_SORT_DOWNSWEEP);
         if (g)
             hipLaunchKernelGGL(k_sort_downsweep<true>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_P),
-                               pph, ps, tiles, E->sort_status, passes, E->sort_ghist, aos ? 1 : 0);
+                               pph, ps, tiles, E->sort_status, passes, E->sort_ghist, gflags);
         else
             hipLaunchKernelGGL(k_sort_downsweep<false>, dim3(tiles), dim3(BLOCK), 0, E->stream, E->d_desc,
                                E->tab(SP_P), pph, ps, tiles, E->sort_status, passes, E->sort_ghist, 0);

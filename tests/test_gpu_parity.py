@@ -270,6 +270,29 @@ def test_full_updates_mixed_with_grid_writers():
         f += 1
 
 
+def test_full_updates_after_setting_unequal_weights():
+    """Full updates normally start from equal weights (the last resampling's, or the initial 1 / N),
+    which the prediction then leaves in place for the sorting pass to form the predicted weights;
+    weights written through the C-ABI between updates are unequal and must be read per particle.
+    Both paths, and the switch back after the next resampling, match the checker."""
+    p = small_params(gs=60, res=0.5, n=20000, b=2000)
+    sc = small_scene(gs=60, res=0.5)
+    gpu = DOGM(p, seed=5)
+    orc = OracleDOGM(p, seed=5)
+    rng = np.random.default_rng(31)
+    for f in range(6):
+        _say(f"unequal weights frame {f}")
+        if f in (2, 4):
+            w = (rng.uniform(0.0, 2.0, p.particle_count) / p.particle_count).astype(np.float32)
+            gpu.set_particles(weight=w)
+            orc.set_particles(weight=w)
+        x, y, yaw = sc.ego_pose(f)
+        m = sc.measurement(f)
+        gpu.update_grid(m, x, y, yaw, 0.1)
+        orc.update_grid(m, x, y, yaw, 0.1)
+        assert_state_equal(f"frame {f}", gpu, orc, ancestors=True, births=True)
+
+
 def test_hostile_weights_in_resampling():
     """Resampling from injected weights with negative values, NaN, zeros, subnormals and a wide
     dynamic range (persistent and birth weights): negative and NaN weights count as zero in the CDF on
