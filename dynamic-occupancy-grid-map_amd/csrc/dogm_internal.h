@@ -136,6 +136,7 @@ struct VehDesc
     // from here and writes the copy (dogm.cu:205-215) as it goes, instead of a separate copy.
     const dogm_measurement_cell* meas_in;
     float* weight_array;
+    float4* cellrec;  // Vehicle::cellrec
     float* born;
     float* m_raw;
     int* pend;
@@ -259,6 +260,9 @@ struct Vehicle
     DevGrid grid{};          // grid.f[GF_FREE] is the current free mass
     float* free_alt = nullptr;
     float* field_tmp = nullptr;  // scratch for eager grid shifts
+    // Per cell (p_A mu_A, (1 - p_A) mu_UA, rho_p, 0): written by the persistent update for cells with
+    // particles when the moments of the same step form the posterior weights.
+    float4* cellrec = nullptr;
     dogm_measurement_cell* meas = nullptr;
     // Device measurement grid of the next full update step, copied into meas by that step's
     // occupancy update (nullptr: meas already holds the grid).

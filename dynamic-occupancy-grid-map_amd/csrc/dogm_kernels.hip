@@ -612,6 +612,7 @@ struct PersOp
     static constexpr int WAVES = IT == 8 ? 6 : 8;
     int pph;
     int norm;
+    int rec;  // write the cell factors of the moments' posterior weights (cellrec)
     __device__ const int* keys(const VehDesc& d) const { return d.part[pph].idx; }
     __device__ void head(const VehDesc&, int, int) const {}
     __device__ void end(const VehDesc&, int, int) const {}
@@ -648,9 +649,18 @@ struct PersOp
     {
         const float acc = fx_to_f32(s[0]);
         const float rho_p = d.f[GF_PERS][key];
-        d.f[GF_MU_A][key] = acc > 0.0f ? rho_p / acc : 0.0f;
+        const float mu_A = acc > 0.0f ? rho_p / acc : 0.0f;
+        d.f[GF_MU_A][key] = mu_A;
         const float pred = d.f[GF_PRED][key];
-        d.f[GF_MU_UA][key] = pred > 0.0f ? rho_p / pred : 0.0f;
+        const float mu_UA = pred > 0.0f ? rho_p / pred : 0.0f;
+        d.f[GF_MU_UA][key] = mu_UA;
+        if (rec)
+        {
+            // The factors of w = (p_A mu_A) w_lik + ((1 - p_A) mu_UA) w_prior in that association
+            // (update_persistent_particles.cu:76-87), and rho_p for the moments.
+            const float pA = d.meas[key].p_A;
+            d.cellrec[key] = make_float4(pA * mu_A, (1.0f - pA) * mu_UA, rho_p, 0.0f);
+        }
     }
 };
 
@@ -684,9 +694,9 @@ __global__ __launch_bounds__(BLOCK) void k_post_weights(const VehDesc* __restric
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post)
 {
     if (E->tune.seg_items == 4)
-        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<4>{pph, fused_normalize ? 1 : 0});
+        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<4>{pph, fused_normalize ? 1 : 0, defer_post ? 1 : 0});
     else
-        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<8>{pph, fused_normalize ? 1 : 0});
+        launch_segreduce(E, K_SEG_PERS, K_SEG_PERS_FIX, PersOp<8>{pph, fused_normalize ? 1 : 0, defer_post ? 1 : 0});
     if (!empty_done)
     {
         launch_begin(E, K_EMPTY_CELLS);
@@ -1007,12 +1017,17 @@ __global__ __launch_bounds__(WAVES_PER_WG* WAVE) __attribute__((amdgpu_waves_per
         started[i] = lh;
     }
 
-    // The persistent masses of the cells ending here, loaded before any store (so that they travel
-    // with the other loads instead of waiting behind the weight_array stores).
+    // The persistent masses of the cells ending here (and with post the cells' weight factors, which
+    // carry them), loaded before any store so that they travel with the other loads instead of
+    // waiting behind the weight_array stores.
     float rho[MOM_ITEMS];
+    float4 cf[MOM_ITEMS];
 #pragma unroll
     for (int i = 0; i < MOM_ITEMS; ++i)
-        rho[i] = end[i] ? d.f[GF_PERS][key[i]] : 0.0f;
+    {
+        cf[i] = post && i < cnt ? d.cellrec[key[i]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rho[i] = post ? cf[i].z : (end[i] ? d.f[GF_PERS][key[i]] : 0.0f);
+    }
     float w[MOM_ITEMS];
     load_run<MOM_ITEMS>(d.weight_array, p0, cnt, w);
     if (post)
@@ -1021,14 +1036,8 @@ __global__ __launch_bounds__(WAVES_PER_WG* WAVE) __attribute__((amdgpu_waves_per
         load_run<MOM_ITEMS>(d.part[pph].weight, p0, cnt, wp);
 #pragma unroll
         for (int i = 0; i < MOM_ITEMS; ++i)
-        {
             if (i < cnt)
-            {
-                const int c = key[i];
-                const float pA = d.meas[c].p_A;
-                w[i] = pA * d.f[GF_MU_A][c] * w[i] + (1.0f - pA) * d.f[GF_MU_UA][c] * wp[i];
-            }
-        }
+                w[i] = cf[i].x * w[i] + cf[i].y * wp[i];  // (p_A mu_A) w + ((1 - p_A) mu_UA) w_prior
         store_run<MOM_ITEMS>(d.weight_array, p0, cnt, w);
     }
     float vx[MOM_ITEMS], vy[MOM_ITEMS];

@@ -27,15 +27,19 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, uint64_t active)
     const unsigned first = __shfl(d, (int)__builtin_ctzll(active), WAVE);
     if ((__ballot(d == first) & active) == active)
         return active;
-    uint64_t peers = active;
+    // Bit-sliced match on 32-bit halves: per bit, the lane's bit spread to 0 / all ones (e), the
+    // wave's mask m of lanes with the bit set, and peers &= ~(m ^ e) (m where the bit is set, its
+    // complement where not).
+    uint32_t plo = (uint32_t)active, phi = (uint32_t)(active >> 32);
 #pragma unroll
     for (int b = 0; b < RADIX_BITS; ++b)
     {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        const uint32_t e = (uint32_t)((int)(d << (31 - b)) >> 31);
+        const uint64_t m = __ballot(e != 0u);
+        plo &= ~((uint32_t)m ^ e);
+        phi &= ~((uint32_t)(m >> 32) ^ e);
     }
-    return peers;
+    return ((uint64_t)phi << 32) | plo;
 }
 
 // Histogram of every pass's digit over all keys, per vehicle: ghist[(v * 4 + pass) * 256 + digit].
