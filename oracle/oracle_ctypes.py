@@ -59,6 +59,8 @@ def lib():
             "oracle_uniform01_oc": (f, [C.c_uint32]),
             "oracle_fx_from_f32": (None, [f, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
             "oracle_fx_from_f32_nonneg_mismatches": (C.c_int, [C.c_void_p, C.c_int]),
+            "oracle_fx_from_f64_nonneg_mismatches": (C.c_int, [C.c_void_p, C.c_int]),
+            "oracle_exponential_sign_violations": (C.c_int, []),
             "oracle_fx_from_f64": (None, [C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
             "oracle_fx_to_f32": (f, [C.c_int64, C.c_uint64]),
             "oracle_fx_to_f64": (C.c_double, [C.c_int64, C.c_uint64]),
@@ -329,6 +331,18 @@ def fx_from_f32_nonneg_mismatches(bits) -> int:
     import numpy as np
     b = np.ascontiguousarray(bits, dtype=np.uint32)
     return lib().oracle_fx_from_f32_nonneg_mismatches(b.ctypes.data, len(b))
+
+
+def fx_from_f64_nonneg_mismatches(bits) -> int:
+    """Count of the double bit patterns in `bits` (uint64 array) where the sign-free conversion
+    differs from fx_from_f64."""
+    import numpy as np
+    b = np.ascontiguousarray(bits, dtype=np.uint64)
+    return lib().oracle_fx_from_f64_nonneg_mismatches(b.ctypes.data, len(b))
+
+
+def exponential_sign_violations() -> int:
+    return lib().oracle_exponential_sign_violations()
 
 
 def fx_from_f64(v: float) -> int:

@@ -1396,6 +1396,30 @@ int oracle_fx_from_f32_nonneg_mismatches(const uint32_t* bits, int n)
     }
     return bad;
 }
+// The same for doubles and fx_from_f64_nonneg.
+int oracle_fx_from_f64_nonneg_mismatches(const uint64_t* bits, int n)
+{
+    int bad = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        double v;
+        std::memcpy(&v, &bits[i], sizeof v);
+        bad += fx_from_f64_nonneg(v) != fx_from_f64(v);
+    }
+    return bad;
+}
+// Every exponential -log(u) of the resampling draws, u = uniform01_oc(x) over all 2^24 values it
+// takes: returns how many are below zero or convert differently through fx_from_f32_nonneg.
+int oracle_exponential_sign_violations(void)
+{
+    int bad = 0;
+    for (uint32_t q = 0; q < (1u << 24); ++q)
+    {
+        const float e = -log_f32(uniform01_oc(q << 8));
+        bad += e < 0.0f || fx_from_f32_nonneg(e) != fx_from_f32(e);
+    }
+    return bad;
+}
 void oracle_fx_from_f64(double v, int64_t* hi, uint64_t* lo)
 {
     const fx_t r = fx_from_f64(v);

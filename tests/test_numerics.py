@@ -152,6 +152,28 @@ def test_fx_from_f32_nonneg_equals_general_form():
         assert oc.fx_from_f32_nonneg_mismatches(bits) == 0, hex(start)
 
 
+def test_fx_from_f64_nonneg_equals_general_form():
+    """Doubles with a clear sign bit convert identically through the sign-free form of the resampling
+    draws: every exponent with random significands, the significand extremes, and the boundaries of
+    the shift ranges (sh = E - 987 at -64, 0, 11, 12, 64, 74, 75)."""
+    rng = np.random.default_rng(12)
+    exps = np.repeat(np.arange(2048, dtype=np.uint64), 512)
+    mant = rng.integers(0, 1 << 52, size=exps.size, dtype=np.uint64)
+    bits = (exps << np.uint64(52)) | mant
+    edge = []
+    for e in range(2048):
+        for m in (0, 1, (1 << 52) - 1, 1 << 51):
+            edge.append((e << 52) | m)
+    bits = np.concatenate([bits, np.array(edge, dtype=np.uint64)])
+    assert oc.fx_from_f64_nonneg_mismatches(bits) == 0
+
+
+def test_resampling_exponentials_have_a_clear_sign():
+    """All 2^24 exponentials -log(u), u in (0, 1], are >= 0 (or -0) and convert identically through the
+    sign-free form the merge kernel uses."""
+    assert oc.exponential_sign_violations() == 0
+
+
 def test_fx_from_f32_exact():
     rng = np.random.default_rng(4)
     vals = [0.0, -0.0, 1.0, -1.0, 0.5, 1e-30, 2.0**-88, 2.0**-89, 3 * 2.0**-90, 2.0**38, -(2.0**38) * 1.5,
