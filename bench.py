@@ -47,11 +47,13 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
     """Algorithmic bytes one launch must move per vehicle (read once what it consumes, write once what
     it hands on); gathers through cell indices count their payload once."""
     table = {
-        "predict": 44 * N,                       # state+weight in, state+weight+idx out
+        # state in, state + idx out (the prior weights are equal after every resampling, so predict
+        # leaves them in place and the gathering pass forms the predicted weights)
+        "predict": 36 * N,
         "sort_downsweep": 16 * N,                # a pass before the last: keys + values in and out
-        # the last pass: keys + values in, the 21 B payload gathered, sorted particles (25 B) and the
-        # weight copy out
-        "sort_gather": (8 + 21 + 25 + 4) * N,
+        # the last pass: keys + values in, the 16 B state gathered, sorted particles (25 B) out (the
+        # weight copy is skipped: the persistent update of the same step rewrites it)
+        "sort_gather": (8 + 16 + 25) * N,
         "seg_occ": 8 * N,                        # keys + weights
         # start, free, meas occ/free in; 6 fields out; the step's measurement grid copied from the
         # caller's device buffer on the way (16 B in, 16 B out per cell)
