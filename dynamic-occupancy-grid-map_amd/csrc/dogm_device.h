@@ -684,15 +684,16 @@ constexpr int SCAN_Q = SCAN_TILE / UNIT;  // units per scan tile
 // these into the vehicle's inclusive segment ends: the resampling CDF index).
 static_assert(CDF_SEG == WAVE * SCAN_ITEMS && CDF_SEGS_PER_TILE * CDF_SEG == SCAN_TILE, "CDF segments are waves");
 
+// Tile `tile` of a space: the tile total into tile_sum[tile] (and the segment ends, see above).
 template <class Src>
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __restrict__ descs,
-                                                      const uint32_t* __restrict__ tab, fx_t* __restrict__ tile_sum,
-                                                      fx_t* __restrict__ seg_end)
+__device__ __forceinline__ void scan_tile_body(const Src& src, const VehDesc* __restrict__ descs,
+                                               const uint32_t* __restrict__ tab, int tile,
+                                               fx_t* __restrict__ tile_sum, fx_t* __restrict__ seg_end)
 {
-    const TileRef tr = tile_ref(tab, SCAN_Q, blockIdx.x);
+    const TileRef tr = tile_ref(tab, SCAN_Q, tile);
     const VehDesc& d = descs[tr.v];
     if (!src.active(d))
-        return;
+        return;  // uniform over the workgroup
     const int n = src.len(d);
     const int p0 = tr.base + threadIdx.x * SCAN_ITEMS;
     const int cnt = max(0, min(SCAN_ITEMS, n - p0));
@@ -707,9 +708,33 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __
     fx_t total;
     const fx_t ex = block_exclusive_scan_fx(s, &total);
     if (threadIdx.x == 0)
-        tile_sum[blockIdx.x] = total;
+        tile_sum[tile] = total;
     if (seg_end && (threadIdx.x & (WAVE - 1)) == WAVE - 1)
-        seg_end[(size_t)CDF_SEGS_PER_TILE * blockIdx.x + threadIdx.x / WAVE] = ex + s;
+        seg_end[(size_t)CDF_SEGS_PER_TILE * tile + threadIdx.x / WAVE] = ex + s;
+}
+
+template <class Src>
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(Src src, const VehDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tab, fx_t* __restrict__ tile_sum,
+                                                      fx_t* __restrict__ seg_end)
+{
+    scan_tile_body(src, descs, tab, blockIdx.x, tile_sum, seg_end);
+}
+
+// The tiles of two independent spaces in one launch: workgroups [0, tiles_a) take space A, the rest
+// space B (one launch instead of two; alternating the spaces measured no faster).
+template <class SrcA, class SrcB>
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles_pair(SrcA a, const uint32_t* __restrict__ tab_a, int tiles_a,
+                                                           fx_t* __restrict__ sum_a, fx_t* __restrict__ seg_end_a,
+                                                           SrcB b, const uint32_t* __restrict__ tab_b, int tiles_b,
+                                                           fx_t* __restrict__ sum_b,
+                                                           const VehDesc* __restrict__ descs)
+{
+    (void)tiles_b;
+    if ((int)blockIdx.x < tiles_a)
+        scan_tile_body(a, descs, tab_a, blockIdx.x, sum_a, seg_end_a);
+    else
+        scan_tile_body(b, descs, tab_b, blockIdx.x - tiles_a, sum_b, (fx_t*)nullptr);
 }
 
 enum ScanMode
@@ -737,15 +762,25 @@ __device__ __forceinline__ bool scan_takes(const VehDesc& d, int only)
 // contiguous chunk), optionally the inclusive segment ends (tile_incl holds the tile-local ends
 // k_scan_tiles wrote; the tile's prefix is added in place), and the scalars derived from the
 // vehicle's total.  Vehicles the filter `only` excludes are skipped.
-template <int Unused = 0>
-__global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ descs, int space,
-                                                    const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
-                                                    fx_t* __restrict__ tile_incl, DevScalars* __restrict__ scs,
-                                                    int mode, int only)
+// tile_sum holds `parts` partial sums per tile (consecutive), which are added first.
+__device__ __forceinline__ fx_t scan_tile_total(const fx_t* __restrict__ tile_sum, int t, int parts)
+{
+    if (parts == 1)
+        return tile_sum[t];
+    fx_t s = 0;
+    for (int q = 0; q < parts; ++q)
+        s += tile_sum[(size_t)parts * t + q];
+    return s;
+}
+
+__device__ __forceinline__ void scan_top_body(const VehDesc* __restrict__ descs, int space,
+                                              const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
+                                              fx_t* __restrict__ tile_incl, DevScalars* __restrict__ scs, int mode,
+                                              int only, int parts = 1)
 {
     const VehDesc& d = descs[blockIdx.x];
     if (!scan_takes(d, only))
-        return;
+        return;  // uniform over the workgroup
     DevScalars* sc = scs + blockIdx.x;
     const int t0 = d.u0[space] / SCAN_Q;
     const int ntiles = d.nu[space] / SCAN_Q;
@@ -759,7 +794,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
         fx_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            x[k] = tc + k < b1 ? tile_sum[tc + k] : (fx_t)0;
+            x[k] = tc + k < b1 ? scan_tile_total(tile_sum, tc + k, parts) : (fx_t)0;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             s += x[k];
@@ -771,13 +806,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
         fx_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            x[k] = tc + k < b1 ? tile_sum[tc + k] : (fx_t)0;
+            x[k] = tc + k < b1 ? scan_tile_total(tile_sum, tc + k, parts) : (fx_t)0;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
         {
             if (tc + k < b1)
             {
-                tile_pre[tc + k] = run;
+                if (tile_pre)
+                    tile_pre[tc + k] = run;
                 if (tile_incl)
                 {
                     fx_t* se = tile_incl + (size_t)CDF_SEGS_PER_TILE * (tc + k);
@@ -812,6 +848,15 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ 
             sc->new_weight = jm / (float)d.N;
         }
     }
+}
+
+template <int Unused = 0>
+__global__ __launch_bounds__(BLOCK) void k_scan_top(const VehDesc* __restrict__ descs, int space,
+                                                    const fx_t* __restrict__ tile_sum, fx_t* __restrict__ tile_pre,
+                                                    fx_t* __restrict__ tile_incl, DevScalars* __restrict__ scs,
+                                                    int mode, int only, int parts = 1)
+{
+    scan_top_body(descs, space, tile_sum, tile_pre, tile_incl, scs, mode, only, parts);
 }
 
 // Sink interface:
@@ -856,20 +901,26 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(Src src, Sink sink, const 
         sink.apply4(d, scs + tr.v, p0, cnt, pre, v);
 }
 
+// unit_sums: the sources' exact sums per 256-element unit were left there by an earlier kernel of the
+// step (SCAN_Q per tile), so the tile pass is skipped.
 template <class Src, class Sink>
 inline void run_exact_scan(Engine* E, int space, const Src& src, const Sink& sink, int mode, int kid_apply,
-                           int only, fx_t* tile_incl = nullptr)
+                           int only, fx_t* tile_incl = nullptr, const fx_t* unit_sums = nullptr)
 {
     const int tiles = E->tiles(space, SCAN_Q);
     if (tiles <= 0)
         return;
-    launch_begin(E, K_SCAN_TILES);
-    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, E->stream, src, E->d_desc, E->tab(space),
-                       E->scan_tile_sum, tile_incl);
-    launch_end(E, K_SCAN_TILES);
+    if (!unit_sums)
+    {
+        launch_begin(E, K_SCAN_TILES);
+        hipLaunchKernelGGL(k_scan_tiles<Src>, dim3(tiles), dim3(BLOCK), 0, E->stream, src, E->d_desc, E->tab(space),
+                           E->scan_tile_sum, tile_incl);
+        launch_end(E, K_SCAN_TILES);
+    }
     launch_begin(E, K_SCAN_TOP);
-    hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, space, E->scan_tile_sum,
-                       E->scan_tile_pre, tile_incl, E->scalars, mode, only);
+    hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, space,
+                       unit_sums ? unit_sums : E->scan_tile_sum, E->scan_tile_pre, tile_incl, E->scalars, mode, only,
+                       unit_sums ? SCAN_Q : 1);
     launch_end(E, K_SCAN_TOP);
     launch_begin(E, kid_apply);
     hipLaunchKernelGGL((k_scan_apply<Src, Sink>), dim3(tiles), dim3(BLOCK), 0, E->stream, src, sink, E->d_desc,

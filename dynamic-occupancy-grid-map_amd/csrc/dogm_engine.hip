@@ -232,7 +232,7 @@ int plan_sizes(Vehicle* const* veh, int m, PlanSizes* ps, std::string* why)
 // Byte offsets of the step buffers in one region (256-byte aligned pieces).
 struct ScratchLayout
 {
-    size_t tab, scan_sum, scan_pre, cdf_incl, seg_flags, seg_head, seg_tail, status, total;
+    size_t tab, scan_sum, scan_pre, exp_sum, cdf_incl, seg_flags, seg_head, seg_tail, status, total;
 };
 
 ScratchLayout scratch_layout(const PlanSizes& ps)
@@ -247,6 +247,7 @@ ScratchLayout scratch_layout(const PlanSizes& ps)
     L.tab = take(sizeof(uint32_t) * (size_t)ps.tab);
     L.scan_sum = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
     L.scan_pre = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
+    L.exp_sum = take(sizeof(fx_t) * (size_t)ps.scan_tiles);
     L.cdf_incl = take(sizeof(fx_t) * CDF_SEGS_PER_TILE * (size_t)ps.scan_tiles);
     L.seg_flags = take((size_t)ps.seg_tiles);
     L.seg_head = take(sizeof(fx_t) * 5 * (size_t)ps.seg_tiles);
@@ -282,13 +283,15 @@ int engine_bind_scratch(Engine* E, void* base, size_t bytes, Vehicle* const* veh
                        "the step needs " + std::to_string(L.total) + " bytes of scratch, " + std::to_string(bytes) +
                            " are bound");
     E->own = Engine::OwnStepBuffers{E->d_tab,         E->tab_cap,        E->scan_tile_sum, E->scan_tile_pre,
-                                    E->cdf_tile_incl, E->scan_tiles_cap, E->seg,           E->seg_tiles_cap,
+                                    E->exp_tile_sum,  E->cdf_tile_incl,  E->scan_tiles_cap, E->seg,
+                                    E->seg_tiles_cap,
                                     E->sort_status,    E->sort_tiles_cap};
     char* b = static_cast<char*>(base);
     E->d_tab = reinterpret_cast<uint32_t*>(b + L.tab);
     E->tab_cap = ps.tab;
     E->scan_tile_sum = reinterpret_cast<fx_t*>(b + L.scan_sum);
     E->scan_tile_pre = reinterpret_cast<fx_t*>(b + L.scan_pre);
+    E->exp_tile_sum = reinterpret_cast<fx_t*>(b + L.exp_sum);
     E->cdf_tile_incl = reinterpret_cast<fx_t*>(b + L.cdf_incl);
     E->scan_tiles_cap = ps.scan_tiles;
     E->seg.flags = reinterpret_cast<uint8_t*>(b + L.seg_flags);
@@ -314,6 +317,7 @@ void engine_unbind_scratch(Engine* E)
     E->tab_cap = o.tab_cap;
     E->scan_tile_sum = o.scan_tile_sum;
     E->scan_tile_pre = o.scan_tile_pre;
+    E->exp_tile_sum = o.exp_tile_sum;
     E->cdf_tile_incl = o.cdf_tile_incl;
     E->scan_tiles_cap = o.scan_tiles_cap;
     E->seg = o.seg;
@@ -380,7 +384,8 @@ void engine_release(Engine* E)
     E->d_err = nullptr;
     {
         void* bufs[] = {E->d_desc,        E->d_upre,         E->d_tab,            E->scan_tile_sum,
-                        E->scan_tile_pre, E->cdf_tile_incl,  E->seg.flags,        E->seg.head_partial,
+                        E->scan_tile_pre, E->exp_tile_sum,   E->cdf_tile_incl,    E->seg.flags,
+                        E->seg.head_partial,
                         E->seg.tail_partial, E->sort_status,  E->sort_ghist,       E->scalars};
         for (void* b : bufs)
             (void)hipFree(b);
@@ -480,12 +485,14 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         {
             if ((st = grow(&E->d_tab, &E->tab_cap, total, "work tables", &why)))
                 return set_err(&E->last_error, st, why);
-            long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c3 = CDF_SEGS_PER_TILE * E->scan_tiles_cap;
+            long long c1 = E->scan_tiles_cap, c2 = E->scan_tiles_cap, c4 = E->scan_tiles_cap,
+                      c3 = CDF_SEGS_PER_TILE * E->scan_tiles_cap;
             if ((st = grow(&E->scan_tile_sum, &c1, scan_tiles, "scan", &why)) ||
                 (st = grow(&E->scan_tile_pre, &c2, scan_tiles, "scan", &why)) ||
+                (st = grow(&E->exp_tile_sum, &c4, scan_tiles, "scan", &why)) ||
                 (st = grow(&E->cdf_tile_incl, &c3, CDF_SEGS_PER_TILE * scan_tiles, "cdf index", &why)))
                 return set_err(&E->last_error, st, why);
-            E->scan_tiles_cap = std::min({c1, c2, c3 / CDF_SEGS_PER_TILE});
+            E->scan_tiles_cap = std::min({c1, c2, c4, c3 / CDF_SEGS_PER_TILE});
             if (seg_tiles > E->seg_tiles_cap)
             {
                 long long a = 0, b = 0, c = 0;
@@ -685,6 +692,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     const bool occ_pers = (stages & ST_OCC) && (stages & ST_PERS);
     const bool assign_occ = (stages & ST_ASSIGN) && (stages & ST_OCC);
     const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
+    const bool occ_birth = (stages & ST_OCC) && (stages & ST_BIRTH);
     const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
     StageRange step_range(replay ? "dogm step (graph replay)" : "dogm step");
     if (!replay)
@@ -715,7 +723,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (stages & ST_OCC)
         {
             StageRange r("gridCellOccupancyUpdate");
-            run_occupancy(E, pph, zero_empty, occ_pers, assign_occ);
+            run_occupancy(E, pph, zero_empty, occ_pers, assign_occ, occ_birth);
         }
         if (stages & ST_PERS)
         {
@@ -725,7 +733,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (stages & ST_BIRTH)
         {
             StageRange r("initializeNewParticles");
-            run_births(E);
+            run_births(E, occ_birth);
         }
         if (stages & ST_MOMENTS)
         {

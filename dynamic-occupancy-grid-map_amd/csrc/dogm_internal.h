@@ -400,6 +400,7 @@ struct Engine
     long long scan_tiles_cap = 0;
     fx_t* scan_tile_sum = nullptr;
     fx_t* scan_tile_pre = nullptr;
+    fx_t* exp_tile_sum = nullptr;  // resampling: the exponentials' tile totals (scanned beside the CDF's)
     // Resampling CDF index: exact mode, the inclusive end of every CDF segment (CDF_SEGS_PER_TILE
     // per scan tile); float modes, two doubles per scan tile (envelope at the end and before it).
     fx_t* cdf_tile_incl = nullptr;
@@ -447,7 +448,7 @@ struct Engine
     {
         uint32_t* d_tab;
         long long tab_cap;
-        fx_t *scan_tile_sum, *scan_tile_pre, *cdf_tile_incl;
+        fx_t *scan_tile_sum, *scan_tile_pre, *exp_tile_sum, *cdf_tile_incl;
         long long scan_tiles_cap;
         SegCarry seg;
         long long seg_tiles_cap;
@@ -567,9 +568,16 @@ void run_predict(Engine* E, int pph, bool hist, bool aos);
 // write_wa: that pass also writes the weight copy into weight_array (not needed when the persistent
 // update of the same step overwrites weight_array).
 void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds, bool write_wa);  // reads pph, writes pph ^ 1
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds);  // free[0] -> free[1]
+// born_sums: the occupancy update leaves the born masses' exact sum per cell unit for the births of the
+// same step (born_unit_sums), whose scan then skips its tile pass.
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds,
+                   bool born_sums);  // free[0] -> free[1]
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
-void run_births(Engine* E);
+void run_births(Engine* E, bool born_sums);
+// Unit sums of the born masses between the occupancy update and the births of a step: the CDF index
+// buffer (SCAN_Q units per scan tile fit its CDF_SEGS_PER_TILE entries per tile), unused until the
+// resampling.
+inline fx_t* born_unit_sums(Engine* E) { return E->cdf_tile_incl; }
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
 // modes: bit m set when a vehicle of the step resamples with CDF mode m (dogm_num::CdfMode).
 void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 1

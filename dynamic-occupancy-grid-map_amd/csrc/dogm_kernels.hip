@@ -467,15 +467,10 @@ struct OccOp
 // Cells whose particles cross reduction tiles get their weight sum here from the reduction's tile
 // partials (the first tile's tail, whole middle tiles, the last tile's head), as the fix-up pass
 // would, and m_raw is written for them.
-__global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
-                                                       const uint32_t* __restrict__ tab, int zero_empty,
-                                                       SegCarry carry, int occ_tile)
+// One cell; returns its born mass rho_b.
+__device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int zero_empty, const SegCarry& carry,
+                                                  int occ_tile)
 {
-    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
-    const VehDesc& d = descs[tr.v];
-    const int c = tr.base + threadIdx.x;
-    if (c >= d.C)
-        return;
     const int gs = d.gs;
     const float* free_src = d.free[0];
     const float one = 1.0f;
@@ -558,6 +553,29 @@ __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict
     d.free[1][c] = free_up;
     d.f[GF_OCC][c] = occ_up;
     d.f[GF_PRED][c] = m_occ_pred;
+    return rho_b;
+}
+
+// One workgroup per 256-cell unit.  With born_sums set (the births run in the same step), the unit's
+// exact sum of born masses goes to born_sums[unit] for the births' top-level scan, which then needs no
+// tile pass over the born masses.
+__global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ tab, int zero_empty,
+                                                       SegCarry carry, int occ_tile, fx_t* __restrict__ born_sums)
+{
+    const TileRef tr = tile_ref(tab, 1, blockIdx.x);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + threadIdx.x;
+    float rho_b = 0.0f;
+    if (c < d.C)
+        rho_b = mass_update_cell(d, c, zero_empty, carry, occ_tile);
+    if (born_sums)
+    {
+        fx_t total;
+        block_exclusive_scan_fx(c < d.C ? fx_from_f32(rho_b) : (fx_t)0, &total);
+        if (threadIdx.x == 0)
+            born_sums[blockIdx.x] = total;
+    }
 }
 
 // normalize_weights (mass_update.cu:51-59) for cells whose predicted occupancy exceeds 1.
@@ -579,7 +597,7 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
     }
 }
 
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds)
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds, bool born_sums)
 {
     if (E->tune.seg_items == 4)
         launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<4>{pph, bounds ? 1 : 0}, false);
@@ -587,7 +605,7 @@ void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, boo
         launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<8>{pph, bounds ? 1 : 0}, false);
     launch_begin(E, K_MASS_UPDATE);
     hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       zero_empty, E->seg, BLOCK * E->tune.seg_items);
+                       zero_empty, E->seg, BLOCK * E->tune.seg_items, born_sums ? born_unit_sums(E) : (fx_t*)nullptr);
     launch_end(E, K_MASS_UPDATE);
     if (defer_normalize)
         return;
@@ -863,11 +881,12 @@ __global__ __launch_bounds__(BLOCK) void k_birth_particles(const VehDesc* __rest
     d.birth.assoc[k] = assoc ? 1 : 0;
 }
 
-void run_births(Engine* E)
+void run_births(Engine* E, bool born_sums)
 {
     // With B = 0 the cell pass still runs (every cell gets n_j = 0 and w_A = w_UA = 0) and the
     // stage still consumes its RNG epoch, so the stream of later stages does not depend on B.
-    run_exact_scan(E, SP_C, SrcBorn{0}, SinkBirth{}, SCAN_ORDERS_B, K_BIRTH_CELLS, ONLY_ALL);
+    run_exact_scan(E, SP_C, SrcBorn{0}, SinkBirth{}, SCAN_ORDERS_B, K_BIRTH_CELLS, ONLY_ALL, nullptr,
+                   born_sums ? born_unit_sums(E) : nullptr);
     const int blocks = E->tiles(SP_B, 1);
     if (blocks <= 0)
         return;

@@ -732,25 +732,41 @@ void k_rs_merge_float(const VehDesc* __restrict__ descs, const uint32_t* __restr
     merge_write(d, sc, pph, base, n, J, anc);
 }
 
+// The resampling's two top-level scans in one launch, in their order (the spacings read the CDF
+// total): the joint CDF of exact-CDF vehicles (segment ends in place, no tile prefixes), then the
+// exponentials of every vehicle (tile prefixes into exp_pre).
+__global__ __launch_bounds__(BLOCK) void k_scan_top_resample(const VehDesc* __restrict__ descs,
+                                                             const fx_t* __restrict__ cdf_sum,
+                                                             fx_t* __restrict__ cdf_incl,
+                                                             const fx_t* __restrict__ exp_sum,
+                                                             fx_t* __restrict__ exp_pre, DevScalars* __restrict__ scs)
+{
+    scan_top_body(descs, SP_J, cdf_sum, (fx_t*)nullptr, cdf_incl, scs, SCAN_RESAMPLE, ONLY_CDF_EXACT);
+    __syncthreads();
+    scan_top_body(descs, SP_E, exp_sum, exp_pre, (fx_t*)nullptr, scs, SCAN_EXP, ONLY_ALL);
+}
+
 void run_resampling(Engine* E, int pph, int modes)
 {
     const bool exact = modes & (1 << CDF_EXACT);
     const int jtiles = E->tiles(SP_J, SCAN_Q);
-    if (exact)
-    {
-        // Tile totals and the coarse index (inclusive tile ends) only; the merge recomputes entries.
-        launch_begin(E, K_SCAN_TILES);
-        hipLaunchKernelGGL(k_scan_tiles<SrcJointPos>, dim3(jtiles), dim3(BLOCK), 0, E->stream, SrcJointPos{},
-                           E->d_desc, E->tab(SP_J), E->scan_tile_sum, E->cdf_tile_incl);
-        launch_end(E, K_SCAN_TILES);
-        launch_begin(E, K_SCAN_TOP);
-        hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_J,
-                           E->scan_tile_sum, E->scan_tile_pre, E->cdf_tile_incl, E->scalars, (int)SCAN_RESAMPLE,
-                           (int)ONLY_CDF_EXACT);
-        launch_end(E, K_SCAN_TOP);
-    }
+    const int etiles = E->tiles(SP_E, SCAN_Q);
+    // Tile totals of the joint CDF (exact-CDF vehicles, with the tile-local segment ends of the coarse
+    // index; the merge recomputes entries) and of the exponentials (multinomial draws) in one launch,
+    // then both top-level scans in one launch.
+    launch_begin(E, K_SCAN_TILES);
+    hipLaunchKernelGGL((k_scan_tiles_pair<SrcJointPos, SrcExp>), dim3(jtiles + etiles), dim3(BLOCK), 0, E->stream,
+                       SrcJointPos{}, E->tab(SP_J), jtiles, E->scan_tile_sum, E->cdf_tile_incl, SrcExp{}, E->tab(SP_E),
+                       etiles, E->exp_tile_sum, E->d_desc);
+    launch_end(E, K_SCAN_TILES);
+    launch_begin(E, K_SCAN_TOP);
+    hipLaunchKernelGGL(k_scan_top_resample, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, E->scan_tile_sum,
+                       E->cdf_tile_incl, E->exp_tile_sum, E->scan_tile_pre, E->scalars);
+    launch_end(E, K_SCAN_TOP);
+    // Float modes: tile totals and prefixes as doubles in the two consumed total buffers (the
+    // exponentials' prefixes stay in scan_tile_pre for the merge).
     double* tsum = reinterpret_cast<double*>(E->scan_tile_sum);
-    double* tpre = reinterpret_cast<double*>(E->scan_tile_pre);
+    double* tpre = reinterpret_cast<double*>(E->exp_tile_sum);
     double* env = reinterpret_cast<double*>(E->cdf_tile_incl);
     const int vb = div_up(E->m, 64);
     auto float_mode = [&](auto tag) {
@@ -770,16 +786,6 @@ void run_resampling(Engine* E, int pph, int modes)
         float_mode(f32c(0));
     if (modes & ~(1 << CDF_EXACT))
         hipLaunchKernelGGL(k_cdff_env, dim3(vb), dim3(64), 0, E->stream, E->d_desc, E->m, env, E->scalars);
-
-    const int etiles = E->tiles(SP_E, SCAN_Q);
-    launch_begin(E, K_EXP_TILES);
-    hipLaunchKernelGGL(k_scan_tiles<SrcExp>, dim3(etiles), dim3(BLOCK), 0, E->stream, SrcExp{}, E->d_desc,
-                       E->tab(SP_E), E->scan_tile_sum, (fx_t*)nullptr);
-    launch_end(E, K_EXP_TILES);
-    launch_begin(E, K_SCAN_TOP);
-    hipLaunchKernelGGL(k_scan_top<0>, dim3(E->m), dim3(BLOCK), 0, E->stream, E->d_desc, (int)SP_E, E->scan_tile_sum,
-                       E->scan_tile_pre, (fx_t*)nullptr, E->scalars, (int)SCAN_EXP, (int)ONLY_ALL);
-    launch_end(E, K_SCAN_TOP);
 
     const int mt = E->tiles(SP_P, RS_TILE / UNIT);
     const bool wide = E->tune.merge_slice == 0 ? mt <= RS_WIDE_MAX_TILES : E->tune.merge_slice == 2;
