@@ -578,6 +578,7 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         d.meas_in = (stages & ST_OCC) ? v->meas_src : nullptr;
         d.weight_array = v->weight_array;
         d.cellrec = v->cellrec;
+        d.cellin = v->cellin;
         d.born = v->born;
         d.m_raw = v->m_raw;
         d.pend = v->pend;
@@ -1115,10 +1116,11 @@ size_t vehicle_footprint(const dogm_params* p)
     const long long C = gs * gs, N = p->particle_count, B = p->new_born_particle_count;
     const long long part = 16 + 4 + 4 + 1;
     // particles x2 + births, 16 grid fields + free_alt + field_tmp + meas(16) + born + m_raw + pend,
-    // weights + ancestors + sort (16 B), CDF (16 B per joint entry), cell factors (16 B), occupancy bits.
+    // weights + ancestors + sort (16 B), CDF (16 B per joint entry), cell factors (16 B) and inputs
+    // (8 B), occupancy bits.
     const long long pad = 8 * (2 * (N & 1) + (B & 1));  // the velocity arrays start 16-byte aligned
     return (size_t)(2 * N * part + B * part + pad + C * 4 * (GF_COUNT + 2 + 2) + C * 16 + C * 12 + N * 8 +
-                    N * 16 + (N + B) * 16 + C * 16 + (C + 63) / 64 * 8);
+                    N * 16 + (N + B) * 16 + C * 24 + (C + 63) / 64 * 8);
 }
 
 int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t instance, std::string* why)
@@ -1153,6 +1155,7 @@ int vehicle_alloc(Vehicle* v, const dogm_params* params, uint64_t seed, uint32_t
         (st = dalloc(&v->sort_keys[0], N, "sort", why, by)) || (st = dalloc(&v->sort_keys[1], N, "sort", why, by)) ||
         (st = dalloc(&v->sort_vals[0], N, "sort", why, by)) || (st = dalloc(&v->sort_vals[1], N, "sort", why, by)) ||
         (st = dalloc(&v->cdf, (size_t)N + B, "cdf", why, by)) || (st = dalloc(&v->cellrec, C, "cell factors", why, by)) ||
+        (st = dalloc(&v->cellin, C, "cell inputs", why, by)) ||
         (st = dalloc(&v->nz, ((size_t)C + 63) / 64, "occupancy bits", why, by)))
         return st;
 
@@ -1200,7 +1203,7 @@ void vehicle_free(Vehicle* v)
                     v->m_raw,    v->pend,         v->ancestors,    v->sort_keys[0], v->sort_keys[1],
                     v->sort_vals[0], v->sort_vals[1], v->cdf,      v->aos_tmp,      v->state_tmp,
                     v->x_occ,    v->x_img,        v->x_mask,       v->x_cnt,        v->x_off,
-                    v->x_idx,    v->x_rec,        v->nz,           v->cellrec};
+                    v->x_idx,    v->x_rec,        v->nz,           v->cellrec,      v->cellin};
     for (void* b : bufs)
         (void)hipFree(b);
     *v = Vehicle{};

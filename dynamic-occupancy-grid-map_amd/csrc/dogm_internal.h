@@ -137,6 +137,7 @@ struct VehDesc
     const dogm_measurement_cell* meas_in;
     float* weight_array;
     float4* cellrec;  // Vehicle::cellrec
+    float2* cellin;   // Vehicle::cellin
     float* born;
     float* m_raw;
     int* pend;
@@ -263,6 +264,9 @@ struct Vehicle
     // Per cell (p_A mu_A, (1 - p_A) mu_UA, rho_p, 0): written by the persistent update for cells with
     // particles when the moments of the same step form the posterior weights.
     float4* cellrec = nullptr;
+    // Per cell with particles (likelihood, Σw before clamping): written by the occupancy update for the
+    // persistent update of the same step (one 8-byte gather per particle instead of two).
+    float2* cellin = nullptr;
     dogm_measurement_cell* meas = nullptr;
     // Device measurement grid of the next full update step, copied into meas by that step's
     // occupancy update (nullptr: meas already holds the grid).

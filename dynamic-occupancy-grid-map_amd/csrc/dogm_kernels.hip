@@ -513,6 +513,7 @@ __device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int z
             for (int f = GF_MEAN_X; f <= GF_COVAR; ++f)
                 d.f[f][c] = 0.0f;
     }
+    const float m_raw_c = m_occ_pred;  // Σw of the cell before the clamp
     if (m_occ_pred > one)
         m_occ_pred = one;
     float fprev;
@@ -536,6 +537,8 @@ __device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int z
     const dogm_measurement_cell mc = d.meas_in ? d.meas_in[c] : d.meas[c];
     if (d.meas_in)
         d.meas[c] = mc;
+    if ((zero_empty & 1) && occupied)
+        d.cellin[c] = make_float2(mc.likelihood, m_raw_c);
     const float pB = d.pB;
     const float m_free_pred = fminf(d.alpha * fprev, one - m_occ_pred);
     const float unknown_pred = one - m_occ_pred - m_free_pred;
@@ -637,28 +640,40 @@ struct PersOp
     __device__ void cont(const VehDesc&, int) const {}
     __device__ void load(const VehDesc& d, int p0, int cnt, const int (&key)[ITEMS], float (&v)[ITEMS][NCH]) const
     {
-        float w[ITEMS], lw[ITEMS];
+        float w[ITEMS], lw[ITEMS], lik[ITEMS];
         load_run<ITEMS>(d.part[pph].weight, p0, cnt, w);
         if (norm)
         {
+            // The occupancy update of this step left (likelihood, Σw) per cell with particles; every
+            // item's pair is loaded before the first store.
+            float m[ITEMS];
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i)
             {
-                if (i < cnt)
+                const float2 ci = i < cnt ? d.cellin[key[i]] : make_float2(0.0f, 0.0f);
+                lik[i] = ci.x;
+                m[i] = ci.y;
+            }
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+            {
+                if (i < cnt && m[i] > 1.0f)
                 {
-                    const float m = d.m_raw[key[i]];
-                    if (m > 1.0f)
-                    {
-                        w[i] = w[i] / m;
-                        d.part[pph].weight[p0 + i] = w[i];
-                    }
+                    w[i] = w[i] / m[i];
+                    d.part[pph].weight[p0 + i] = w[i];
                 }
             }
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+                lik[i] = i < cnt ? d.meas[key[i]].likelihood : 0.0f;
         }
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
         {
-            lw[i] = i < cnt ? d.meas[key[i]].likelihood * w[i] : 0.0f;
+            lw[i] = i < cnt ? lik[i] * w[i] : 0.0f;
             v[i][0] = lw[i];
         }
         store_run<ITEMS>(d.weight_array, p0, cnt, lw);
