@@ -684,6 +684,18 @@ constexpr int SCAN_Q = SCAN_TILE / UNIT;  // units per scan tile
 // these into the vehicle's inclusive segment ends: the resampling CDF index).
 static_assert(CDF_SEG == WAVE * SCAN_ITEMS && CDF_SEGS_PER_TILE * CDF_SEG == SCAN_TILE, "CDF segments are waves");
 
+// The N + 1 exponentials as a scan source (tile totals only); systematic draws need none.
+struct SrcExp
+{
+    static constexpr bool kNonNeg = true;  // -log(u) >= 0 (or -0)
+    __device__ bool active(const VehDesc& d) const { return d.draw_mode == DRAW_MULTINOMIAL; }
+    __device__ int len(const VehDesc& d) const { return d.N + 1; }
+    __device__ void load4(const VehDesc& d, int p0, int, float (&v)[4]) const
+    {
+        resample_exponentials4(d.seed, d.inst, d.ep_rs, (uint32_t)(p0 >> 2), v);
+    }
+};
+
 // Tile `tile` of a space: the tile total into tile_sum[tile] (and the segment ends, see above).
 template <class Src>
 __device__ __forceinline__ void scan_tile_body(const Src& src, const VehDesc* __restrict__ descs,

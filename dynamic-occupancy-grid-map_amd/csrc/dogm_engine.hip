@@ -694,6 +694,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
     const bool assign_occ = (stages & ST_ASSIGN) && (stages & ST_OCC);
     const bool pers_mom = (stages & ST_PERS) && (stages & ST_MOMENTS);
     const bool occ_birth = (stages & ST_OCC) && (stages & ST_BIRTH);
+    const bool occ_resample = (stages & ST_OCC) && (stages & ST_RESAMPLE);
     const int zero_empty = (stages & ST_OCC) ? (((stages & ST_PERS) ? 1 : 0) | ((stages & ST_MOMENTS) ? 2 : 0)) : 0;
     StageRange step_range(replay ? "dogm step (graph replay)" : "dogm step");
     if (!replay)
@@ -724,7 +725,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (stages & ST_OCC)
         {
             StageRange r("gridCellOccupancyUpdate");
-            run_occupancy(E, pph, zero_empty, occ_pers, assign_occ, occ_birth);
+            run_occupancy(E, pph, zero_empty, occ_pers, assign_occ, occ_birth, occ_resample);
         }
         if (stages & ST_PERS)
         {
@@ -747,7 +748,7 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
         if (!replay)
         {
             StageRange r("resampling");
-            run_resampling(E, pph, modes);
+            run_resampling(E, pph, modes, occ_resample);
         }
         pph ^= 1;
     }

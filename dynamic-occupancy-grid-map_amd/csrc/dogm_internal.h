@@ -574,8 +574,10 @@ void run_predict(Engine* E, int pph, bool hist, bool aos);
 void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds, bool write_wa);  // reads pph, writes pph ^ 1
 // born_sums: the occupancy update leaves the born masses' exact sum per cell unit for the births of the
 // same step (born_unit_sums), whose scan then skips its tile pass.
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds,
-                   bool born_sums);  // free[0] -> free[1]
+// exp_sums: the same launch computes the resampling exponentials' tile totals (exp_tile_sum) for the
+// resampling of the same step (exp_done there).
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds, bool born_sums,
+                   bool exp_sums);  // free[0] -> free[1]
 void run_persistent(Engine* E, int pph, bool fused_normalize, bool empty_done, bool defer_post);
 void run_births(Engine* E, bool born_sums);
 // Unit sums of the born masses between the occupancy update and the births of a step: the CDF index
@@ -584,7 +586,7 @@ void run_births(Engine* E, bool born_sums);
 inline fx_t* born_unit_sums(Engine* E) { return E->cdf_tile_incl; }
 void run_moments(Engine* E, int pph, bool fused_post, bool empty_done);
 // modes: bit m set when a vehicle of the step resamples with CDF mode m (dogm_num::CdfMode).
-void run_resampling(Engine* E, int pph, int modes);  // reads pph, writes pph ^ 1
+void run_resampling(Engine* E, int pph, int modes, bool exp_done);  // reads pph, writes pph ^ 1
 // Flags of the gathering sort pass.
 constexpr int GATHER_AOS = 1;  // the state is in the CDF buffer as 16-byte records (predict, same step)
 constexpr int GATHER_WA = 2;   // also write the weight copy into weight_array

@@ -561,11 +561,20 @@ __device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int z
 
 // One workgroup per 256-cell unit.  With born_sums set (the births run in the same step), the unit's
 // exact sum of born masses goes to born_sums[unit] for the births' top-level scan, which then needs no
-// tile pass over the born masses.
+// tile pass over the born masses.  With exp_sums set (the resampling runs in the same step), the
+// workgroups after the `units` cell units compute the tile totals of the resampling's exponentials
+// (ALU work beside the memory-bound cell update) into exp_sums.
 __global__ __launch_bounds__(BLOCK) void k_mass_update(const VehDesc* __restrict__ descs,
                                                        const uint32_t* __restrict__ tab, int zero_empty,
-                                                       SegCarry carry, int occ_tile, fx_t* __restrict__ born_sums)
+                                                       SegCarry carry, int occ_tile, fx_t* __restrict__ born_sums,
+                                                       int units, const uint32_t* __restrict__ tab_e,
+                                                       fx_t* __restrict__ exp_sums)
 {
+    if ((int)blockIdx.x >= units)
+    {
+        scan_tile_body(SrcExp{}, descs, tab_e, (int)blockIdx.x - units, exp_sums, (fx_t*)nullptr);
+        return;
+    }
     const TileRef tr = tile_ref(tab, 1, blockIdx.x);
     const VehDesc& d = descs[tr.v];
     const int c = tr.base + threadIdx.x;
@@ -600,15 +609,19 @@ __global__ __launch_bounds__(BLOCK) void k_normalize(const VehDesc* __restrict__
     }
 }
 
-void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds, bool born_sums)
+void run_occupancy(Engine* E, int pph, int zero_empty, bool defer_normalize, bool bounds, bool born_sums,
+                   bool exp_sums)
 {
     if (E->tune.seg_items == 4)
         launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<4>{pph, bounds ? 1 : 0}, false);
     else
         launch_segreduce(E, K_SEG_OCC, K_SEG_OCC_FIX, OccOp<8>{pph, bounds ? 1 : 0}, false);
     launch_begin(E, K_MASS_UPDATE);
-    hipLaunchKernelGGL(k_mass_update, dim3(E->tiles(SP_C, 1)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
-                       zero_empty, E->seg, BLOCK * E->tune.seg_items, born_sums ? born_unit_sums(E) : (fx_t*)nullptr);
+    const int units = E->tiles(SP_C, 1);
+    const int etiles = exp_sums ? E->tiles(SP_E, SCAN_Q) : 0;
+    hipLaunchKernelGGL(k_mass_update, dim3(units + etiles), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C),
+                       zero_empty, E->seg, BLOCK * E->tune.seg_items, born_sums ? born_unit_sums(E) : (fx_t*)nullptr,
+                       units, E->tab(SP_E), E->exp_tile_sum);
     launch_end(E, K_MASS_UPDATE);
     if (defer_normalize)
         return;

@@ -71,18 +71,6 @@ __device__ __forceinline__ void exp4(const VehDesc& d, int p0, int n, fx_t* e)
         e[i] = p0 + i < n ? fx_from_f32_nonneg(f[i]) : (fx_t)0;  // -log(u) >= 0 (or -0)
 }
 
-// The N + 1 exponentials as a scan source (tile totals only); systematic draws need none.
-struct SrcExp
-{
-    static constexpr bool kNonNeg = true;  // -log(u) >= 0 (or -0)
-    __device__ bool active(const VehDesc& d) const { return d.draw_mode == DRAW_MULTINOMIAL; }
-    __device__ int len(const VehDesc& d) const { return d.N + 1; }
-    __device__ void load4(const VehDesc& d, int p0, int, float (&v)[4]) const
-    {
-        resample_exponentials4(d.seed, d.inst, d.ep_rs, (uint32_t)(p0 >> 2), v);
-    }
-};
-
 // ---------------------------------------------------------------------------------------------
 // Float CDF modes (precision study).  Per tile of 1024 joint weights: 4-element sequential folds
 // per thread, the 64-lane DPP wave scan, sequential carries across the 4 waves, and a sequential
@@ -746,11 +734,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top_resample(const VehDesc* __re
     scan_top_body(descs, SP_E, exp_sum, exp_pre, (fx_t*)nullptr, scs, SCAN_EXP, ONLY_ALL);
 }
 
-void run_resampling(Engine* E, int pph, int modes)
+void run_resampling(Engine* E, int pph, int modes, bool exp_done)
 {
     const bool exact = modes & (1 << CDF_EXACT);
     const int jtiles = E->tiles(SP_J, SCAN_Q);
-    const int etiles = E->tiles(SP_E, SCAN_Q);
+    const int etiles = exp_done ? 0 : E->tiles(SP_E, SCAN_Q);
     // Tile totals of the joint CDF (exact-CDF vehicles, with the tile-local segment ends of the coarse
     // index; the merge recomputes entries) and of the exponentials (multinomial draws) in one launch,
     // then both top-level scans in one launch.
