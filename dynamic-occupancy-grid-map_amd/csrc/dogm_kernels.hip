@@ -222,14 +222,51 @@ __device__ __forceinline__ int clamp_cell(float x, int gs)
 // vehicle's CDF buffer, free until the resampling, as 16-byte (x, y, vx, vy) records, so that the
 // gather reads one 16-byte record per particle at random instead of two 8-byte ones from two arrays;
 // nothing else reads the positions or velocities in between.
+// start = end = -1 for the cells of work-table group `group` (THREADS x 4 cells), four cells per
+// thread with 16-byte stores.  With valid occupancy bits only groups of four containing a cell that
+// had particles are written (the others still hold -1, and rewriting -1 is harmless).
+template <int THREADS>
+__device__ __forceinline__ void reset_bounds_cells(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
+                                                   int group)
+{
+    const TileRef tr = tile_ref(tab, THREADS * 4 / UNIT, group);
+    const VehDesc& d = descs[tr.v];
+    const int c = tr.base + 4 * threadIdx.x;
+    if (c >= d.C)
+        return;
+    if (d.nz_valid && ((d.nz[c >> 6] >> (c & 63)) & 0xfull) == 0)
+        return;
+    if (c + 4 <= d.C)
+    {
+        const int4 m1 = make_int4(-1, -1, -1, -1);
+        *reinterpret_cast<int4*>(d.start + c) = m1;
+        *reinterpret_cast<int4*>(d.end + c) = m1;
+    }
+    else
+    {
+        for (int k = c; k < d.C; ++k)
+        {
+            d.start[k] = -1;
+            d.end[k] = -1;
+        }
+    }
+}
+
 template <int PRED_BLOCK>
 __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restrict__ descs, const uint32_t* __restrict__ tab,
                                                    int pph, int passes, unsigned* __restrict__ ghist,
                                                    unsigned* __restrict__ tickets, unsigned* __restrict__ thist,
-                                                   int tiles, int aos)
+                                                   int tiles, int aos, const uint32_t* __restrict__ tab_c)
 {
     constexpr int PRED_ITEMS = SORT_TILE / PRED_BLOCK;
     static_assert(PRED_ITEMS * PRED_BLOCK == SORT_TILE, "predict workgroups are the sort's tiles");
+    if ((int)blockIdx.x >= tiles)
+    {
+        // The assignment's bounds reset, in workgroups after the particle tiles (memory work beside
+        // the ALU-bound prediction).
+        reset_bounds_cells<PRED_BLOCK>(descs, tab_c, (int)blockIdx.x - tiles);
+        return;
+    }
     __shared__ unsigned s_h[4][RADIX];
     if (passes > 0 && blockIdx.x == 0 && threadIdx.x < 4)
         tickets[threadIdx.x] = 0u;
@@ -334,46 +371,26 @@ void run_predict(Engine* E, int pph, bool hist, bool aos)
     E->ghist_clean = false;  // until the sort of this step has cleared them again
     unsigned* thist = E->tune.sort == SORT_REDUCE_SCAN ? E->sort_status : nullptr;
     const int passes = hist ? E->passes : 0;
+    // With the assignment in the same step (hist), its bounds reset runs in extra workgroups.
+    const int pb = E->tune.pred_block == 256 ? 256 : 512;
+    const int resets = hist ? E->tiles(SP_C, pb * 4 / UNIT) : 0;
     launch_begin(E, K_PREDICT);
-    if (E->tune.pred_block == 256)
-        hipLaunchKernelGGL(k_predict<256>, dim3(blocks), dim3(256), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0);
+    if (pb == 256)
+        hipLaunchKernelGGL(k_predict<256>, dim3(blocks + resets), dim3(256), 0, E->stream, E->d_desc, E->tab(SP_P),
+                           pph, passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0, E->tab(SP_C));
     else
-        hipLaunchKernelGGL(k_predict<512>, dim3(blocks), dim3(512), 0, E->stream, E->d_desc, E->tab(SP_P), pph,
-                           passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0);
+        hipLaunchKernelGGL(k_predict<512>, dim3(blocks + resets), dim3(512), 0, E->stream, E->d_desc, E->tab(SP_P),
+                           pph, passes, E->sort_ghist, E->sort_tickets, thist, blocks, aos ? 1 : 0, E->tab(SP_C));
     launch_end(E, K_PREDICT);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Assignment (dogm/src/dogm.cu:262-281, init.cu:86-93, particle_to_grid.cu:16-44)
 // ---------------------------------------------------------------------------------------------
-// start = end = -1 for every cell, four cells per thread with 16-byte stores.  With valid occupancy
-// bits only groups of four containing a cell that had particles are written (the others still hold
-// -1, and rewriting -1 is harmless).
 __global__ __launch_bounds__(BLOCK) void k_reset_bounds(const VehDesc* __restrict__ descs,
                                                         const uint32_t* __restrict__ tab)
 {
-    const TileRef tr = tile_ref(tab, 4, blockIdx.x);
-    const VehDesc& d = descs[tr.v];
-    const int c = tr.base + 4 * threadIdx.x;
-    if (c >= d.C)
-        return;
-    if (d.nz_valid && ((d.nz[c >> 6] >> (c & 63)) & 0xfull) == 0)
-        return;
-    if (c + 4 <= d.C)
-    {
-        const int4 m1 = make_int4(-1, -1, -1, -1);
-        *reinterpret_cast<int4*>(d.start + c) = m1;
-        *reinterpret_cast<int4*>(d.end + c) = m1;
-    }
-    else
-    {
-        for (int k = c; k < d.C; ++k)
-        {
-            d.start[k] = -1;
-            d.end[k] = -1;
-        }
-    }
+    reset_bounds_cells<BLOCK>(descs, tab, blockIdx.x);
 }
 
 // Gathers the payload into sorted order and records first/last occurrence of every cell.
@@ -407,9 +424,13 @@ __global__ __launch_bounds__(BLOCK) void k_sort_gather(const VehDesc* __restrict
 
 void run_assignment(Engine* E, int pph, bool hist_done, bool defer_bounds, bool write_wa)
 {
-    launch_begin(E, K_RESET_BOUNDS);
-    hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 4)), dim3(BLOCK), 0, E->stream, E->d_desc, E->tab(SP_C));
-    launch_end(E, K_RESET_BOUNDS);
+    if (!hist_done)  // else predict's launch reset the bounds
+    {
+        launch_begin(E, K_RESET_BOUNDS);
+        hipLaunchKernelGGL(k_reset_bounds, dim3(E->tiles(SP_C, 4)), dim3(BLOCK), 0, E->stream, E->d_desc,
+                           E->tab(SP_C));
+        launch_end(E, K_RESET_BOUNDS);
+    }
     run_sort(E, pph, hist_done, defer_bounds, write_wa || !defer_bounds);
     if (defer_bounds)
         return;  // the last sort pass wrote the particles; the occupancy reduction records the bounds
