@@ -318,11 +318,13 @@ struct CdfStage<fx_t, LEN>
         const int s = cdf_slot(i);
         return (fx_t)(((ufx_t)(uint64_t)hi[s] << 64) | (ufx_t)lo[s]);
     }
+    // Both words are read at once (one LDS round trip per probe) and compared without a branch.
     __device__ bool less(int i, const fx_t& r) const
     {
         const int s = cdf_slot(i);
         const int64_t h = hi[s], rh = (int64_t)(r >> 64);
-        return h != rh ? h < rh : lo[s] < (uint64_t)r;
+        const uint64_t l = lo[s];
+        return (h < rh) | ((h == rh) & (l < (uint64_t)r));
     }
 };
 
@@ -575,12 +577,13 @@ void k_rs_merge_exact(const VehDesc* __restrict__ descs, const uint32_t* __restr
             if (anc[i] < 0 && base + (int)threadIdx.x + i * BLOCK < n && !(cmax < r[i]))
             {
                 // lo counts the entries < r[i]: after the probe at lo + step - 1 the count lies in
-                // [lo, lo + step); entries at clen and beyond count as +infinity.
+                // [lo, lo + step).  Entries at clen and beyond count as +infinity; probing the last
+                // entry (>= r[i]) in their place gives the same answer without a branch.
                 int lo = 0;
                 for (int step = span >> 1; step > 0; step >>= 1)
                 {
-                    const int j = lo + step - 1;
-                    if (j < clen && s_cdf.less(j, r[i]))
+                    const int j = min(lo + step, clen) - 1;
+                    if (s_cdf.less(j, r[i]))
                         lo += step;
                 }
                 anc[i] = c0 + lo;
