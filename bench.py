@@ -47,9 +47,10 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
     """Algorithmic bytes one launch must move per vehicle (read once what it consumes, write once what
     it hands on); gathers through cell indices count their payload once."""
     table = {
-        # state in, state + idx out (the prior weights are equal after every resampling, so predict
-        # leaves them in place and the gathering pass forms the predicted weights)
-        "predict": 36 * N,
+        # ancestor index, its 16 B state and association in; state, idx and association out (the
+        # resampling leaves the new set as ancestors, read here; the prior weights are equal after every
+        # resampling, so predict leaves them in place and the gathering pass forms the predicted weights)
+        "predict": 42 * N,
         "sort_downsweep": 16 * N,                # a pass before the last: keys + values in and out
         # the last pass: keys + values in, the 16 B state gathered, sorted particles (25 B) out (the
         # weight copy is skipped: the persistent update of the same step rewrites it)
@@ -60,7 +61,7 @@ def kernel_bytes(name: str, N: int, B: int, C: int, passes: int) -> float | None
         "mass_update": 72 * C,
         "seg_pers": 12 * N,                      # keys + weights in, likelihood-weighted weights out
         "seg_moments": 24 * N,                   # keys, lik-weighted and prior weights, vx, vy in; posterior out
-        "rs_merge": 50 * N + 4 * B,              # joint weights, 21 B ancestor payload in; 25 B particles out
+        "rs_merge": 8 * N + 4 * B,               # joint weights in; ancestor indices out
         "birth_particles": 25 * B,
         "birth_cells": 20 * C,
         "reset_bounds": 8 * C,

@@ -152,6 +152,8 @@ int state_staging(Engine* E, Vehicle* v, int n, float4** out)
 int dogm_hip::read_particles(Engine* E, Vehicle* v, const DevParticles& p, int n, dogm_vec4* state, int* idx,
                              float* weight, uint8_t* assoc)
 {
+    // A deferred resampled set is written out first (the state staging below reuses part[1]).
+    vehicle_materialize(E, v);
     int st;
     if (state)
     {
@@ -178,6 +180,7 @@ int dogm_hip::write_particles(Engine* E, Vehicle* v, const DevParticles& p, int 
                 return fail(E, DOGM_ERR_INVALID_ARGUMENT,
                             "grid_cell_idx[" + std::to_string(i) + "] = " + std::to_string(idx[i]) +
                                 " is outside [0, grid_size^2)");
+    vehicle_materialize(E, v);  // the staging reuses part[1], and a deferred set reads the birth set
     int st;
     if (state)
     {

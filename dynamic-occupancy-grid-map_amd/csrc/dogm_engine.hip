@@ -22,6 +22,7 @@ const char* const kKernelNames[K_COUNT] = {
     "seg_moments_fix", "cdf_write",      "exp_tiles",       "rs_merge",        "fill",            "pack_grid",
     "unpack_grid",    "build_tables",    "extract_mark",    "extract_scan",    "extract_compact",
     "extract_gather", "ego_particles",   "ego_field",       "pack_state",      "unpack_state",
+    "materialize",
 };
 
 // ---- timing ------------------------------------------------------------------------------------
@@ -522,7 +523,10 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         run_build_tables(E);
     }
 
-    // Per-step descriptors.
+    // Per-step descriptors.  A deferred resampled set is read through its ancestors by a prediction
+    // that the sort and the occupancy update follow in the same step (engine_run writes it out before
+    // any other step that reads particles).
+    const bool full_predict = (stages & ST_PREDICT) && (stages & ST_ASSIGN) && (stages & ST_OCC);
     for (int i = 0; i < m; ++i)
     {
         Vehicle* v = veh[i];
@@ -537,6 +541,8 @@ int engine_plan(Engine* E, Vehicle* const* veh, int m, int stages)
         uint32_t e = v->epoch;
         d.init = (stages & ST_INIT) && v->init_pending ? 1 : 0;
         d.wuni = v->w_uniform || d.init ? 1 : 0;  // initialisation sets every weight to 1 / N
+        d.rs_gather = v->rs_pending && !d.init && full_predict ? 1 : 0;
+        d.rs_defer = (stages & ST_RESAMPLE) ? 1 : 0;
         d.ep_init = e;
         e += d.init ? 1u : 0u;
         d.ep_pred = e;
@@ -644,6 +650,23 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
             const int cs = vehicle_set_measurement(E, v, src, 1);
             if (cs != DOGM_OK)
                 return cs;
+        }
+    }
+    // Deferred resampled sets (Vehicle::rs_pending): dropped where this step initialises the particles,
+    // read through their ancestors by a full step's prediction, written out before any other stage
+    // that reads or writes particles.
+    {
+        const bool full_predict = (stages & ST_PREDICT) && (stages & ST_ASSIGN) && (stages & ST_OCC);
+        const int particle_stages = ST_PREDICT | ST_ASSIGN | ST_OCC | ST_PERS | ST_BIRTH | ST_MOMENTS | ST_RESAMPLE;
+        for (int i = 0; i < m; ++i)
+        {
+            Vehicle* v = veh[i];
+            if (!v->rs_pending)
+                continue;
+            if ((stages & ST_INIT) && v->init_pending)
+                v->rs_pending = false;  // initialisation writes every particle
+            else if (!full_predict && (stages & particle_stages))
+                vehicle_materialize(E, v);
         }
     }
     if (!E->tune_locked)
@@ -800,6 +823,11 @@ int engine_run(Engine* E, Vehicle* const* veh, int m, int stages)
             v->w_uniform = false;
         else
             v->w_uniform = d.wuni != 0;
+        // A resampling leaves a deferred set (merge_write); a prediction that read one consumed it.
+        if (d.rs_defer)
+            v->rs_pending = true;
+        else if (d.rs_gather)
+            v->rs_pending = false;
         if (pph)
             std::swap(v->part[0], v->part[1]);
         if (stages & ST_OCC)
@@ -988,7 +1016,10 @@ static void shift_grid_eager(Engine* E, Vehicle* v, int ix, int iy)
 static void flush_pending_xform(Engine* E, Vehicle* v)
 {
     if (v->pend_xp)
+    {
+        vehicle_materialize(E, v);
         run_ego_particles(E, v, v->pend_xt);
+    }
     if (v->pend_xg)
     {
         run_ego_field(E, v, v->grid.f[GF_FREE], v->field_tmp, v->pend_xt);
@@ -1033,6 +1064,7 @@ static void ego_metric_update(Engine* E, Vehicle* v, float new_x, float new_y, f
         v->pend_xt = t;
         return;
     }
+    vehicle_materialize(E, v);
     run_ego_particles(E, v, t);
     auto xform = [&](void** field) {
         run_ego_field(E, v, *field, v->field_tmp, t);
@@ -1083,6 +1115,7 @@ void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float 
         }
         else
         {
+            vehicle_materialize(E, v);
             run_move_particles(E, v, (float)x_move, (float)y_move);
             shift_grid_eager(E, v, x_move, y_move);
         }
@@ -1092,6 +1125,14 @@ void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float 
     }
 }
 
+
+void vehicle_materialize(Engine* E, Vehicle* v)
+{
+    if (!v->rs_pending)
+        return;
+    run_materialize(E, v);
+    v->rs_pending = false;
+}
 
 int vehicle_validate(const dogm_params* p, std::string* why)
 {

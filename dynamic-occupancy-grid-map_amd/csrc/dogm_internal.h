@@ -114,6 +114,8 @@ struct VehDesc
     float dt;
     int init;                // first measurement: particles are initialised in this step
     int wuni;                // every weight predict reads equals weight[0] (resampled or initialised)
+    int rs_gather;           // part[0] is a deferred resampled set: predict reads it through ancestors
+    int rs_defer;            // the resampling writes ancestors and weight[0] only (a deferred set)
     int move;                // pending particle shift (fused into predict)
     float mx, my;
     int shifted;             // pending grid shift (fused into the occupancy update)
@@ -226,6 +228,7 @@ enum KernelId
     K_EGO_FIELD,
     K_PACK_STATE,
     K_UNPACK_STATE,
+    K_MATERIALIZE,
     K_COUNT
 };
 
@@ -252,6 +255,11 @@ struct Vehicle
     uint64_t uid = 0;    // unique per allocation (work tables are reused while the list is unchanged)
     bool init_pending = false;  // initialise particles in the next step that includes ST_INIT
     bool w_uniform = false;     // every weight of part[0] equals part[0].weight[0]
+    // part[0] is the resampled set by reference only: particle k is particle ancestors[k] of part[1]
+    // (ancestors[k] < N) or of the birth set (ancestors[k] - N), with weight part[0].weight[0].  The
+    // next step's predict reads it that way; anything else that reads or writes part[0], part[1] or
+    // the birth set first writes it out (vehicle_materialize).
+    bool rs_pending = false;
     int cdf_mode = 0;           // resampling CDF arithmetic (dogm_num::CdfMode)
     int draw_mode = 0;          // resampling draws (dogm_num::DrawMode)
     float dt = 0.0f;            // time step of the next step
@@ -515,6 +523,8 @@ int vehicle_take_measurement(Engine* E, Vehicle* v, const dogm_measurement_cell*
 // dogm.cu:161-203.  lazy: the shift is applied inside the next step's predict (particles) and
 // occupancy update (free mass); otherwise at once.
 void vehicle_update_pose(Engine* E, Vehicle* v, float new_x, float new_y, float new_yaw, bool lazy);
+// Writes out v's deferred resampled set, if any (Vehicle::rs_pending), on E's stream.
+void vehicle_materialize(Engine* E, Vehicle* v);
 
 // One updateGrid frame for each of `m` distinct vehicles in one step.
 struct FrameIn
@@ -594,6 +604,8 @@ constexpr int GATHER_WA = 2;   // also write the weight copy into weight_array
 void run_sort(Engine* E, int pph, bool hist_done, bool gather, bool write_wa);
 // Single-vehicle kernels (no tables): eager ego shift and grid packing.
 void run_move_particles(Engine* E, const Vehicle* v, float mx, float my);
+// Writes out a deferred resampled set (Vehicle::rs_pending) into part[0].
+void run_materialize(Engine* E, const Vehicle* v);
 void run_ego_particles(Engine* E, const Vehicle* v, const EgoXform& t);
 void run_ego_field(Engine* E, const Vehicle* v, const void* src, void* dst, const EgoXform& t);
 void run_shift_field(Engine* E, const Vehicle* v, const void* src, void* dst, int ix, int iy);

@@ -287,6 +287,8 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
         __syncthreads();
     }
     const DevParticles P = d.part[pph];
+    const DevParticles Q = d.part[pph ^ 1];  // sources of a deferred resampled set
+    const DevParticles Bq = d.birth;
     const int gs = d.gs;
     const float dt = d.dt;
     const float zero = 0.0f, one = 1.0f;
@@ -297,9 +299,28 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
         int cell = 0;
         if (valid)
         {
-            const float2 ps = P.pos[i], vs = P.vel[i];
+            float2 ps, vs;
+            if (d.rs_gather)
+            {
+                // A deferred resampled set (see merge_write): the state of the particle's ancestor
+                // (part[pph ^ 1] below N, the birth set above), and its association flag written here
+                // for the sort.
+                const int a = d.ancestors[i];
+                const bool from_p = a < n;
+                const int o = from_p ? a : a - n;
+                ps = (from_p ? Q.pos : Bq.pos)[o];
+                vs = (from_p ? Q.vel : Bq.vel)[o];
+                P.assoc[i] = (from_p ? Q.assoc : Bq.assoc)[o];
+            }
+            else
+            {
+                ps = P.pos[i];
+                vs = P.vel[i];
+            }
             float4 s = make_float4(ps.x, ps.y, vs.x, vs.y);
-            const bool keep_w = aos && d.wuni;  // the gathering pass forms the weight
+            // The gathering pass forms the weight (always so for a deferred set: the host reads one
+            // through its ancestors only in full steps, whose weights are uniform).
+            const bool keep_w = aos && d.wuni;
             const float w_in = keep_w ? 0.0f : P.weight[i];
             if (d.move)
             {

@@ -436,7 +436,9 @@ __device__ __forceinline__ int merge_next_draw(int base, int n, const int (&anc)
 
 // The resampled particles k = base + threadIdx.x + i * BLOCK: ancestor index, state copied from the
 // ancestor (posterior particle or new-born one) and the uniform new weight.  A draw without an
-// ancestor (above the CDF total by rounding) takes the last joint entry.
+// ancestor (above the CDF total by rounding) takes the last joint entry.  A deferred set
+// (d.rs_defer) gets the ancestors and weight[0] only: the next prediction reads the states through
+// the ancestors, and every other reader first writes the set out (k_materialize).
 __device__ __forceinline__ void merge_write(const VehDesc& d, const DevScalars* sc, int pph, int base, int n, int J,
                                             const int (&anc)[RS_ITEMS])
 {
@@ -452,6 +454,12 @@ __device__ __forceinline__ void merge_write(const VehDesc& d, const DevScalars* 
             break;
         const int a = anc[i] < 0 ? J - 1 : min(anc[i], J - 1);
         d.ancestors[k] = a;
+        if (d.rs_defer)
+        {
+            if (k == 0)
+                out.weight[0] = nw;
+            continue;
+        }
         const bool from_p = a < n;
         const int o = from_p ? a : a - n;
         const float2* ps = from_p ? P.pos : Bp.pos;
@@ -726,6 +734,37 @@ void k_rs_merge_float(const VehDesc* __restrict__ descs, const uint32_t* __restr
 // The resampling's two top-level scans in one launch, in their order (the spacings read the CDF
 // total): the joint CDF of exact-CDF vehicles (segment ends in place, no tile prefixes), then the
 // exponentials of every vehicle (tile prefixes into exp_pre).
+// Writes out a deferred resampled set (see merge_write): particle k of dst copied from its ancestor
+// (particle anc[k] of src below n, birth particle anc[k] - n above) with the new weight, which the
+// resampling left in dst.weight[0].
+__global__ __launch_bounds__(BLOCK) void k_materialize(DevParticles dst, DevParticles src, DevParticles birth,
+                                                       const int* __restrict__ anc, int n)
+{
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n)
+        return;
+    const int a = anc[k];
+    const bool from_p = a < n;
+    const int o = from_p ? a : a - n;
+    const DevParticles& s = from_p ? src : birth;
+    dst.pos[k] = s.pos[o];
+    dst.vel[k] = s.vel[o];
+    dst.idx[k] = s.idx[o];
+    dst.assoc[k] = s.assoc[o];
+    if (k > 0)
+        dst.weight[k] = dst.weight[0];
+}
+
+void run_materialize(Engine* E, const Vehicle* v)
+{
+    if (v->N <= 0)
+        return;
+    launch_begin(E, K_MATERIALIZE);
+    hipLaunchKernelGGL(k_materialize, dim3(div_up(v->N, BLOCK)), dim3(BLOCK), 0, E->stream, v->part[0], v->part[1],
+                       v->birth, v->ancestors, v->N);
+    launch_end(E, K_MATERIALIZE);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_scan_top_resample(const VehDesc* __restrict__ descs,
                                                              const fx_t* __restrict__ cdf_sum,
                                                              fx_t* __restrict__ cdf_incl,

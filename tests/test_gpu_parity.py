@@ -293,6 +293,66 @@ def test_full_updates_after_setting_unequal_weights():
         assert_state_equal(f"frame {f}", gpu, orc, ancestors=True, births=True)
 
 
+def test_deferred_resampled_sets():
+    """A resampling leaves the new particle set as ancestor indices, which the next full update's
+    prediction reads the states through; every other reader or writer of the particles first writes
+    the set out.  Full updates run back to back without particle read-backs (the prediction path),
+    with a birth read-back, an eager pose update, a stage-by-stage frame, a particle write and a
+    particle read-back between them; every array matches the checker."""
+    p = small_params(gs=64, res=0.25, n=20000, b=2000)
+    sc = small_scene(gs=64, res=0.25, n_boxes=6, n_beams=240, ego_speed=5.0, yaw_rate=0.2, seed=13)
+    gpu = DOGM(p, seed=17)
+    orc = OracleDOGM(p, seed=17)
+    rng = np.random.default_rng(8)
+    f = 0
+
+    def full(check_particles=False):
+        nonlocal f
+        _say(f"deferred sets frame {f}")
+        m = sc.measurement(f)
+        x, y, yaw = sc.ego_pose(f)
+        gpu.update_grid(m, x, y, yaw, 0.1)
+        orc.update_grid(m, x, y, yaw, 0.1)
+        assert_state_equal(f"frame {f}", gpu, orc, ancestors=True, particles=check_particles)
+        f += 1
+
+    for _ in range(4):
+        full()
+    full(check_particles=True)
+    full()
+    _say("birth read-back")
+    assert_particles_equal("births", gpu.get_birth_particles(), orc.get_birth_particles())
+    full()
+    full()
+    _say("eager pose update")
+    x, y, yaw = sc.ego_pose(f)
+    gpu.update_pose(x + 1.0, y - 0.75, yaw)
+    orc.update_pose(x + 1.0, y - 0.75, yaw)
+    full()
+    _say("stage-by-stage frame")
+    m = sc.measurement(f)
+    x, y, yaw = sc.ego_pose(f)
+    for o in (gpu, orc):
+        o.update_measurement_grid(m)
+        o.update_pose(x, y, yaw)
+        for name, args in STAGES:
+            getattr(o, name)(*args)
+    assert_state_equal(f"stages frame {f}", gpu, orc, ancestors=True, particles=False)
+    f += 1
+    full()
+    _say("particle write")
+    w = (rng.uniform(0.0, 2.0, p.particle_count) / p.particle_count).astype(np.float32)
+    gpu.set_particles(weight=w)
+    orc.set_particles(weight=w)
+    full()
+    full()
+    _say("metric ego mode")
+    gpu.set_ego_mode("metric")
+    orc.set_ego_mode("metric")
+    full()
+    full(check_particles=True)
+
+
 def test_hostile_weights_in_resampling():
     """Resampling from injected weights with negative values, NaN, zeros, subnormals and a wide
     dynamic range (persistent and birth weights): negative and NaN weights count as zero in the CDF on
