@@ -15,8 +15,8 @@ def test_roofline_kernel_rows_are_found():
     name = bench.PROFILED_NAME["rs_merge"]
     traffic = profiling.lookup_traffic(str(SUMMARY), name)
     grid, valu = profiling.lookup_valu(str(SUMMARY), name)
-    # at least the algorithmic bytes of 32 vehicles (50 N + 4 B each)
-    assert traffic and traffic > 32 * (50 * 2_000_000 + 4 * 200_000)
+    # at least the algorithmic bytes of 32 vehicles
+    assert traffic and traffic > 32 * bench.kernel_bytes("rs_merge", 2_000_000, 200_000, 1_440_000, 3)
     assert grid == 32 * 500224 and 500 < valu < 5000
 
 
