@@ -140,6 +140,31 @@ __device__ __forceinline__ fx_t block_exclusive_scan_fx(fx_t v, fx_t* total)
     return carry + incl - v;
 }
 
+// Stores of streaming outputs, which no later work of the same launch reads and which are far out of
+// the caches by the time the next stage reads them: vector stores with the nontemporal hint.
+template <class T>
+__device__ __forceinline__ void stream_store(T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+__device__ __forceinline__ void stream_store(float2* p, float2 v)
+{
+    using V = float __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(V{v.x, v.y}, reinterpret_cast<V*>(p));
+}
+
+__device__ __forceinline__ void stream_store(float4* p, float4 v)
+{
+    using V = float __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(V{v.x, v.y, v.z, v.w}, reinterpret_cast<V*>(p));
+}
+
+__device__ __forceinline__ void stream_store(dogm_measurement_cell* p, dogm_measurement_cell v)
+{
+    stream_store(reinterpret_cast<float4*>(p), make_float4(v.free_mass, v.occ_mass, v.likelihood, v.p_A));
+}
+
 // Predicted weight of a particle (dogm/src/kernel/predict.cu:34-43): persistence probability times
 // the prior weight, 0 once the predicted position o leaves the grid.  predict and the gathering sort
 // pass (uniform weights) both form it here, so the two give the same bits.

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             cell = clamp_cell(o.x, gs) + gs * clamp_cell(o.y, gs);
             if (aos)
             {
-                reinterpret_cast<float4*>(d.cdf)[i] = o;
+                stream_store(reinterpret_cast<float4*>(d.cdf) + i, o);
             }
             else
             {
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(PRED_BLOCK) void k_predict(const VehDesc* __restric
             }
             if (!keep_w)
                 P.weight[i] = w;
-            P.idx[i] = cell;
+            stream_store(&P.idx[i], cell);
         }
         for (int ps = 0; ps < passes; ++ps)
             wave_count(s_h[ps], ((unsigned)cell >> (ps * RADIX_BITS)) & (RADIX - 1), valid);
@@ -578,9 +578,9 @@ __device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int z
     // the caller's device buffer (dogm.cu:205-215).
     const dogm_measurement_cell mc = d.meas_in ? d.meas_in[c] : d.meas[c];
     if (d.meas_in)
-        d.meas[c] = mc;
+        stream_store(&d.meas[c], mc);
     if ((zero_empty & 1) && occupied)
-        d.cellin[c] = make_float2(mc.likelihood, m_raw_c);
+        stream_store(&d.cellin[c], make_float2(mc.likelihood, m_raw_c));
     const float pB = d.pB;
     const float m_free_pred = fminf(d.alpha * fprev, one - m_occ_pred);
     const float unknown_pred = one - m_occ_pred - m_free_pred;
@@ -592,12 +592,12 @@ __device__ __forceinline__ float mass_update_cell(const VehDesc& d, int c, int z
         (m_free_pred * meas_unknown + unknown_pred * mc.free_mass + m_free_pred * mc.free_mass) / (one - K);
     const float rho_b = (occ_up * pB * (one - m_occ_pred)) / (m_occ_pred + pB * (one - m_occ_pred));
     const float rho_p = occ_up - rho_b;
-    d.born[c] = rho_b;
-    d.f[GF_PERS][c] = rho_p;
-    d.f[GF_NEW_BORN][c] = rho_b;
-    d.free[1][c] = free_up;
-    d.f[GF_OCC][c] = occ_up;
-    d.f[GF_PRED][c] = m_occ_pred;
+    stream_store(&d.born[c], rho_b);
+    stream_store(&d.f[GF_PERS][c], rho_p);
+    stream_store(&d.f[GF_NEW_BORN][c], rho_b);
+    stream_store(&d.free[1][c], free_up);
+    stream_store(&d.f[GF_OCC][c], occ_up);
+    stream_store(&d.f[GF_PRED][c], m_occ_pred);
     return rho_b;
 }
 
