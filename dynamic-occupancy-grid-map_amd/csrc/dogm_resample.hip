@@ -11,9 +11,9 @@
 //    the coarse index (two dependent loads instead of a 22-level binary search), the CDF entries of
 //    those segments are rebuilt in LDS from the weights (one wave per segment: exact wave scan +
 //    the index entry before the segment), and every draw finds its ancestor there (the reference's
-//    lower_bound).  The reference's resamplingKernel gather is deferred: in a full step only the
-//    ancestors are written, and the next prediction reads each state through its ancestor;
-//    otherwise (or when anything else reads the particles first) k_materialize copies them.
+//    lower_bound).  The reference's resamplingKernel gather is deferred: only the ancestors (and the
+//    new weight) are written, the next full step's prediction reads each state through its
+//    ancestor, and anything else that reads the particles first has k_materialize copy them.
 // No atomics on global memory and no global sort: the result is a pure function of the inputs.
 // Every vehicle of a step resamples from its own CDF with its own draws.
 #include "dogm_device.h"
